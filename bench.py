@@ -1,0 +1,249 @@
+"""Throughput benchmark of the exhaustive FLC scan on one MI355X.
+
+Workload (default): cryo-ET FLC with a wedge x CTF template filter and a
+continuous-wedge target filter; 512x512x200 float32 tomogram with 200
+embedded copies at SNR 0.1, 48^3 blob template, thresholded + dilated mask,
+uniform random rotations.  A step scores one batch of rotations over the
+whole padded tomogram and folds it into the MaxScoreOverRotations state
+(the public scoring function with the analyzer proxy, exactly as ``scan``
+runs it).  Setup (target FFTs, filters) is outside the timed region; inputs
+are resident in HBM.
+
+Prints one JSON line.  ``value`` is rotations/s of the whole job (one GPU).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--batch", type=int, default=None, help="rotations per step")
+    p.add_argument("--workload", default="flc_512x512x200_t48")
+    p.add_argument("--path", choices=("fused", "plain"), default="fused")
+    p.add_argument("--copies", type=int, default=None)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-rotations", type=int, default=None, help="rotations in the CPU sample")
+    p.add_argument("--compare-plain", action="store_true", help="also time the plain per-rotation path")
+    return p.parse_args()
+
+
+# ------------------------------------------------------------------ CPU leg
+_CPU_ORACLE = None
+
+
+def _cpu_worker(rotations):
+    t0 = time.perf_counter()
+    for r in rotations:
+        _CPU_ORACLE.raw_scores(r)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(target, template, mask, tfilt, gfilt, rotations, score, n_rot):
+    """The oracle's float32 dataflow (scipy.fft, one thread per process) over
+    a bounded rotation sample, one process per host core, forked after the
+    target setup (done once).  Runs before the GPU is touched."""
+    import multiprocessing as mp
+
+    from oracle.scan import ScanOracle
+
+    global _CPU_ORACLE
+    cores = len(os.sched_getaffinity(0))
+    cores = max(1, min(cores, 16))
+    n_rot = n_rot or cores
+    t0 = time.perf_counter()
+    _CPU_ORACLE = ScanOracle(score, target, template, mask, tfilt, pad_target=False, dtype=np.float32,
+                             workers=1, target_filter=gfilt)
+    setup_s = time.perf_counter() - t0
+    chunks = [rotations[i::cores] for i in range(cores)]
+    chunks = [c for c in chunks if len(c)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(len(chunks)) as pool:
+        pool.map(_cpu_worker, chunks)
+    wall = time.perf_counter() - t0
+    _CPU_ORACLE = None
+    return {
+        "value": n_rot / wall,
+        "unit": "rotations/s",
+        "cores": len(chunks),
+        "kind": "port",
+        "sample": f"{n_rot} rotations of the same workload, float32 numpy/scipy restatement of the "
+        f"reference numpy backend, 1 FFT thread per process; target setup {setup_s:.1f}s excluded",
+    }
+
+
+# ------------------------------------------------------------------ GPU leg
+def build_scan(workload, target, template, mask, filters, rotations, path, batch):
+    import torch
+
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+    from pytme_amd.backends import backend as be
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+
+    be.change_backend("hip", device="cuda", fused=(path == "fused"), rotation_batch=batch)
+    md = MatchingData(target, template, template_mask=mask, rotations=rotations)
+    md.template_filter, md.target_filter = filters
+    setup_fn, scoring = MATCHING_EXHAUSTIVE_REGISTER[workload["score"]]
+    setup, cb_args, chunks = prepare_scan(
+        md, setup_fn, n_jobs=1, callback_class_args={"score_threshold": 0.0}, pad_target=False,
+        pad_template_filter=False, interpolation_order=1,
+    )
+    proxy = SharedAnalyzerProxy(MaxScoreOverRotations, cb_args)
+    rot_dev = chunks[0]
+    torch.cuda.synchronize()
+    return scoring, setup, proxy, rot_dev
+
+
+def time_steps(scoring, setup, proxy, rot_dev, batch, steps, warmup, timing=False):
+    import torch
+
+    from pytme_amd import _lib
+
+    n = rot_dev.shape[0]
+    pos = 0
+
+    def step():
+        nonlocal pos
+        if pos + batch > n:
+            pos = 0
+        scoring(rotations=rot_dev[pos : pos + batch], callback=proxy, **setup)
+        pos += batch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    if timing:
+        _lib.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    report = None
+    if timing:
+        report = _lib.timing_report()
+        _lib.timing_enable(False)
+    return t1 - t0, report
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the scan shards by rotation "
+            "or by target split across GPUs, which this build does not launch.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    from pytme_amd.synthetic import WORKLOADS, make_workload
+
+    spec = WORKLOADS[args.workload]
+    target, template, mask, rotations, filters, spec = make_workload(args.workload, copies=args.copies)
+    batch = args.batch or (16 if args.path == "fused" else 4)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        tfilt = gfilt = None
+        if filters[0] is not None:
+            from pytme_amd.matching_data import MatchingData
+
+            conv, fast, ft, shift = MatchingData._fourier_padding(target.shape, template.shape)
+            tfilt = filters[0](shape=template.shape, return_real_fourier=True)["data"]
+            gfilt = filters[1](shape=fast, return_real_fourier=True)["data"]
+        cpu_rots = rotations[: args.cpu_rotations or min(16, len(os.sched_getaffinity(0)))]
+        cpu = cpu_baseline(target, template, mask, tfilt, gfilt, cpu_rots, spec["score"], len(cpu_rots))
+
+    import torch
+
+    scoring, setup, proxy, rot_dev = build_scan(spec, target, template, mask, filters, rotations, args.path, batch)
+    elapsed, report = time_steps(scoring, setup, proxy, rot_dev, batch, args.steps, args.warmup, timing=True)
+    n_rot = args.steps * batch
+    value = n_rot / elapsed
+
+    fast = tuple(int(x) for x in setup["fast_shape"])
+    F = int(np.prod(fast))
+    C = int(np.prod(fast[:-1])) * (fast[-1] // 2 + 1)
+    model_bytes = (112 * C + 16 * F / batch) if spec["score"] == "FLC" else (40 * C + 4 * F + 16 * F / batch)
+
+    dominant = max(report.items(), key=lambda kv: kv[1][0]) if report else None
+    roofline = None
+    if dominant:
+        name, (ms, count, by) = dominant
+        achieved = by / (ms * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm",
+            "kernel": name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launches": count,
+            "avg_launch_us": round(ms * 1e3 / max(count, 1), 2),
+            "bytes_per_launch": by / max(count, 1),
+        }
+    kernels = {k: {"ms": round(v[0], 3), "launches": v[1], "GB/s": round(v[2] / max(v[0], 1e-9) / 1e6, 1)}
+               for k, v in sorted(report.items(), key=lambda kv: -kv[1][0])} if report else {}
+
+    extra = {}
+    if args.compare_plain and args.path == "fused":
+        from pytme_amd.backends import backend as be
+
+        be.change_backend("hip", device="cuda", fused=False)
+        p_steps = max(2, args.steps // 4)
+        p_el, _ = time_steps(scoring, setup, proxy, rot_dev, 4, p_steps, 1)
+        extra["plain_path_rotations_per_s"] = round(p_steps * 4 / p_el, 2)
+
+    out = {
+        "metric": "FLC rotations/s, 512x512x200 tomogram vs 48^3 template" if args.workload.startswith("flc_512")
+        else f"{spec['score']} rotations/s ({args.workload})",
+        "value": round(value, 3),
+        "unit": "rotations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "score": spec["score"],
+            "tomogram": list(spec["shape"]),
+            "template": spec["T"],
+            "fast_shape": list(fast),
+            "rotations_per_step": batch,
+            "path": args.path,
+            "filters": "wedge x CTF template filter, continuous-wedge target filter" if spec["filters"] else None,
+        },
+        "whole_step_model_GBps": round(model_bytes * value / 1e9, 1),
+        "whole_step_model_frac": round(model_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

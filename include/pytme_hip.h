@@ -1,0 +1,108 @@
+/*
+ * pytme_hip.h - C ABI of libpytme_hip.so, the gfx950 HIP kernels behind the
+ * `hip` matching backend of pytme_amd.
+ *
+ * Conventions
+ *   - Every function returns TM_OK (0) or an error status; tm_last_error()
+ *     gives the message (thread-local).  TM_ERR_VALUE = bad argument/shape,
+ *     TM_ERR_OOM = device allocation failure, TM_ERR_HIP = other HIP error.
+ *   - Array arguments are raw device pointers to C-ordered data: float32,
+ *     int32, double, or interleaved complex64 (passed as void*).
+ *   - `stream` is a hipStream_t (NULL = default stream); work is enqueued
+ *     asynchronously and ordered on that stream.
+ *   - No torch types appear in the ABI.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the pyTME repository).  The Python side binds these with ctypes in
+ * pytme_amd/_lib.py.
+ */
+#ifndef PYTME_HIP_H
+#define PYTME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TM_OK 0
+#define TM_ERR_VALUE 1
+#define TM_ERR_OOM 2
+#define TM_ERR_HIP 3
+
+typedef struct tm_fft_plan tm_fft_plan;
+
+/* ---- library ----------------------------------------------------------- */
+int tm_abi_version(void);
+const char* tm_last_error(void);
+/* Replaces MatchingBackend.device_count (tme/backends/matching_backend.py:1262). */
+int tm_device_count(int* n);
+/* Wait for `stream`; surfaces asynchronous kernel faults as TM_ERR_HIP. */
+int tm_stream_sync(void* stream);
+/* Optional per-kernel timing: while enabled, every library launch is
+ * bracketed by HIP events on its stream; query returns total device ms,
+ * launch count and algorithmic HBM bytes per kernel name.  No counterpart
+ * in the reference (it has no per-stage timing, see tme/matching_exhaustive.py). */
+int tm_timing_enable(int on);
+int tm_timing_names(char* buf, int len);
+int tm_timing_query(const char* name, double* total_ms, int64_t* count, double* bytes);
+
+/* ---- FFT: numpy.fft.rfftn / irfftn semantics ------------------------------
+ * Replaces MatchingBackend.build_fft / rfftn / irfftn
+ * (tme/backends/matching_backend.py:1089-1140; numpy backend pyFFTW plans,
+ * tme/backends/npfftw_backend.py:243-306).  `shape` is the real shape of one
+ * transform (rank 1..3, each axis <= 4096 with prime factors <= 13); `batch`
+ * transforms are contiguous.  Forward is unnormalised; the inverse applies
+ * 1/N and overwrites its complex input (pyFFTW overwrite_input=True). */
+int tm_fft_plan_create(int ndim, const int64_t* shape, int64_t batch, tm_fft_plan** plan);
+int tm_fft_plan_destroy(tm_fft_plan* plan);
+int tm_fft_r2c(const tm_fft_plan* plan, const float* in, void* out, void* stream);
+int tm_fft_c2r(const tm_fft_plan* plan, void* in, float* out, void* stream);
+/* Radix plan of a length (host only, for inspection). */
+int tm_fft_factor(int n, int* radices, int max_out, int* n_out);
+
+/* ---- rotation ---------------------------------------------------------------
+ * Replaces NumpyFFTWBackend.rigid_transform / _rigid_transform at spline
+ * order 1, mode="constant" (tme/backends/npfftw_backend.py:326-449;
+ * scipy.ndimage.affine_transform semantics).  Writes the shape[0..2] box of
+ * each output; optionally also rotates a mask and returns the FP64 sum of
+ * each rotated mask (the FLC n_obs, tme/matching_scores.py:743). */
+int64_t tm_rotate_workspace(const int* shape, int nbatch);
+int tm_rotate_trilinear(const float* src, const float* src_mask, const int* shape, const float* rots_dev,
+                        const double* mats_dev, const double* center, const double* translation, int nbatch,
+                        float* dst, float* dst_mask, const int64_t* dst_strides, int64_t dst_batch_stride,
+                        double* n_obs_out, double* workspace, void* stream);
+
+/* ---- pointwise / reductions ---------------------------------------------- */
+int64_t tm_reduce_workspace(void);
+/* Replaces MatchingBackend.fill (matching_backend.py:629-640). */
+int tm_fill_f32(float* p, float value, int64_t n, void* stream);
+int tm_fill_i32(int* p, int value, int64_t n, void* stream);
+/* Spectral products of the scoring loop: be.multiply on complex spectra
+ * (tme/matching_scores.py:619, 747-753) and the Fourier filter multiply
+ * (tme/matching_scores.py:121-123).  Operand b / f repeats with period nb/nf. */
+int tm_cmul_cc(const void* a, const void* b, void* out, int64_t n, int64_t nb, void* stream);
+int tm_cmul_cr(const void* a, const float* f, void* out, int64_t n, int64_t nf, void* stream);
+/* be.add/subtract/multiply/divide on float32 (op 0..3); b == NULL uses scalar s
+ * (tme/matching_scores.py:622-623 numerator / inv_denominator). */
+int tm_binary_f32(int op, const float* a, const float* b, float s, float* out, int64_t n, int64_t nb,
+                  void* stream);
+/* Replaces NumpyFFTWBackend.norm_scores (tme/backends/npfftw_backend.py:537-587);
+ * arr == NULL means arr = 1 (flcSphericalMask_setup, tme/matching_scores.py:438). */
+int tm_norm_scores(const float* arr, const float* exp_sq, const float* sq_exp, const double* n_obs_dev,
+                   double n_obs_host, float eps, float* out, int64_t n, int64_t n_ex, void* stream);
+/* Replaces NumpyFFTWBackend.max_score_over_rotations (npfftw_backend.py:503-535). */
+int tm_max_update(const float* scores, float* max_scores, int* rotations, int rotation_index, int64_t n,
+                  void* stream);
+/* be.sum over a float32 array, accumulated deterministically in FP64
+ * (tme/matching_scores.py:411-413, 557, 743). */
+int tm_sum_f64(const float* x, int64_t n, double* out_dev, double* workspace, void* stream);
+/* Replaces normalize_template (tme/matching_utils.py:61-99): FP64 moments. */
+int tm_normalize_template(float* t, const float* m, int64_t n, const double* n_obs_dev, double n_obs_host,
+                          double* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYTME_HIP_H */

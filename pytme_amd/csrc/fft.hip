@@ -1,0 +1,301 @@
+// Batched real<->complex n-D FFTs (n <= 3) for C-ordered float32 volumes.
+//
+// Forward (numpy.fft.rfftn semantics, unnormalised):
+//   pass 0: R2C along the last (contiguous) axis, real rows -> n_last/2+1 bins
+//   pass k: C2C along axis ndim-1-k of the half spectrum, in place
+// Inverse (numpy.fft.irfftn semantics, 1/N normalisation folded into the
+// final pass; the input half spectrum is used as scratch and overwritten):
+//   C2C along axes 0..ndim-2 in place, then C2R along the last axis.
+// Each pass is one kernel: a workgroup loads a tile of lines into LDS with
+// coalesced global accesses, runs the whole mixed-radix transform in LDS
+// (fft_core.h) and writes the tile back, so every pass streams the volume
+// through HBM exactly once (read + write).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+#include "fft_core.h"
+#include "tm_common.h"
+
+#define TM_FFT_THREADS 256
+#define TM_FFT_MAX_LEN 4096
+
+// --------------------------------------------------------------------------
+// kernels
+
+// C2C along the middle axis of a [outer][n][inner] complex array, in place.
+// A block owns TI consecutive inner indices of one outer index.
+template <int DIR>
+__global__ void __launch_bounds__(TM_FFT_THREADS)
+k_c2c_strided(float2* __restrict__ data, LinePlan p, int64_t inner, int TI, int64_t tiles_per_outer,
+              float scale) {
+  extern __shared__ float2 lds[];
+  const int n = p.n;
+  // element-major tile: element e of line l at e*ti + l
+  const int64_t o = blockIdx.x / tiles_per_outer;
+  const int64_t i0 = (blockIdx.x - o * tiles_per_outer) * (int64_t)TI;
+  const int ti = (int)((inner - i0) < TI ? (inner - i0) : TI);
+  float2* A = lds;
+  float2* B = lds + n * ti;
+  float2* base = data + o * (int64_t)n * inner + i0;
+  const int tid = threadIdx.x;
+  for (int f = tid; f < n * ti; f += TM_FFT_THREADS) {
+    const int e = f / ti, l = f - e * ti;
+    A[f] = base[(int64_t)e * inner + l];
+  }
+  __syncthreads();
+  float2* R = lds_fft_lines<DIR, true>(A, B, ti, 1, ti, p, tid, TM_FFT_THREADS);
+  for (int f = tid; f < n * ti; f += TM_FFT_THREADS) {
+    const int e = f / ti, l = f - e * ti;
+    base[(int64_t)e * inner + l] = c_scale(R[f], scale);
+  }
+}
+
+// R2C along contiguous rows: in rows of n reals (stride n), out rows of
+// n/2+1 complex (stride n/2+1).
+__global__ void __launch_bounds__(TM_FFT_THREADS)
+k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, int64_t nrows, int RL) {
+  extern __shared__ float2 lds[];
+  const int n = p.n, nc = n / 2 + 1;
+  const int ld = n + 1;
+  float2* A = lds;
+  float2* B = lds + RL * ld;
+  const int64_t r0 = (int64_t)blockIdx.x * RL;
+  const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
+  const int tid = threadIdx.x;
+  const float* src = in + r0 * n;
+  for (int f = tid; f < n * rl; f += TM_FFT_THREADS) {
+    const int l = f / n, e = f - l * n;
+    A[l * ld + e] = make_float2(src[f], 0.f);
+  }
+  __syncthreads();
+  float2* R = lds_fft_lines<-1, false>(A, B, rl, ld, 1, p, tid, TM_FFT_THREADS);
+  float2* dst = out + r0 * nc;
+  for (int f = tid; f < nc * rl; f += TM_FFT_THREADS) {
+    const int l = f / nc, e = f - l * nc;
+    dst[f] = R[l * ld + e];
+  }
+}
+
+// C2R along contiguous rows of n/2+1 complex -> n reals, times scale.  The
+// missing half is the Hermitian mirror; taking the real part of the complex
+// inverse discards the imaginary parts of the DC and Nyquist bins exactly as
+// numpy.fft.irfft does.
+__global__ void __launch_bounds__(TM_FFT_THREADS)
+k_c2r_rows(const float2* __restrict__ in, float* __restrict__ out, LinePlan p, int64_t nrows, int RL,
+           float scale) {
+  extern __shared__ float2 lds[];
+  const int n = p.n, nc = n / 2 + 1;
+  const int ld = n + 1;
+  float2* A = lds;
+  float2* B = lds + RL * ld;
+  const int64_t r0 = (int64_t)blockIdx.x * RL;
+  const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
+  const int tid = threadIdx.x;
+  const float2* src = in + r0 * nc;
+  for (int f = tid; f < nc * rl; f += TM_FFT_THREADS) {
+    const int l = f / nc, e = f - l * nc;
+    const float2 v = src[f];
+    A[l * ld + e] = v;
+    if (e > 0 && n - e >= nc) A[l * ld + (n - e)] = c_conj(v);
+  }
+  __syncthreads();
+  float2* R = lds_fft_lines<1, false>(A, B, rl, ld, 1, p, tid, TM_FFT_THREADS);
+  float* dst = out + r0 * n;
+  for (int f = tid; f < n * rl; f += TM_FFT_THREADS) {
+    const int l = f / n, e = f - l * n;
+    dst[f] = R[l * ld + e].x * scale;
+  }
+}
+
+// --------------------------------------------------------------------------
+// host-side plans
+
+static void factor_length(int n, std::vector<int>& out) {
+  int p2 = 0;
+  while (n % 2 == 0) { n /= 2; ++p2; }
+  // radix-8 stages, finishing with 4 or 4*4 / 2 so no radix-2 stage is used
+  // unless the length is 2 * odd.
+  int n8 = p2 / 3, rem = p2 % 3;
+  if (rem == 1 && n8 > 0) { n8 -= 1; rem = 4; }
+  for (int i = 0; i < n8; ++i) out.push_back(8);
+  if (rem == 2) out.push_back(4);
+  else if (rem == 4) { out.push_back(4); out.push_back(4); }
+  else if (rem == 1) out.push_back(2);
+  const int odd[] = {5, 3, 7, 11, 13};
+  for (int r : odd)
+    while (n % r == 0) { out.push_back(r); n /= r; }
+  if (n != 1) out.push_back(-n);  // unsupported prime factor
+}
+
+struct tm_line {
+  LinePlan lp;
+  float2* tw_dev;
+};
+
+struct tm_fft_plan {
+  int ndim;
+  int64_t shape[3];   // real shape
+  int64_t cshape[3];  // half-spectrum shape
+  int64_t batch;
+  tm_line lines[3];
+};
+
+static int make_line(int n, tm_line* out) {
+  std::vector<int> rad;
+  factor_length(n, rad);
+  TM_REQUIRE(n >= 1 && n <= TM_FFT_MAX_LEN, "FFT length %d outside [1, %d]", n, TM_FFT_MAX_LEN);
+  TM_REQUIRE((int)rad.size() <= TM_MAX_STAGES, "FFT length %d needs too many stages", n);
+  for (int r : rad) TM_REQUIRE(r > 0, "FFT length %d has prime factor %d > %d", n, -r, TM_MAX_GENERIC_RADIX);
+  memset(out, 0, sizeof(*out));
+  out->lp.n = n;
+  out->lp.nst = (int)rad.size();
+  for (size_t i = 0; i < rad.size(); ++i) out->lp.radix[i] = rad[i];
+  std::vector<float2> tw(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)n;
+    tw[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  TM_CHECK_HIP(hipMalloc(&out->tw_dev, sizeof(float2) * n));
+  TM_CHECK_HIP(hipMemcpy(out->tw_dev, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  out->lp.tw = out->tw_dev;
+  return TM_OK;
+}
+
+static int tile_lines(int n, int64_t avail) {
+  int t = 4096 / n;
+  if (t > 16) t = 16;
+  if (t < 1) t = 1;
+  if (avail < t) t = (int)avail;
+  return t;
+}
+
+static int launch_c2c(const LinePlan& lp, float2* data, int64_t outer, int64_t inner, int dir, float scale,
+                      hipStream_t s) {
+  const int TI = tile_lines(lp.n, inner);
+  const int64_t tpo = tm_ceil_div(inner, TI);
+  const int64_t blocks = outer * tpo;
+  const size_t shm = 2 * (size_t)TI * lp.n * sizeof(float2);
+  const double bytes = 16.0 * (double)lp.n * (double)inner * (double)outer;
+  if (dir < 0)
+    TM_TIMED("fft_c2c_axis", bytes, s,
+             hipLaunchKernelGGL(k_c2c_strided<-1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, data,
+                                lp, inner, TI, tpo, scale));
+  else
+    TM_TIMED("fft_c2c_axis", bytes, s,
+             hipLaunchKernelGGL(k_c2c_strided<1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, data,
+                                lp, inner, TI, tpo, scale));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+static int rows_per_block(int n, int64_t nrows) {
+  int r = 2048 / (n + 1);
+  if (r < 1) r = 1;
+  if (r > 64) r = 64;
+  if (nrows < r) r = (int)nrows;
+  return r;
+}
+
+extern "C" {
+
+int tm_fft_plan_create(int ndim, const int64_t* shape, int64_t batch, tm_fft_plan** out) {
+  TM_REQUIRE(out != nullptr, "null plan output");
+  TM_REQUIRE(ndim >= 1 && ndim <= 3, "FFT rank %d not in [1, 3]", ndim);
+  TM_REQUIRE(batch >= 1, "FFT batch must be >= 1");
+  tm_fft_plan* p = (tm_fft_plan*)calloc(1, sizeof(tm_fft_plan));
+  TM_REQUIRE(p != nullptr, "host allocation failed");
+  p->ndim = ndim;
+  p->batch = batch;
+  for (int i = 0; i < ndim; ++i) {
+    if (shape[i] < 1 || shape[i] > TM_FFT_MAX_LEN) {
+      free(p);
+      tm_set_error("FFT axis length %lld outside [1, %d]", (long long)shape[i], TM_FFT_MAX_LEN);
+      return TM_ERR_VALUE;
+    }
+    p->shape[i] = shape[i];
+    p->cshape[i] = shape[i];
+  }
+  p->cshape[ndim - 1] = shape[ndim - 1] / 2 + 1;
+  for (int i = 0; i < ndim; ++i) {
+    int rc = make_line((int)shape[i], &p->lines[i]);
+    if (rc != TM_OK) {
+      for (int j = 0; j < i; ++j) hipFree(p->lines[j].tw_dev);
+      free(p);
+      return rc;
+    }
+  }
+  *out = p;
+  return TM_OK;
+}
+
+int tm_fft_plan_destroy(tm_fft_plan* p) {
+  if (!p) return TM_OK;
+  for (int i = 0; i < p->ndim; ++i)
+    if (p->lines[i].tw_dev) hipFree(p->lines[i].tw_dev);
+  free(p);
+  return TM_OK;
+}
+
+int tm_fft_r2c(const tm_fft_plan* p, const float* in, void* out_, void* stream) {
+  TM_REQUIRE(p && in && out_, "null argument to tm_fft_r2c");
+  hipStream_t s = (hipStream_t)stream;
+  float2* out = (float2*)out_;
+  const int nd = p->ndim;
+  const LinePlan& last = p->lines[nd - 1].lp;
+  int64_t nrows = p->batch;
+  for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
+  const int RL = rows_per_block(last.n, nrows);
+  const size_t shm = 2 * (size_t)RL * (last.n + 1) * sizeof(float2);
+  TM_TIMED("fft_r2c_rows", (4.0 * last.n + 8.0 * (last.n / 2 + 1)) * (double)nrows, s,
+           hipLaunchKernelGGL(k_r2c_rows, dim3((unsigned)tm_ceil_div(nrows, RL)), dim3(TM_FFT_THREADS), shm, s,
+                              in, out, last, nrows, RL));
+  TM_CHECK_LAUNCH();
+  for (int ax = nd - 2; ax >= 0; --ax) {
+    int64_t outer = p->batch, inner = 1;
+    for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
+    for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
+    int rc = launch_c2c(p->lines[ax].lp, out, outer, inner, -1, 1.0f, s);
+    if (rc != TM_OK) return rc;
+  }
+  return TM_OK;
+}
+
+int tm_fft_c2r(const tm_fft_plan* p, void* in_, float* out, void* stream) {
+  TM_REQUIRE(p && in_ && out, "null argument to tm_fft_c2r");
+  hipStream_t s = (hipStream_t)stream;
+  float2* in = (float2*)in_;
+  const int nd = p->ndim;
+  double N = 1.0;
+  for (int i = 0; i < nd; ++i) N *= (double)p->shape[i];
+  for (int ax = 0; ax <= nd - 2; ++ax) {
+    int64_t outer = p->batch, inner = 1;
+    for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
+    for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
+    int rc = launch_c2c(p->lines[ax].lp, in, outer, inner, 1, 1.0f, s);
+    if (rc != TM_OK) return rc;
+  }
+  const LinePlan& last = p->lines[nd - 1].lp;
+  int64_t nrows = p->batch;
+  for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
+  const int RL = rows_per_block(last.n, nrows);
+  const size_t shm = 2 * (size_t)RL * (last.n + 1) * sizeof(float2);
+  TM_TIMED("fft_c2r_rows", (4.0 * last.n + 8.0 * (last.n / 2 + 1)) * (double)nrows, s,
+           hipLaunchKernelGGL(k_c2r_rows, dim3((unsigned)tm_ceil_div(nrows, RL)), dim3(TM_FFT_THREADS), shm, s,
+                              in, out, last, nrows, RL, (float)(1.0 / N)));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+// Expose the radix plan for a length (host-only helper used by tests).
+int tm_fft_factor(int n, int* radices, int max_out, int* n_out) {
+  std::vector<int> rad;
+  factor_length(n, rad);
+  TM_REQUIRE((int)rad.size() <= max_out, "output too small");
+  for (size_t i = 0; i < rad.size(); ++i) radices[i] = rad[i];
+  *n_out = (int)rad.size();
+  return TM_OK;
+}
+
+}  // extern "C"

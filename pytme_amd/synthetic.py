@@ -1,0 +1,148 @@
+"""Seeded synthetic tomograms and templates for tests and benchmarks.
+
+The reference has no generator; these follow the workload definitions used
+for measurement: a template is a sum of isotropic Gaussian blobs, a
+tomogram is N(0, 1) noise with rotated template copies added at a given
+signal-to-noise ratio.
+"""
+
+from typing import Dict, Tuple
+
+import numpy as np
+from scipy.ndimage import affine_transform, binary_dilation
+
+from .matching_utils import elliptical_mask
+from .rotations import uniform_rotations
+
+__all__ = ["blob_template", "threshold_mask", "spherical_mask", "embed_copies", "make_workload"]
+
+
+def blob_template(size: int, n_blobs: int = 20, seed: int = 0) -> np.ndarray:
+    """Sum of ``n_blobs`` Gaussians (sigma U[1.5, 3] voxels) inside a ball of
+    radius 12*size/32 around the centre, scaled to maximum 1 (float32)."""
+    rng = np.random.default_rng(seed)
+    c0 = np.array([size // 2] * 3, dtype=np.float64)
+    radius = 12.0 * size / 32.0
+    grid = np.indices((size,) * 3, dtype=np.float64)
+    out = np.zeros((size,) * 3)
+    for _ in range(n_blobs):
+        while True:
+            p = rng.uniform(-radius, radius, 3)
+            if np.linalg.norm(p) <= radius:
+                break
+        sigma = rng.uniform(1.5, 3.0)
+        d2 = sum((grid[k] - (c0[k] + p[k])) ** 2 for k in range(3))
+        out += np.exp(-d2 / (2 * sigma * sigma))
+    return (out / out.max()).astype(np.float32)
+
+
+def threshold_mask(template: np.ndarray, fraction: float = 0.1, dilation: int = 2) -> np.ndarray:
+    """``template > fraction*max`` dilated ``dilation`` times (float32)."""
+    m = template > fraction * template.max()
+    if dilation:
+        m = binary_dilation(m, iterations=dilation)
+    return m.astype(np.float32)
+
+
+def spherical_mask(size: int, radius: float) -> np.ndarray:
+    return elliptical_mask(shape=(size,) * 3, radius=radius).astype(np.float32)
+
+
+def _rotate(arr: np.ndarray, rot: np.ndarray) -> np.ndarray:
+    c = (np.asarray(arr.shape) - 1) / 2
+    rinv = np.linalg.inv(rot.astype(np.float64))
+    return affine_transform(arr, rinv, offset=c - rinv @ c, order=1, mode="constant")
+
+
+def embed_copies(
+    shape: Tuple[int],
+    template: np.ndarray,
+    mask: np.ndarray,
+    n_copies: int,
+    snr: float,
+    seed: int,
+) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """N(0,1) noise plus ``n_copies`` rotated template copies.
+
+    Copies are scaled so that var(signal inside the mask) / var(noise) =
+    ``snr``; positions are at least T/2 from the border.  Returns the
+    target (float32), the copy centres (n, 3) and the rotations (n, 3, 3).
+    """
+    rng = np.random.default_rng(seed)
+    target = rng.standard_normal(shape, dtype=np.float32)
+    t = template.astype(np.float64)
+    sel = mask > 0
+    scale = np.sqrt(snr / max(t[sel].var(), 1e-12))
+    rots = uniform_rotations(n_copies, seed=seed + 17)
+    T = np.asarray(template.shape)
+    lo = T // 2
+    hi = np.asarray(shape) - (T - T // 2)
+    centres = np.stack([rng.integers(lo[k], hi[k] + 1, size=n_copies) for k in range(3)], axis=1)
+    for i in range(n_copies):
+        cp = (_rotate(t, rots[i]) * scale).astype(np.float32)
+        start = centres[i] - T // 2
+        box = tuple(slice(int(s), int(s) + int(n)) for s, n in zip(start, T))
+        target[box] += cp
+    return target, centres, rots
+
+
+WORKLOADS: Dict[str, Dict] = {
+    # FLCSphericalMask scan, 128^3 tomogram, 32^3 template, 1,000 rotations
+    "flcsph_128_t32": dict(
+        shape=(128, 128, 128), T=32, copies=5, snr=0.5, mask="sphere", radius=14, score="FLCSphericalMask",
+        rotations=1000, filters=False, seed=1000,
+    ),
+    # cryo-ET FLC with wedge + CTF, 512x512x200 tomogram, 48^3 template, 10,000 rotations
+    "flc_512x512x200_t48": dict(
+        shape=(512, 512, 200), T=48, copies=200, snr=0.1, mask="threshold", score="FLC", rotations=10000,
+        filters=True, tilt=(-60, 60, 3), defocus=30000.0, sampling=6.8, voltage=300e3, cs=2.7e7,
+        amplitude_contrast=0.07, seed=1001,
+    ),
+    # large single-template scan, 1024x1024x256 tomogram, 64^3 template
+    "flc_1024x1024x256_t64": dict(
+        shape=(1024, 1024, 256), T=64, copies=1000, snr=0.1, mask="threshold", score="FLC", rotations=30000,
+        filters=True, tilt=(-54, 54, 3), defocus=40000.0, sampling=5.4, voltage=300e3, cs=2.7e7,
+        amplitude_contrast=0.07, seed=1004,
+    ),
+}
+
+
+def make_workload(name: str, shape=None, copies=None, n_rotations=None):
+    """Build (target, template, mask, rotations, filters, spec) for a workload.
+
+    ``shape``/``copies``/``n_rotations`` override the defaults (tests use
+    reduced sizes).  ``filters`` is ``(template_filter, target_filter)`` as
+    :class:`~pytme_amd.filters.Compose` objects or ``(None, None)``.
+    """
+    spec = dict(WORKLOADS[name])
+    if shape is not None:
+        spec["shape"] = tuple(shape)
+    if copies is not None:
+        spec["copies"] = copies
+    if n_rotations is not None:
+        spec["rotations"] = n_rotations
+    T = spec["T"]
+    template = blob_template(T, seed=spec["seed"])
+    if spec["mask"] == "sphere":
+        mask = spherical_mask(T, spec["radius"])
+    else:
+        mask = threshold_mask(template)
+    target, centres, rots = embed_copies(spec["shape"], template, mask, spec["copies"], spec["snr"], spec["seed"])
+    rotations = uniform_rotations(spec["rotations"], seed=spec["seed"] + 1, include_identity=True)
+    filters = (None, None)
+    if spec["filters"]:
+        from .filters import Compose, CTFReconstructed, WedgeReconstructed
+
+        lo, hi, step = spec["tilt"]
+        angles = np.arange(lo, hi + step / 2, step, dtype=np.float64)
+        wedge = WedgeReconstructed(angles=angles, weight_wedge=False, create_continuous_wedge=False,
+                                   opening_axis=2, tilt_axis=0)
+        ctf = CTFReconstructed(defocus_x=spec["defocus"], sampling_rate=spec["sampling"],
+                               acceleration_voltage=spec["voltage"], spherical_aberration=spec["cs"],
+                               amplitude_contrast=spec["amplitude_contrast"], flip_phase=True)
+        target_wedge = WedgeReconstructed(angles=(lo, hi), create_continuous_wedge=True, weight_wedge=False,
+                                          opening_axis=2, tilt_axis=0)
+        filters = (Compose((wedge, ctf)), Compose((target_wedge,)))
+    spec["centres"] = centres
+    spec["copy_rotations"] = rots
+    return target, template, mask, rotations, filters, spec

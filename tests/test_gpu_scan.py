@@ -1,0 +1,120 @@
+"""End-to-end scan parity: pytme_amd hip backend vs the FP64 oracle."""
+
+import numpy as np
+import pytest
+from scipy.ndimage import laplace
+
+from oracle import scan as orc
+from tests._parity import assert_scan_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(score, target, template, mask, rotations, fused, threshold=0.0, template_filter=None,
+         target_filter=None, target_splits=None, pad=False, **backend_kwargs):
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.backends import backend as be
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, scan_subsets
+
+    be.change_backend("hip", device="cuda", fused=fused, **backend_kwargs)
+    md = MatchingData(target.copy(), template.copy(), template_mask=None if mask is None else mask.copy(),
+                      rotations=rotations)
+    if template_filter is not None:
+        md.template_filter = template_filter
+    if target_filter is not None:
+        md.target_filter = target_filter
+    setup, scoring = MATCHING_EXHAUSTIVE_REGISTER[score]
+    return scan_subsets(
+        matching_data=md, matching_setup=setup, matching_score=scoring,
+        callback_class=MaxScoreOverRotations, callback_class_args={"score_threshold": threshold},
+        target_splits=target_splits or {}, pad_target_edges=pad, job_schedule=(1, 1),
+        interpolation_order=1, pad_template_filter=False,
+    )
+
+
+class TestReferenceIntegrationCase:
+    """tests/test_matching_exhaustive.py:16-97 of the reference, on the GPU."""
+
+    @pytest.mark.parametrize("score", ["CC", "LCC", "FLCSphericalMask", "FLC"])
+    @pytest.mark.parametrize("pad_edge", [False, True])
+    @pytest.mark.parametrize("fused", [False, True])
+    def test_peak(self, score, pad_edge, fused):
+        target = np.zeros((80, 80, 80), np.float32)
+        target[25:31, 22:28, 12:16] = 1
+        template = np.zeros((41, 41, 35), np.float32)
+        template[20:26, 25:31, 17:21] = 1
+        mask = np.ones_like(template)
+        rot = np.eye(3, dtype=np.float32)[None]
+        print(f"integration case {score} pad={pad_edge} fused={fused}", flush=True)
+        s, off, r, mapping = _run(score, target, template, mask, rot, fused, target_splits={0: 2}, pad=pad_edge)
+        ref_s, ref_r = orc.scan_subsets(score, target, template, rot, template_mask=mask, target_splits={0: 2},
+                                        pad_target_edges=pad_edge)
+        peak = np.unravel_index(np.argmax(s), s.shape)
+        assert np.allclose(peak, [25, 17, 12])
+        expected = 1.0
+        if score == "CC":
+            expected = template.sum()
+        elif score == "LCC":
+            expected = (laplace(template) * laplace(template)).sum()
+        assert np.allclose(s[peak], expected, rtol=0.3)
+        scale = max(1.0, float(np.abs(ref_s).max()))
+        assert np.abs(s - ref_s).max() <= 1e-4 * scale
+        assert 0 in mapping and np.allclose(mapping[0], np.eye(3))
+
+
+def _workload(seed, shape=(40, 36, 30), T=12, n_rot=12, copies=3):
+    from pytme_amd.rotations import uniform_rotations
+    from pytme_amd.synthetic import blob_template, embed_copies, threshold_mask
+
+    template = blob_template(T, n_blobs=8, seed=seed)
+    mask = threshold_mask(template, dilation=1)
+    target, _, _ = embed_copies(shape, template, mask, copies, 0.5, seed)
+    rots = uniform_rotations(n_rot, seed=seed + 3, include_identity=True)
+    return target, template, mask, rots
+
+
+@pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask", "CC", "LCC"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_random_workload_parity(score, fused):
+    target, template, mask, rots = _workload(seed=11)
+    print(f"random workload {score} fused={fused}", flush=True)
+    threshold = -1.0 if score in ("FLC", "FLCSphericalMask") else -1e30
+    s, off, r, mapping = _run(score, target, template, mask, rots, fused, threshold=threshold)
+    _, _, raws, oc = orc.scan(score, target, template, rots, template_mask=mask, score_threshold=threshold,
+                              return_raw=True)
+    tol = 1e-4 if score in ("FLC", "FLCSphericalMask") else 1e-4 * max(1.0, max(np.abs(x).max() for x in raws))
+    n_diff, err = assert_scan_parity(s, r, raws, oc, threshold, tol=tol, tie_tol=max(2e-5, tol / 5))
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+    assert len(mapping) == rots.shape[0]
+
+
+@pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_filtered_parity(score, fused):
+    """Wedge x CTF template filter at template size plus a continuous-wedge
+    target filter (the cryo-ET configuration of scripts/match_template.py)."""
+    from pytme_amd.filters import Compose, CTFReconstructed, WedgeReconstructed
+    from pytme_amd.matching_data import MatchingData
+
+    target, template, mask, rots = _workload(seed=21, T=16)
+    if score == "FLCSphericalMask":
+        from pytme_amd.synthetic import spherical_mask
+
+        mask = spherical_mask(16, 7)
+    wedge = WedgeReconstructed(angles=np.arange(-60, 61, 3.0), opening_axis=2, tilt_axis=0)
+    ctf = CTFReconstructed(defocus_x=30000, sampling_rate=6.8, acceleration_voltage=300e3,
+                           spherical_aberration=2.7e7, amplitude_contrast=0.07, flip_phase=True)
+    comp = Compose((wedge, ctf))
+    tcomp = Compose((WedgeReconstructed(angles=(-60, 60), create_continuous_wedge=True, opening_axis=2,
+                                        tilt_axis=0),))
+    print(f"filtered {score} fused={fused}", flush=True)
+    s, off, r, _ = _run(score, target, template, mask, rots, fused, threshold=-1.0, template_filter=comp,
+                        target_filter=tcomp)
+    conv, fast, ft, shift = MatchingData._fourier_padding(target.shape, template.shape)
+    tfilt = comp(shape=template.shape, return_real_fourier=True)["data"]
+    gfilt = tcomp(shape=fast, return_real_fourier=True)["data"]
+    _, _, raws, oc = orc.scan(score, target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True, template_filter=tfilt, target_filter=gfilt)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)

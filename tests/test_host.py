@@ -1,0 +1,235 @@
+"""Host-side logic and the C-ABI surface (CPU only; no GPU calls)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.signal import correlate
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+class TestBackendManager:
+    def test_add_backend_rejects_non_backend(self):
+        from pytme_amd.backends import BackendManager
+
+        with pytest.raises(ValueError):
+            BackendManager().add_backend("bad", dict)
+
+    def test_unknown_backend(self):
+        from pytme_amd.backends import BackendManager
+
+        with pytest.raises(NotImplementedError):
+            BackendManager().change_backend("does-not-exist")
+
+    def test_hip_listed_and_constructible_on_cpu(self):
+        from pytme_amd.backends import BackendManager, HipBackend
+
+        m = BackendManager()
+        assert "hip" in m.available_backends()
+        m.add_backend("hip2", HipBackend)
+        m.change_backend("hip2", device="cpu")
+        assert m._backend_name == "hip2"
+        z = m.zeros((2, 3), m._float_dtype)
+        assert tuple(z.shape) == (2, 3)
+
+    def test_compute_refuses_cpu_device(self, cpu_backend):
+        be = cpu_backend
+        x = be.zeros((4, 4, 4), be._float_dtype)
+        with pytest.raises(RuntimeError):
+            be.rigid_transform(x, np.eye(3, dtype=np.float32), out=be.zeros((4, 4, 4), be._float_dtype), order=1)
+        with pytest.raises(RuntimeError):
+            be.rfftn(x)
+        with pytest.raises(RuntimeError):
+            be.norm_scores(x, x, x, 1.0, 1e-7, x)
+
+    def test_compute_convolution_shapes(self, cpu_backend):
+        conv, fast, ft = cpu_backend.compute_convolution_shapes((512, 512, 200), (1, 1, 1))
+        assert fast == [512, 512, 200] and ft == [512, 512, 101]
+        from scipy.fft import next_fast_len
+
+        from pytme_amd.backends.hip_backend import _next_fast_len
+
+        for n in range(1, 600):
+            assert _next_fast_len(n) == next_fast_len(n, real=True)
+
+
+class TestRegistry:
+    def test_register_and_duplicate(self):
+        from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, register_matching_exhaustive
+        from pytme_amd.memory import MATCHING_MEMORY_REGISTRY
+
+        key = "FLC"
+        setup, scoring = MATCHING_EXHAUSTIVE_REGISTER[key]
+        register_matching_exhaustive("TEST_FLC", setup, scoring, MATCHING_MEMORY_REGISTRY[key])
+        with pytest.raises(ValueError):
+            register_matching_exhaustive(key, setup, scoring, MATCHING_MEMORY_REGISTRY[key])
+        with pytest.raises(ValueError):
+            register_matching_exhaustive("TEST2", setup, scoring, dict)
+        del MATCHING_EXHAUSTIVE_REGISTER["TEST_FLC"]
+
+    def test_scores_registered(self):
+        from pytme_amd.matching_scores import MATCHING_EXHAUSTIVE_REGISTER
+
+        assert set(MATCHING_EXHAUSTIVE_REGISTER) >= {"CC", "LCC", "FLC", "FLCSphericalMask"}
+
+
+class TestMatchingData:
+    def test_fourier_padding(self, cpu_backend):
+        from oracle import scan as orc
+        from pytme_amd.matching_data import MatchingData
+
+        for tshape, Tshape in [((512, 512, 200), (48,) * 3), ((128,) * 3, (32,) * 3), ((80,) * 3, (41, 41, 35)),
+                               ((20, 30, 40), (7, 8, 9))]:
+            for pad in (False, True):
+                assert MatchingData._fourier_padding(tshape, Tshape, pad) == orc.fourier_padding(tshape, Tshape, pad)
+
+    def test_template_reversed(self, cpu_backend):
+        from pytme_amd.matching_data import MatchingData
+
+        t = np.arange(24, dtype=np.float32).reshape(2, 3, 4)
+        md = MatchingData(np.zeros((8, 8, 8), np.float32), t)
+        assert np.array_equal(md.template, t[::-1, ::-1, ::-1])
+        assert md.template_mask.shape == t.shape and np.all(md.template_mask == 1)
+
+    def test_subset_reflect_padding(self, cpu_backend):
+        from oracle import scan as orc
+        from pytme_amd.matching_data import MatchingData
+
+        a = np.random.default_rng(0).random((20, 18, 16))
+        md = MatchingData(a, np.ones((5, 5, 5)))
+        sl = (slice(0, 10), slice(4, 18), slice(8, 16))
+        got = md.subset_array(a, sl, np.array([4, 3, 6]))
+        assert np.array_equal(got, orc.subset_array(a, sl, np.array([4, 3, 6])))
+
+    def test_split_rotations(self, cpu_backend):
+        from pytme_amd.matching_data import MatchingData
+
+        md = MatchingData(np.zeros((8, 8, 8)), np.ones((3, 3, 3)), rotations=np.tile(np.eye(3), (10, 1, 1)))
+        parts = md._split_rotations_on_jobs(3)
+        assert [p.shape[0] for p in parts] == [3, 3, 4]
+
+
+class TestUtils:
+    def test_split_shape_matches_oracle(self):
+        from oracle import scan as orc
+        from pytme_amd.matching_utils import split_shape
+
+        for shape, splits in [((80, 80, 80), {0: 2}), ((17, 9, 30), {0: 3, 2: 4}), ((5, 5), {})]:
+            assert split_shape(shape, splits) == orc.split_shape(shape, splits)
+
+    @pytest.mark.parametrize("mode", ["full", "same", "valid"])
+    def test_convolution_mode_shapes(self, mode):
+        # reference tests/test_matching_utils.py:123-143: shapes equal scipy's
+        from pytme_amd.matching_utils import apply_convolution_mode
+
+        a, b = np.random.rand(12, 10, 9), np.random.rand(5, 4, 3)
+        full = correlate(a, b, mode="full")
+        out = apply_convolution_mode(full, mode, a.shape, b.shape)
+        assert out.shape == correlate(a, b, mode=mode).shape
+
+    def test_elliptical_mask(self):
+        from pytme_amd.matching_utils import elliptical_mask
+
+        m = elliptical_mask((32, 32, 32), radius=14)
+        assert m[16, 16, 16] == 1 and m[16, 16, 30] == 1 and m[16, 16, 31] == 0 and m[0, 0, 0] == 0
+
+
+class TestAnalyzerMerge:
+    @pytest.mark.parametrize("threshold", [0, 1e10, -1e10])
+    def test_merge_closed_form(self, threshold, cpu_backend):
+        # reference tests/test_analyzer.py:195-221 with host results
+        from pytme_amd.analyzer import MaxScoreOverRotations
+
+        data = np.random.default_rng(1).random((20, 20, 20)).astype(np.float32)
+        eye = np.eye(3, dtype=np.float32)
+        r1 = (np.maximum(data, threshold).astype(np.float32), np.zeros(3, int),
+              np.where(data > threshold, 0, -1).astype(np.int32), {0: eye})
+        r2 = (np.maximum(2 * data, threshold).astype(np.float32), np.zeros(3, int),
+              np.where(2 * data > threshold, 0, -1).astype(np.int32), {0: eye * 2})
+        s, off, rot, mapping = MaxScoreOverRotations.merge([r1, r2], score_threshold=threshold)
+        assert np.allclose(s, np.maximum(np.maximum(data, 2 * data), threshold))
+        assert len(mapping) == 2
+
+    def test_merge_offsets(self, cpu_backend):
+        from pytme_amd.analyzer import MaxScoreOverRotations
+
+        eye = np.eye(3, dtype=np.float32)
+        a = (np.ones((4, 4, 4), np.float32), np.array([0, 0, 0]), np.zeros((4, 4, 4), np.int32), {0: eye})
+        b = (2 * np.ones((4, 4, 4), np.float32), np.array([2, 0, 0]), np.zeros((4, 4, 4), np.int32), {0: eye})
+        s, off, rot, mapping = MaxScoreOverRotations.merge([a, b])
+        assert s.shape == (6, 4, 4) and np.all(s[:2] == 1) and np.all(s[2:] == 2)
+
+
+class TestFilters:
+    def test_electron_wavelength(self):
+        from pytme_amd.filters.ctf import electron_wavelength
+
+        assert abs(electron_wavelength(300e3) - 0.019687) < 1e-6
+
+    def test_ctf_zeros_and_dc(self):
+        from pytme_amd.filters.ctf import create_ctf, electron_wavelength
+
+        n, apix, dz, A = 256, 1.0, 20000.0, 0.07
+        ctf = create_ctf((n,), defocus_x=dz, sampling_rate=apix, spherical_aberration=0.0, amplitude_contrast=A)
+        # with Cs = 0: chi = pi*lam*dz*k^2 + atan(A/sqrt(1-A^2)) ; DC value = -A
+        assert np.isclose(ctf[n // 2], -np.sin(np.arctan(A / np.sqrt(1 - A * A))), atol=1e-6)
+        lam = electron_wavelength(300e3)
+        phi = np.arctan(A / np.sqrt(1 - A * A))
+        k1 = np.sqrt((np.pi - phi) / (np.pi * lam * dz))  # first zero of sin(chi)
+        k = (np.arange(n) - n // 2) / n
+        pos = ctf[n // 2:]
+        first = np.argmax(np.sign(pos[1:]) != np.sign(pos[:-1]))  # first sign change for k > 0
+        assert k[n // 2 + first] <= k1 <= k[n // 2 + first + 1]
+        assert np.all(np.abs(ctf) <= 1 + 1e-6)
+
+    def test_wedge_binary_symmetric_dc(self):
+        from pytme_amd.filters import WedgeReconstructed
+
+        w = WedgeReconstructed(angles=np.arange(-60, 61, 3.0), opening_axis=2, tilt_axis=0)
+        full = w(shape=(48, 48, 48), return_real_fourier=False)["data"]
+        assert set(np.unique(full)) <= {0.0, 1.0}
+        assert full[0, 0, 0] == 1
+        # Friedel symmetry of the binary support: W(-k) == W(k)
+        flipped = np.roll(full[::-1, ::-1, ::-1], 1, axis=(0, 1, 2))
+        assert np.mean(flipped == full) > 0.97
+        half = w(shape=(48, 48, 48), return_real_fourier=True)["data"]
+        assert half.shape == (48, 48, 25)
+
+    def test_continuous_wedge_angles(self):
+        from pytme_amd.filters import WedgeReconstructed
+
+        w = WedgeReconstructed(angles=(-60, 60), create_continuous_wedge=True, opening_axis=2, tilt_axis=0)
+        full = w(shape=(64, 64, 64), return_real_fourier=False)["data"]
+        # sampled where |k_tilt| >= tan(30 deg) |k_opening|: the whole tilt
+        # axis is sampled, the opening axis (beyond DC) lies in the missing wedge
+        c = np.fft.fftshift(full)
+        assert np.all(c[12:52, 32, 32] == 1)
+        assert np.all(c[32, 32, 33:50] == 0) and c[32, 32, 32] == 1
+        assert c[32 + 6, 32, 32 + 10] == 1 and c[32 + 5, 32, 32 + 10] == 0
+
+
+class TestCABI:
+    def test_library_exports_header(self):
+        from pytme_amd import _lib
+
+        lib = _lib.lib()
+        header = open(os.path.join(ROOT, "include", "pytme_hip.h")).read()
+        names = set(re.findall(r"^\s*(?:int64_t|int|const char\*)\s+(tm_\w+)\s*\(", header, re.M))
+        assert len(names) >= 20
+        for name in names:
+            assert hasattr(lib, name), name
+        assert set(_lib.SIGNATURES) >= names
+        assert lib.tm_abi_version() == 1
+
+    def test_fft_factor_host_only(self):
+        from pytme_amd import _lib
+
+        lib = _lib.lib()
+        r = (ctypes.c_int * 16)()
+        k = ctypes.c_int()
+        assert lib.tm_fft_factor(200, r, 16, ctypes.byref(k)) == 0
+        assert list(r)[: k.value] == [8, 5, 5]
+        assert int(np.prod(list(r)[: k.value])) == 200
