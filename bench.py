@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-rotations", type=int, default=None, help="rotations in the CPU sample")
     p.add_argument("--compare-plain", action="store_true", help="also time the plain per-rotation path")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run a 1 GiB device copy first (reference dispatch for HBM byte counters)")
     return p.parse_args()
 
 
@@ -171,6 +173,13 @@ def main():
         cpu = cpu_baseline(target, template, mask, tfilt, gfilt, cpu_rots, spec["score"], len(cpu_rots))
 
     import torch
+
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     scoring, setup, proxy, rot_dev = build_scan(spec, target, template, mask, filters, rotations, args.path, batch)
     elapsed, report = time_steps(scoring, setup, proxy, rot_dev, batch, args.steps, args.warmup, timing=True)

@@ -101,6 +101,29 @@ int tm_sum_f64(const float* x, int64_t n, double* out_dev, double* workspace, vo
 int tm_normalize_template(float* t, const float* m, int64_t n, const double* n_obs_dev, double n_obs_host,
                           double* workspace, void* stream);
 
+/* ---- fused batched scan ----------------------------------------------------
+ * Replaces the per-rotation loops of corr_scoring (tme/matching_scores.py:
+ * 600-624, mode 1: CC / LCC / FLCSphericalMask) and flc_scoring (:727-757,
+ * mode 0: FLC) together with MaxScoreOverRotations.__call__ ->
+ * max_score_over_rotations (tme/analyzer/aggregation.py:142-189,
+ * tme/backends/npfftw_backend.py:503-535) for a batch of rotations: rotate,
+ * filter, normalise, correlate through the FFT and fold into the max/argmax
+ * state without materialising per-rotation score volumes.  Inputs are the
+ * setup products of flc_setup / flcSphericalMask_setup / cc_setup
+ * (tme/matching_scores.py:153-189, 357-450): the reversed template and mask,
+ * the template-size filter (or NULL), the target spectra and, for
+ * FLCSphericalMask, the precomputed inverse denominator. */
+typedef struct tm_scan_plan tm_scan_plan;
+int64_t tm_scan_workspace_bytes(int mode, const int64_t* fast_shape, const int* template_shape, int max_batch,
+                                int with_filter);
+int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* template_shape, const float* templ,
+                        const float* mask, const float* filter, const void* ft_target, const void* ft_target2,
+                        const float* inv_denominator, int normalize, double mask_sum, int max_batch,
+                        tm_scan_plan** plan);
+int tm_scan_plan_destroy(tm_scan_plan* plan);
+int tm_scan_batch(tm_scan_plan* plan, const float* rotations, const int* rotation_indices, int nbatch,
+                  float* max_scores, int* max_rotations, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

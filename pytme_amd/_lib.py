@@ -62,6 +62,17 @@ SIGNATURES = {
     "tm_normalize_template": (
         c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double, c_void_p, c_void_p]
     ),
+    # fused batched scan
+    "tm_scan_workspace_bytes": (c_int64, [c_int, c_int64_p, c_int_p, c_int, c_int]),
+    "tm_scan_plan_create": (
+        c_int,
+        [
+            c_int, c_int64_p, c_int_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+            c_int, c_double, c_int, POINTER(c_void_p),
+        ],
+    ),
+    "tm_scan_plan_destroy": (c_int, [c_void_p]),
+    "tm_scan_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
 }
 
 _LIB = None
@@ -72,6 +83,11 @@ def lib():
     global _LIB
     if _LIB is not None:
         return _LIB
+    # torch's bundled HIP runtime must be the one the library binds to: load it
+    # first so the library's libamdhip64 dependency resolves to that copy
+    # instead of a second runtime instance.
+    import torch  # noqa: F401
+
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m pytme_amd._build` "

@@ -1,5 +1,160 @@
-"""Batched fused scan of the hip backend (placeholder until the fused kernels land)."""
+"""Batched fused scan of the hip backend.
+
+The registered scoring functions (``flc_scoring`` / ``corr_scoring``) offer
+their whole rotation set here first.  When the analyzer is an unshared
+:class:`~pytme_amd.analyzer.MaxScoreOverRotations` behind a
+:class:`~pytme_amd.analyzer.proxy.SharedAnalyzerProxy`, the rotations are
+scored in batches by ``tm_scan_batch`` (see ``csrc/fused.hip``), which folds
+them straight into the analyzer's max/argmax state: the per-rotation score
+volume is never materialised.  Rotation indices are assigned exactly as the
+analyzer's ``__call__`` would assign them (call order, or rotation bytes),
+so results, tie rule and the rotation mapping are those of the
+per-rotation loop.
+
+Anything else (other analyzers, interpolation orders, unusual filter or
+numerator shapes) returns ``NotImplemented`` and the caller runs the plain
+per-rotation loop on the same GPU kernels.
+"""
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from .. import _lib
+
+__all__ = ["run", "choose_batch"]
+
+_PLAN = {"key": None, "handle": None, "keep": None}
 
 
-def run(backend, **kwargs):
-    return NotImplemented
+def _free_plan():
+    if _PLAN["handle"] is not None:
+        _lib.call("tm_scan_plan_destroy", _PLAN["handle"])
+    _PLAN.update(key=None, handle=None, keep=None)
+
+
+def choose_batch(fast_shape, template_shape, mode: int, with_filter: bool, free_bytes: int,
+                 requested: Optional[int] = None, n_rotations: int = 1 << 30) -> int:
+    """Rotations per fused launch.
+
+    The per-rotation cost is dominated by the three HBM passes over the
+    NQ correlation spectra; larger batches amortise the resident target
+    spectra (read once per batch in pass A) and the max/argmax state
+    (read-modified-written once per batch in pass C).  The batch is the
+    largest of 16 that keeps the workspace under a quarter of free HBM.
+    """
+    if requested:
+        return int(max(1, min(requested, n_rotations)))
+    lib = _lib.lib()
+    fs = _lib.i64_array(fast_shape)
+    ts = _lib.i32_array(template_shape)
+    b = 16
+    while b > 1 and lib.tm_scan_workspace_bytes(mode, fs, ts, b, int(with_filter)) > free_bytes // 4:
+        b //= 2
+    return int(max(1, min(b, n_rotations)))
+
+
+def _proxy_state(callback):
+    from ..analyzer.aggregation import MaxScoreOverRotations
+    from ..analyzer.proxy import SharedAnalyzerProxy
+
+    if not isinstance(callback, SharedAnalyzerProxy):
+        return None
+    analyzer = callback._analyzer
+    if type(analyzer) is not MaxScoreOverRotations or callback._shared or analyzer._jax_mode:
+        return None
+    return analyzer, callback._state
+
+
+def run(backend, score: str, **kw):
+    """Fused scan of ``kw['rotations']`` into ``kw['callback']`` or NotImplemented."""
+    t = backend._torch
+    if int(kw.get("interpolation_order", 1)) != 1:
+        return NotImplemented
+    got = _proxy_state(kw.get("callback"))
+    if got is None:
+        return NotImplemented
+    analyzer, state = got
+    max_scores, max_rot, _ = state
+
+    template = backend.from_sharedarr(kw["template"])
+    ft_target = backend.from_sharedarr(kw["ft_target"])
+    tfilter = backend.from_sharedarr(kw["template_filter"])
+    fast_shape = tuple(int(x) for x in kw["fast_shape"])
+    rotations = kw["rotations"]
+    if template.ndim != 3 or len(fast_shape) != 3:
+        return NotImplemented
+    tshape = tuple(int(x) for x in template.shape)
+    if tuple(max_scores.shape) != fast_shape or not backend._dev_f32(max_scores):
+        return NotImplemented
+    if max_rot.dtype != t.int32 or not max_rot.is_contiguous():
+        return NotImplemented
+
+    filt = None
+    if backend.size(tfilter) != 1:
+        if tuple(tfilter.shape) != (tshape[0], tshape[1], tshape[2] // 2 + 1):
+            return NotImplemented
+        filt = tfilter.to(t.float32).contiguous()
+
+    mask = ft2 = inv_den = None
+    normalize, mask_sum = 0, 0.0
+    if score == "flc":
+        mode = 0
+        mask = backend.from_sharedarr(kw["template_mask"]).to(t.float32).contiguous()
+        ft2 = backend.from_sharedarr(kw["ft_target2"]).contiguous()
+        if tuple(mask.shape) != tshape:
+            return NotImplemented
+    elif score == "corr":
+        mode = 1
+        numerator = backend.from_sharedarr(kw["numerator"])
+        inv = backend.from_sharedarr(kw["inv_denominator"])
+        if tuple(numerator.shape) == fast_shape:
+            return NotImplemented  # F-shaped numerators are not produced by the registered scores
+        if tuple(inv.shape) == fast_shape:
+            inv_den = inv.to(t.float32).contiguous()
+        tm = kw.get("template_mask")
+        if tm is not None:
+            mask = backend.from_sharedarr(tm).to(t.float32).contiguous()
+            if tuple(mask.shape) != tshape:
+                return NotImplemented
+            normalize = 1
+            mask_sum = float(mask.double().sum())
+    else:
+        return NotImplemented
+
+    templ = template.to(t.float32).contiguous()
+    ft_target = ft_target.contiguous()
+    if rotations.ndim != 3 or rotations.shape[0] == 0:
+        return NotImplemented
+    rots = backend.to_backend_array(rotations).to(t.float32).contiguous()
+    n = int(rots.shape[0])
+
+    free = backend.get_available_memory()
+    B = choose_batch(fast_shape, tshape, mode, filt is not None, free, backend.rotation_batch, n)
+
+    keep = (templ, mask, filt, ft_target, ft2, inv_den)
+    key = (
+        t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,
+        tuple(0 if x is None else x.data_ptr() for x in keep),
+    )
+    if _PLAN["key"] != key:
+        _free_plan()
+        handle = ctypes.c_void_p()
+        ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
+        _lib.call(
+            "tm_scan_plan_create", mode, _lib.i64_array(fast_shape), _lib.i32_array(tshape), ptr(templ), ptr(mask),
+            ptr(filt), ptr(ft_target), ptr(ft2), ptr(inv_den), normalize, mask_sum, B, ctypes.byref(handle),
+        )
+        _PLAN.update(key=key, handle=handle.value, keep=keep)
+
+    idx = analyzer.rotation_indices(state, rots)
+    idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
+    stream = backend._stream()
+    for start in range(0, n, B):
+        nb = min(B, n - start)
+        _lib.call(
+            "tm_scan_batch", _PLAN["handle"], rots[start].data_ptr(), idx_dev[start].data_ptr(), nb,
+            max_scores.data_ptr(), max_rot.data_ptr(), stream,
+        )
+    return kw["callback"]

@@ -16,46 +16,49 @@
 #include <vector>
 
 #include "fft_core.h"
+#include "fft_plan.h"
 #include "tm_common.h"
 
-#define TM_FFT_THREADS 256
-#define TM_FFT_MAX_LEN 4096
 
 // --------------------------------------------------------------------------
 // kernels
 
-// C2C along the middle axis of a [outer][n][inner] complex array, in place.
-// A block owns TI consecutive inner indices of one outer index.
+// C2C along the middle axis: in is [outer][lin][inner], out is
+// [outer][n][inner] (lin <= n; the missing lin..n-1 inputs are zeros, the
+// pruned transform of a zero-padded line).  in == out (lin == n) is the
+// in-place case.  A block owns TI consecutive inner indices of one outer
+// index; the tile is element-major in LDS (element e of line l at e*ti + l)
+// so both the global accesses and the butterflies are conflict-free.
 template <int DIR>
 __global__ void __launch_bounds__(TM_FFT_THREADS)
-k_c2c_strided(float2* __restrict__ data, LinePlan p, int64_t inner, int TI, int64_t tiles_per_outer,
-              float scale) {
+k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner, int TI,
+              int64_t tiles_per_outer, float scale) {
   extern __shared__ float2 lds[];
   const int n = p.n;
-  // element-major tile: element e of line l at e*ti + l
   const int64_t o = blockIdx.x / tiles_per_outer;
   const int64_t i0 = (blockIdx.x - o * tiles_per_outer) * (int64_t)TI;
   const int ti = (int)((inner - i0) < TI ? (inner - i0) : TI);
   float2* A = lds;
   float2* B = lds + n * ti;
-  float2* base = data + o * (int64_t)n * inner + i0;
+  const float2* src = in + o * (int64_t)lin * inner + i0;
+  float2* dst = out + o * (int64_t)n * inner + i0;
   const int tid = threadIdx.x;
   for (int f = tid; f < n * ti; f += TM_FFT_THREADS) {
     const int e = f / ti, l = f - e * ti;
-    A[f] = base[(int64_t)e * inner + l];
+    A[f] = e < lin ? src[(int64_t)e * inner + l] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   float2* R = lds_fft_lines<DIR, true>(A, B, ti, 1, ti, p, tid, TM_FFT_THREADS);
   for (int f = tid; f < n * ti; f += TM_FFT_THREADS) {
     const int e = f / ti, l = f - e * ti;
-    base[(int64_t)e * inner + l] = c_scale(R[f], scale);
+    dst[(int64_t)e * inner + l] = c_scale(R[f], scale);
   }
 }
 
-// R2C along contiguous rows: in rows of n reals (stride n), out rows of
-// n/2+1 complex (stride n/2+1).
+// R2C along contiguous rows: in rows of lin <= n reals (stride lin,
+// zero-padded to n), out rows of n/2+1 complex (stride n/2+1).
 __global__ void __launch_bounds__(TM_FFT_THREADS)
-k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, int64_t nrows, int RL) {
+k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, int lin, int64_t nrows, int RL) {
   extern __shared__ float2 lds[];
   const int n = p.n, nc = n / 2 + 1;
   const int ld = n + 1;
@@ -64,10 +67,10 @@ k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, i
   const int64_t r0 = (int64_t)blockIdx.x * RL;
   const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
   const int tid = threadIdx.x;
-  const float* src = in + r0 * n;
+  const float* src = in + r0 * lin;
   for (int f = tid; f < n * rl; f += TM_FFT_THREADS) {
     const int l = f / n, e = f - l * n;
-    A[l * ld + e] = make_float2(src[f], 0.f);
+    A[l * ld + e] = make_float2(e < lin ? src[(int64_t)l * lin + e] : 0.f, 0.f);
   }
   __syncthreads();
   float2* R = lds_fft_lines<-1, false>(A, B, rl, ld, 1, p, tid, TM_FFT_THREADS);
@@ -129,18 +132,6 @@ static void factor_length(int n, std::vector<int>& out) {
   if (n != 1) out.push_back(-n);  // unsupported prime factor
 }
 
-struct tm_line {
-  LinePlan lp;
-  float2* tw_dev;
-};
-
-struct tm_fft_plan {
-  int ndim;
-  int64_t shape[3];   // real shape
-  int64_t cshape[3];  // half-spectrum shape
-  int64_t batch;
-  tm_line lines[3];
-};
 
 static int make_line(int n, tm_line* out) {
   std::vector<int> rad;
@@ -171,21 +162,22 @@ static int tile_lines(int n, int64_t avail) {
   return t;
 }
 
-static int launch_c2c(const LinePlan& lp, float2* data, int64_t outer, int64_t inner, int dir, float scale,
-                      hipStream_t s) {
+int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
+                  int dir, float scale, hipStream_t s, const char* name) {
   const int TI = tile_lines(lp.n, inner);
   const int64_t tpo = tm_ceil_div(inner, TI);
   const int64_t blocks = outer * tpo;
+  TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
   const size_t shm = 2 * (size_t)TI * lp.n * sizeof(float2);
-  const double bytes = 16.0 * (double)lp.n * (double)inner * (double)outer;
+  const double bytes = 8.0 * ((double)lin + (double)lp.n) * (double)inner * (double)outer;
   if (dir < 0)
-    TM_TIMED("fft_c2c_axis", bytes, s,
-             hipLaunchKernelGGL(k_c2c_strided<-1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, data,
-                                lp, inner, TI, tpo, scale));
+    TM_TIMED(name, bytes, s,
+             hipLaunchKernelGGL(k_c2c_strided<-1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out,
+                                lp, lin, inner, TI, tpo, scale));
   else
-    TM_TIMED("fft_c2c_axis", bytes, s,
-             hipLaunchKernelGGL(k_c2c_strided<1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, data,
-                                lp, inner, TI, tpo, scale));
+    TM_TIMED(name, bytes, s,
+             hipLaunchKernelGGL(k_c2c_strided<1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out,
+                                lp, lin, inner, TI, tpo, scale));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -196,6 +188,65 @@ static int rows_per_block(int n, int64_t nrows) {
   if (r > 64) r = 64;
   if (nrows < r) r = (int)nrows;
   return r;
+}
+
+int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, hipStream_t s,
+                       const char* name) {
+  const int RL = rows_per_block(lp.n, nrows);
+  const size_t shm = 2 * (size_t)RL * (lp.n + 1) * sizeof(float2);
+  const int64_t blocks = tm_ceil_div(nrows, RL);
+  TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
+  TM_TIMED(name, (4.0 * lin + 8.0 * (lp.n / 2 + 1)) * (double)nrows, s,
+           hipLaunchKernelGGL(k_r2c_rows, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out, lp, lin,
+                              nrows, RL));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+int tm_launch_c2r_rows(const LinePlan& lp, const float2* in, float* out, int64_t nrows, float scale, hipStream_t s,
+                       const char* name) {
+  const int RL = rows_per_block(lp.n, nrows);
+  const size_t shm = 2 * (size_t)RL * (lp.n + 1) * sizeof(float2);
+  const int64_t blocks = tm_ceil_div(nrows, RL);
+  TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
+  TM_TIMED(name, (4.0 * lp.n + 8.0 * (lp.n / 2 + 1)) * (double)nrows, s,
+           hipLaunchKernelGGL(k_c2r_rows, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out, lp, nrows,
+                              RL, scale));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+int tm_fft_r2c_batch(const tm_fft_plan* p, int64_t batch, const float* in, float2* out, hipStream_t s) {
+  const int nd = p->ndim;
+  int64_t nrows = batch;
+  for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
+  const LinePlan& last = p->lines[nd - 1].lp;
+  int rc = tm_launch_r2c_rows(last, in, last.n, out, nrows, s, "fft_r2c_rows");
+  if (rc != TM_OK) return rc;
+  for (int ax = nd - 2; ax >= 0; --ax) {
+    int64_t outer = batch, inner = 1;
+    for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
+    for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
+    rc = tm_launch_c2c(p->lines[ax].lp, out, p->lines[ax].lp.n, out, outer, inner, -1, 1.0f, s, "fft_c2c_axis");
+    if (rc != TM_OK) return rc;
+  }
+  return TM_OK;
+}
+
+int tm_fft_c2r_batch(const tm_fft_plan* p, int64_t batch, float2* in, float* out, hipStream_t s) {
+  const int nd = p->ndim;
+  double N = 1.0;
+  for (int i = 0; i < nd; ++i) N *= (double)p->shape[i];
+  for (int ax = 0; ax <= nd - 2; ++ax) {
+    int64_t outer = batch, inner = 1;
+    for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
+    for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
+    int rc = tm_launch_c2c(p->lines[ax].lp, in, p->lines[ax].lp.n, in, outer, inner, 1, 1.0f, s, "fft_c2c_axis");
+    if (rc != TM_OK) return rc;
+  }
+  int64_t nrows = batch;
+  for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
+  return tm_launch_c2r_rows(p->lines[nd - 1].lp, in, out, nrows, (float)(1.0 / N), s, "fft_c2r_rows");
 }
 
 extern "C" {
@@ -238,54 +289,14 @@ int tm_fft_plan_destroy(tm_fft_plan* p) {
   return TM_OK;
 }
 
-int tm_fft_r2c(const tm_fft_plan* p, const float* in, void* out_, void* stream) {
-  TM_REQUIRE(p && in && out_, "null argument to tm_fft_r2c");
-  hipStream_t s = (hipStream_t)stream;
-  float2* out = (float2*)out_;
-  const int nd = p->ndim;
-  const LinePlan& last = p->lines[nd - 1].lp;
-  int64_t nrows = p->batch;
-  for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
-  const int RL = rows_per_block(last.n, nrows);
-  const size_t shm = 2 * (size_t)RL * (last.n + 1) * sizeof(float2);
-  TM_TIMED("fft_r2c_rows", (4.0 * last.n + 8.0 * (last.n / 2 + 1)) * (double)nrows, s,
-           hipLaunchKernelGGL(k_r2c_rows, dim3((unsigned)tm_ceil_div(nrows, RL)), dim3(TM_FFT_THREADS), shm, s,
-                              in, out, last, nrows, RL));
-  TM_CHECK_LAUNCH();
-  for (int ax = nd - 2; ax >= 0; --ax) {
-    int64_t outer = p->batch, inner = 1;
-    for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
-    for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
-    int rc = launch_c2c(p->lines[ax].lp, out, outer, inner, -1, 1.0f, s);
-    if (rc != TM_OK) return rc;
-  }
-  return TM_OK;
+int tm_fft_r2c(const tm_fft_plan* p, const float* in, void* out, void* stream) {
+  TM_REQUIRE(p && in && out, "null argument to tm_fft_r2c");
+  return tm_fft_r2c_batch(p, p->batch, in, (float2*)out, (hipStream_t)stream);
 }
 
-int tm_fft_c2r(const tm_fft_plan* p, void* in_, float* out, void* stream) {
-  TM_REQUIRE(p && in_ && out, "null argument to tm_fft_c2r");
-  hipStream_t s = (hipStream_t)stream;
-  float2* in = (float2*)in_;
-  const int nd = p->ndim;
-  double N = 1.0;
-  for (int i = 0; i < nd; ++i) N *= (double)p->shape[i];
-  for (int ax = 0; ax <= nd - 2; ++ax) {
-    int64_t outer = p->batch, inner = 1;
-    for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
-    for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
-    int rc = launch_c2c(p->lines[ax].lp, in, outer, inner, 1, 1.0f, s);
-    if (rc != TM_OK) return rc;
-  }
-  const LinePlan& last = p->lines[nd - 1].lp;
-  int64_t nrows = p->batch;
-  for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
-  const int RL = rows_per_block(last.n, nrows);
-  const size_t shm = 2 * (size_t)RL * (last.n + 1) * sizeof(float2);
-  TM_TIMED("fft_c2r_rows", (4.0 * last.n + 8.0 * (last.n / 2 + 1)) * (double)nrows, s,
-           hipLaunchKernelGGL(k_c2r_rows, dim3((unsigned)tm_ceil_div(nrows, RL)), dim3(TM_FFT_THREADS), shm, s,
-                              in, out, last, nrows, RL, (float)(1.0 / N)));
-  TM_CHECK_LAUNCH();
-  return TM_OK;
+int tm_fft_c2r(const tm_fft_plan* p, void* in, float* out, void* stream) {
+  TM_REQUIRE(p && in && out, "null argument to tm_fft_c2r");
+  return tm_fft_c2r_batch(p, p->batch, (float2*)in, out, (hipStream_t)stream);
 }
 
 // Expose the radix plan for a length (host-only helper used by tests).

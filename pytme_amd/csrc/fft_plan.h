@@ -1,0 +1,35 @@
+// FFT plan structure and pass launchers shared by the FFT entry points and
+// the fused scan kernels.
+#pragma once
+#include "fft_core.h"
+#include "tm_common.h"
+
+#define TM_FFT_THREADS 256
+#define TM_FFT_MAX_LEN 4096
+
+struct tm_line {
+  LinePlan lp;
+  float2* tw_dev;
+};
+
+struct tm_fft_plan {
+  int ndim;
+  int64_t shape[3];   // real shape
+  int64_t cshape[3];  // half-spectrum shape
+  int64_t batch;
+  tm_line lines[3];
+};
+
+// C2C along the middle axis of [outer][lin][inner] -> [outer][n][inner]
+// (lin <= n zero-padded; dir -1 forward, +1 inverse unnormalised).
+int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
+                  int dir, float scale, hipStream_t s, const char* name);
+// R2C of nrows contiguous rows of lin <= n reals (zero-padded to n).
+int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, hipStream_t s,
+                       const char* name);
+// C2R of nrows contiguous half-spectrum rows, times scale.
+int tm_launch_c2r_rows(const LinePlan& lp, const float2* in, float* out, int64_t nrows, float scale, hipStream_t s,
+                       const char* name);
+// Whole n-D transforms of `batch` contiguous arrays with plan p.
+int tm_fft_r2c_batch(const tm_fft_plan* p, int64_t batch, const float* in, float2* out, hipStream_t s);
+int tm_fft_c2r_batch(const tm_fft_plan* p, int64_t batch, float2* in, float* out, hipStream_t s);

@@ -1,0 +1,542 @@
+// Batched fused exhaustive scan: one call scores a batch of B rotations over
+// the whole padded tomogram and folds them into the max/argmax state.
+//
+// Data layout (C order, complex64 interleaved), fast shape (N0, N1, N2),
+// N2c = N2/2 + 1, template box (T0, T1, T2):
+//   TR[b][T0][T1][T2], MR[b][...]  rotated template / mask (float32)
+//   Q1[q][b][T0][T1][N2c]          x-transforms of the template boxes
+//   P [q][b][T0][N1][N2c]          x,y-transforms (only T0 z-planes exist)
+//   Q [o][b][N0][N1][N2c]          correlation spectra after the z pass
+// with q in {mask, template} (FLC) or {template} (CORR) and o in
+// {S1, S2, X} (FLC) or {X} (CORR).
+//
+// Per batch:
+//   rotate      TR, MR (+ FP64 sum of each rotated mask)       T^3 work
+//   filter      batched T^3 rfft * filter * irfft (optional)   T^3 work
+//   normalize   FP64 masked moments, t = (t - mean)/std * m    T^3 work
+//   x pass      R2C of the T0*T1 template rows, zero-padded    small
+//   y pass      C2C of T1 -> N1 zero-padded lines              ~T0/N0 of a pass
+//   pass A      per (ky, kx) column: forward z-FFT of the pruned template
+//               columns, multiply with the resident target spectra,
+//               inverse z-FFT of every product -> Q           reads spectra once per batch
+//   pass B      inverse y-FFT of Q in place                    one HBM pass
+//   pass C      per (z, y) row: C2R of the NQ rows, FLC normalisation (or
+//               the FLCSphericalMask denominator), max/argmax over the
+//               batch in registers, one read-modify-write of the state
+// The target spectra are read once per batch instead of once per rotation
+// and the score volume never exists in HBM.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fft_core.h"
+#include "fft_plan.h"
+#include "tm_common.h"
+
+#define TM_FU_THREADS 256
+#define TM_FU_MAXP 16  // positions per thread: tile elements <= 4096 = 16 * 256
+#define TM_MOM_BLOCKS 32
+
+int tm_rotate_launch(const float* src, const float* srcm, const int* shape, const float* rots, int nbatch,
+                     float* dst, float* dstm, int64_t ds0, int64_t ds1, int64_t dbatch, double* n_obs,
+                     double* workspace, hipStream_t s);
+extern "C" int tm_cmul_cr(const void* a, const float* f, void* out, int64_t n, int64_t nf, void* stream);
+
+// --------------------------------------------------------------------------
+// batched normalize_template (tme/matching_utils.py:61-99)
+
+// partials[b][blk][2] = (sum t*m, sum t*t*m) over a fixed split of the box
+__global__ void __launch_bounds__(TM_FU_THREADS)
+k_moments_batched(const float* __restrict__ t, const float* __restrict__ m, int64_t n, int64_t m_bstride,
+                  double* __restrict__ partials) {
+  __shared__ double sh[2][TM_FU_THREADS];
+  const int b = blockIdx.y;
+  const float* tb = t + (int64_t)b * n;
+  const float* mb = m + (int64_t)b * m_bstride;
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * TM_FU_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * TM_FU_THREADS) {
+    const double a = tb[i], w = mb[i];
+    s0 += a * w;
+    s1 += a * a * w;
+  }
+  const int tid = threadIdx.x;
+  sh[0][tid] = s0;
+  sh[1][tid] = s1;
+  __syncthreads();
+  for (int s = TM_FU_THREADS / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      sh[0][tid] += sh[0][tid + s];
+      sh[1][tid] += sh[1][tid + s];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double* p = partials + 2 * ((int64_t)b * gridDim.x + blockIdx.x);
+    p[0] = sh[0][0];
+    p[1] = sh[1][0];
+  }
+}
+
+// ms[b] = (mean, std) with n = n_dev[b] (per rotation) or n_host
+__global__ void k_moments_finalize(const double* __restrict__ partials, int nparts, const double* __restrict__ n_dev,
+                                   double n_host, int nbatch, double* __restrict__ ms) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatch) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = 0; i < nparts; ++i) {
+    s0 += partials[2 * ((int64_t)b * nparts + i)];
+    s1 += partials[2 * ((int64_t)b * nparts + i) + 1];
+  }
+  const double nn = n_dev ? n_dev[b] : n_host;
+  const double mean = s0 / nn;
+  double var = s1 / nn - mean * mean;
+  var = var > 0.0 ? var : 0.0;
+  ms[2 * b] = mean;
+  ms[2 * b + 1] = sqrt(var);
+}
+
+__global__ void k_normalize_batched(float* __restrict__ t, const float* __restrict__ m, int64_t n, int64_t m_bstride,
+                                    const double* __restrict__ ms) {
+  const int b = blockIdx.y;
+  const float mean = (float)ms[2 * b];
+  const float sd = (float)ms[2 * b + 1];
+  float* tb = t + (int64_t)b * n;
+  const float* mb = m + (int64_t)b * m_bstride;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    tb[i] = (tb[i] - mean) / sd * mb[i];
+}
+
+// --------------------------------------------------------------------------
+// pass A: z-axis forward (pruned) x target spectra x z-axis inverse
+
+template <int NQ>
+__global__ void __launch_bounds__(TM_FU_THREADS)
+k_pass_a(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
+         float2* __restrict__ Q, LinePlan lp, int T0, int N1, int N2c, int B, int TI, int tiles_per_row) {
+  extern __shared__ float2 lds[];
+  const int N0 = lp.n;
+  const int tid = threadIdx.x;
+  const int ky = blockIdx.x / tiles_per_row;
+  const int kx0 = (blockIdx.x - ky * tiles_per_row) * TI;
+  const int ti = (N2c - kx0) < TI ? (N2c - kx0) : TI;
+  const int npos = N0 * ti;
+  const int64_t plane = (int64_t)N1 * N2c;
+  const int64_t col = (int64_t)ky * N2c + kx0;
+  float2* A = lds;
+  float2* Bf = lds + npos;
+
+  float2 F[TM_FU_MAXP], F2[TM_FU_MAXP], M[TM_FU_MAXP];
+#pragma unroll
+  for (int i = 0; i < TM_FU_MAXP; ++i) {
+    const int p = tid + i * TM_FU_THREADS;
+    F[i] = F2[i] = M[i] = make_float2(0.f, 0.f);
+    if (p < npos) {
+      const int z = p / ti, l = p - z * ti;
+      F[i] = Ft[z * plane + col + l];
+      if (NQ == 3) F2[i] = Ft2[z * plane + col + l];
+    }
+  }
+
+  for (int b = 0; b < B; ++b) {
+    if (NQ == 3) {
+      // mask spectrum along z (inputs beyond T0 are zero)
+      const float2* src = P + ((int64_t)(0 * B + b) * T0) * plane + col;
+      for (int p = tid; p < npos; p += TM_FU_THREADS) {
+        const int z = p / ti, l = p - z * ti;
+        A[p] = z < T0 ? src[z * plane + l] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+      float2* R = lds_fft_lines<-1, true>(A, Bf, ti, 1, ti, lp, tid, TM_FU_THREADS);
+#pragma unroll
+      for (int i = 0; i < TM_FU_MAXP; ++i) {
+        const int p = tid + i * TM_FU_THREADS;
+        if (p < npos) M[i] = R[p];
+      }
+      __syncthreads();
+    }
+    {
+      // template spectrum, X = F * t
+      const float2* src = P + ((int64_t)((NQ == 3 ? 1 : 0) * B + b) * T0) * plane + col;
+      for (int p = tid; p < npos; p += TM_FU_THREADS) {
+        const int z = p / ti, l = p - z * ti;
+        A[p] = z < T0 ? src[z * plane + l] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+      float2* R = lds_fft_lines<-1, true>(A, Bf, ti, 1, ti, lp, tid, TM_FU_THREADS);
+#pragma unroll
+      for (int i = 0; i < TM_FU_MAXP; ++i) {
+        const int p = tid + i * TM_FU_THREADS;
+        if (p < npos) R[p] = c_mul(F[i], R[p]);
+      }
+      __syncthreads();
+      float2* O = lds_fft_lines<1, true>(R, R == A ? Bf : A, ti, 1, ti, lp, tid, TM_FU_THREADS);
+      float2* dst = Q + ((int64_t)((NQ - 1) * B + b) * N0) * plane + col;
+      for (int p = tid; p < npos; p += TM_FU_THREADS) {
+        const int z = p / ti, l = p - z * ti;
+        dst[z * plane + l] = O[p];
+      }
+      __syncthreads();
+    }
+    if (NQ == 3) {
+#pragma unroll 1
+      for (int o = 0; o < 2; ++o) {
+#pragma unroll
+        for (int i = 0; i < TM_FU_MAXP; ++i) {
+          const int p = tid + i * TM_FU_THREADS;
+          if (p < npos) A[p] = c_mul(o == 0 ? F[i] : F2[i], M[i]);
+        }
+        __syncthreads();
+        float2* O = lds_fft_lines<1, true>(A, Bf, ti, 1, ti, lp, tid, TM_FU_THREADS);
+        float2* dst = Q + ((int64_t)(o * B + b) * N0) * plane + col;
+        for (int p = tid; p < npos; p += TM_FU_THREADS) {
+          const int z = p / ti, l = p - z * ti;
+          dst[z * plane + l] = O[p];
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// pass C: C2R along x, score, max/argmax over the batch, state update
+
+__device__ __forceinline__ float fu_flc_norm(float x, float ex2, float ex, float n, float eps) {
+#pragma clang fp contract(off)
+  float m = ex / n;
+  m = m * m;
+  float v = ex2 / n - m;
+  v = fmaxf(v, 0.f);
+  float sd = sqrtf(v);
+  sd = sd < eps ? 1.f : sd;
+  return x / (sd * n);
+}
+
+template <int NQ>
+__global__ void __launch_bounds__(TM_FU_THREADS)
+k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int RL, int B, const double* __restrict__ n_obs,
+         const float* __restrict__ inv_den, const int* __restrict__ rot_idx, float scale, float eps,
+         float* __restrict__ max_scores, int* __restrict__ max_rot) {
+  extern __shared__ float2 lds[];
+  const int N2 = lp.n, N2c = N2 / 2 + 1;
+  const int ld = N2 + 1;
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * RL;
+  const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
+  const int npos = rl * N2;
+  float2* A = lds;
+  float2* Bf = lds + RL * ld;
+
+  float best[TM_FU_MAXP], v1[TM_FU_MAXP], v2[TM_FU_MAXP];
+  int arg[TM_FU_MAXP];
+#pragma unroll
+  for (int i = 0; i < TM_FU_MAXP; ++i) {
+    best[i] = -INFINITY;
+    arg[i] = -1;
+    v1[i] = v2[i] = 0.f;
+  }
+  for (int b = 0; b < B; ++b) {
+#pragma unroll 1
+    for (int o = 0; o < NQ; ++o) {
+      const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * N2c;
+      for (int f = tid; f < N2c * rl; f += TM_FU_THREADS) {
+        const int l = f / N2c, e = f - l * N2c;
+        const float2 v = src[f];
+        A[l * ld + e] = v;
+        if (e > 0 && N2 - e >= N2c) A[l * ld + (N2 - e)] = c_conj(v);
+      }
+      __syncthreads();
+      float2* R = lds_fft_lines<1, false>(A, Bf, rl, ld, 1, lp, tid, TM_FU_THREADS);
+#pragma unroll
+      for (int i = 0; i < TM_FU_MAXP; ++i) {
+        const int p = tid + i * TM_FU_THREADS;
+        if (p < npos) {
+          const int l = p / N2, e = p - l * N2;
+          const float x = R[l * ld + e].x * scale;
+          if (o == NQ - 1) {
+            float s;
+            if (NQ == 3) {
+              s = fu_flc_norm(x, v2[i], v1[i], (float)n_obs[b], eps);
+            } else {
+              s = inv_den ? x * inv_den[(r0 + l) * N2 + e] : x;
+            }
+            if (s > best[i]) {
+              best[i] = s;
+              arg[i] = rot_idx[b];
+            }
+          } else if (o == 0) {
+            v1[i] = x;
+          } else {
+            v2[i] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TM_FU_MAXP; ++i) {
+    const int p = tid + i * TM_FU_THREADS;
+    if (p < npos) {
+      const int l = p / N2, e = p - l * N2;
+      const int64_t g = (r0 + l) * N2 + e;
+      if (best[i] > max_scores[g]) {
+        max_scores[g] = best[i];
+        max_rot[g] = arg[i];
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// plan
+
+struct tm_scan_plan {
+  int mode;  // 0 FLC, 1 CORR
+  int nq_in, nq_out;
+  int N[3], N2c;
+  int T[3], T2c;
+  int max_batch;
+  int normalize;
+  double mask_sum;
+  const float* templ;
+  const float* mask;
+  const float* filter;
+  const float2* ft;
+  const float2* ft2;
+  const float* inv_den;
+  tm_fft_plan* fast;   // line plans of N0, N1, N2
+  tm_fft_plan* tplan;  // template-size plan (filter)
+  // workspace
+  float *TR, *MR;
+  float2 *specT, *Q1, *P, *Q;
+  double *n_obs, *rot_ws, *mom_ws, *ms;
+};
+
+extern "C" int tm_fft_plan_create(int ndim, const int64_t* shape, int64_t batch, tm_fft_plan** out);
+extern "C" int tm_fft_plan_destroy(tm_fft_plan* p);
+
+static void scan_plan_free(tm_scan_plan* p) {
+  if (!p) return;
+  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->n_obs, p->rot_ws, p->mom_ws, p->ms};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (p->fast) tm_fft_plan_destroy(p->fast);
+  if (p->tplan) tm_fft_plan_destroy(p->tplan);
+  free(p);
+}
+
+#define TM_ALLOC(ptr, bytes)                               \
+  do {                                                     \
+    hipError_t _e = hipMalloc((void**)&(ptr), (bytes));    \
+    if (_e != hipSuccess) {                                \
+      tm_set_error("hipMalloc(%zu) failed: %s", (size_t)(bytes), hipGetErrorString(_e)); \
+      scan_plan_free(p);                                   \
+      return _e == hipErrorOutOfMemory ? TM_ERR_OOM : TM_ERR_HIP; \
+    }                                                      \
+  } while (0)
+
+extern "C" {
+
+int64_t tm_scan_workspace_bytes(int mode, const int64_t* fast_shape, const int* tshape, int max_batch,
+                                int with_filter) {
+  const int64_t N0 = fast_shape[0], N1 = fast_shape[1], N2c = fast_shape[2] / 2 + 1;
+  const int64_t T0 = tshape[0], T1 = tshape[1], T2 = tshape[2];
+  const int64_t nq_in = mode == 0 ? 2 : 1, nq_out = mode == 0 ? 3 : 1, B = max_batch;
+  int64_t bytes = 0;
+  bytes += 4 * B * T0 * T1 * T2 * (mode == 0 ? 2 : 1);
+  if (with_filter) bytes += 8 * B * T0 * T1 * (T2 / 2 + 1);
+  bytes += 8 * nq_in * B * T0 * T1 * N2c;
+  bytes += 8 * nq_in * B * T0 * N1 * N2c;
+  bytes += 8 * nq_out * B * N0 * N1 * N2c;
+  return bytes;
+}
+
+// Create a fused scan plan.  mode 0 = FLC (flc_setup/flc_scoring), mode 1 =
+// correlation family (CC/LCC/FLCSphericalMask: corr_scoring).  Device
+// pointers stay owned by the caller and must outlive the plan.
+int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, const float* templ,
+                        const float* mask, const float* filter, const void* ft_target, const void* ft_target2,
+                        const float* inv_denominator, int normalize, double mask_sum, int max_batch,
+                        tm_scan_plan** out) {
+  TM_REQUIRE(out && fast_shape && tshape && templ && ft_target, "null argument to tm_scan_plan_create");
+  TM_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (FLC) or 1 (correlation)");
+  TM_REQUIRE(mode == 1 || (mask && ft_target2), "FLC needs a mask and the squared-target spectrum");
+  TM_REQUIRE(!normalize || mask, "normalisation needs a mask");
+  TM_REQUIRE(max_batch >= 1 && max_batch <= 4096, "max_batch %d outside [1, 4096]", max_batch);
+  for (int i = 0; i < 3; ++i)
+    TM_REQUIRE(tshape[i] >= 1 && tshape[i] <= fast_shape[i], "template axis %d (%d) exceeds the FFT shape", i,
+               tshape[i]);
+  TM_REQUIRE(fast_shape[0] <= 4096 && fast_shape[1] <= 4096 && fast_shape[2] <= 4096, "FFT axis longer than 4096");
+  tm_scan_plan* p = (tm_scan_plan*)calloc(1, sizeof(tm_scan_plan));
+  TM_REQUIRE(p, "host allocation failed");
+  p->mode = mode;
+  p->nq_in = mode == 0 ? 2 : 1;
+  p->nq_out = mode == 0 ? 3 : 1;
+  for (int i = 0; i < 3; ++i) {
+    p->N[i] = (int)fast_shape[i];
+    p->T[i] = tshape[i];
+  }
+  p->N2c = p->N[2] / 2 + 1;
+  p->T2c = p->T[2] / 2 + 1;
+  p->max_batch = max_batch;
+  p->normalize = mode == 0 ? 1 : normalize;
+  p->mask_sum = mask_sum;
+  p->templ = templ;
+  p->mask = mask;
+  p->filter = filter;
+  p->ft = (const float2*)ft_target;
+  p->ft2 = (const float2*)ft_target2;
+  p->inv_den = inv_denominator;
+  int rc = tm_fft_plan_create(3, fast_shape, 1, &p->fast);
+  if (rc != TM_OK) {
+    scan_plan_free(p);
+    return rc;
+  }
+  if (filter) {
+    const int64_t ts[3] = {tshape[0], tshape[1], tshape[2]};
+    rc = tm_fft_plan_create(3, ts, 1, &p->tplan);
+    if (rc != TM_OK) {
+      scan_plan_free(p);
+      return rc;
+    }
+  }
+  const int64_t B = max_batch, T3 = (int64_t)p->T[0] * p->T[1] * p->T[2];
+  const int64_t N0 = p->N[0], N1 = p->N[1], N2c = p->N2c, T0 = p->T[0], T1 = p->T[1];
+  TM_ALLOC(p->TR, sizeof(float) * B * T3);
+  if (mode == 0) TM_ALLOC(p->MR, sizeof(float) * B * T3);
+  if (filter) TM_ALLOC(p->specT, sizeof(float2) * B * T0 * T1 * p->T2c);
+  TM_ALLOC(p->Q1, sizeof(float2) * p->nq_in * B * T0 * T1 * N2c);
+  TM_ALLOC(p->P, sizeof(float2) * p->nq_in * B * T0 * N1 * N2c);
+  TM_ALLOC(p->Q, sizeof(float2) * p->nq_out * B * N0 * N1 * N2c);
+  TM_ALLOC(p->n_obs, sizeof(double) * B);
+  TM_ALLOC(p->rot_ws, sizeof(double) * (tm_ceil_div(T3, 256) * B + 8));
+  TM_ALLOC(p->mom_ws, sizeof(double) * 2 * TM_MOM_BLOCKS * B);
+  TM_ALLOC(p->ms, sizeof(double) * 2 * B);
+  *out = p;
+  return TM_OK;
+}
+
+int tm_scan_plan_destroy(tm_scan_plan* p) {
+  scan_plan_free(p);
+  return TM_OK;
+}
+
+// Score rotations rots_dev[0..nbatch) (float32 3x3 each, device) with
+// analyzer indices rot_index_dev[0..nbatch) (device int32) and fold them into
+// max_scores / max_rot (fast_shape, device) with the strict '>' rule.
+int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch, float* max_scores,
+                  int* max_rot, void* stream) {
+  TM_REQUIRE(p && rots_dev && rot_index_dev && max_scores && max_rot, "null argument to tm_scan_batch");
+  TM_REQUIRE(nbatch >= 1 && nbatch <= p->max_batch, "batch %d outside [1, %d]", nbatch, p->max_batch);
+  hipStream_t s = (hipStream_t)stream;
+  const int B = nbatch;
+  const int T0 = p->T[0], T1 = p->T[1], T2 = p->T[2];
+  const int64_t T3 = (int64_t)T0 * T1 * T2;
+  const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2], N2c = p->N2c;
+  const bool flc = p->mode == 0;
+  int rc;
+
+  // 1. rotate template (and mask) about the box centre
+  rc = tm_rotate_launch(p->templ, flc ? p->mask : nullptr, p->T, rots_dev, B, p->TR, flc ? p->MR : nullptr,
+                        (int64_t)T1 * T2, T2, T3, flc ? p->n_obs : nullptr, flc ? p->rot_ws : nullptr, s);
+  if (rc != TM_OK) return rc;
+
+  // 2. template-size Fourier filter
+  if (p->filter) {
+    rc = tm_fft_r2c_batch(p->tplan, B, p->TR, p->specT, s);
+    if (rc != TM_OK) return rc;
+    const int64_t nf = (int64_t)T0 * T1 * p->T2c;
+    rc = tm_cmul_cr(p->specT, p->filter, p->specT, nf * B, nf, stream);
+    if (rc != TM_OK) return rc;
+    rc = tm_fft_c2r_batch(p->tplan, B, p->specT, p->TR, s);
+    if (rc != TM_OK) return rc;
+  }
+
+  // 3. normalise inside the (rotated / fixed) mask with FP64 moments
+  if (p->normalize) {
+    const float* m = flc ? p->MR : p->mask;
+    const int64_t mstride = flc ? T3 : 0;
+    TM_TIMED("fused_normalize", 12.0 * T3 * B, s,
+             hipLaunchKernelGGL(k_moments_batched, dim3(TM_MOM_BLOCKS, B), dim3(TM_FU_THREADS), 0, s, p->TR, m, T3,
+                                mstride, p->mom_ws));
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_moments_finalize, dim3((unsigned)tm_ceil_div(B, 64)), dim3(64), 0, s, p->mom_ws,
+                       TM_MOM_BLOCKS, flc ? p->n_obs : (const double*)nullptr, p->mask_sum, B, p->ms);
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_normalize_batched, dim3((unsigned)tm_ceil_div(T3, 1024), B), dim3(TM_FU_THREADS), 0, s,
+                       p->TR, m, T3, mstride, (const double*)p->ms);
+    TM_CHECK_LAUNCH();
+  }
+
+  // 4. x pass (R2C of the zero-padded template rows) and y pass (T1 -> N1)
+  const LinePlan& lx = p->fast->lines[2].lp;
+  const LinePlan& ly = p->fast->lines[1].lp;
+  const LinePlan& lz = p->fast->lines[0].lp;
+  const int64_t q1_stride = (int64_t)B * T0 * T1 * N2c;
+  if (flc) {
+    rc = tm_launch_r2c_rows(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, s, "fused_x_pass");
+    if (rc != TM_OK) return rc;
+  }
+  rc = tm_launch_r2c_rows(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, s, "fused_x_pass");
+  if (rc != TM_OK) return rc;
+  rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, -1, 1.0f, s, "fused_y_pass");
+  if (rc != TM_OK) return rc;
+
+  // 5. pass A: z-axis, spectral products
+  {
+    int TI = 4096 / N0;
+    if (TI > 16) TI = 16;
+    if (TI > N2c) TI = N2c;
+    TM_REQUIRE(TI >= 1 && (int64_t)N0 * TI <= TM_FU_MAXP * TM_FU_THREADS, "z axis too long for pass A");
+    const int tpr = (int)tm_ceil_div(N2c, TI);
+    const int64_t blocks = (int64_t)N1 * tpr;
+    const size_t shm = 2 * (size_t)N0 * TI * sizeof(float2);
+    // target spectra once per launch + pruned template columns + products
+    const double C = (double)N0 * N1 * N2c;
+    const double bytes = 8.0 * C * (flc ? 2 : 1) + 8.0 * p->nq_in * B * (double)T0 * N1 * N2c +
+                         8.0 * C * p->nq_out * B;
+    if (flc)
+      TM_TIMED("fused_pass_a", bytes, s,
+               hipLaunchKernelGGL(k_pass_a<3>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->P, p->ft,
+                                  p->ft2, p->Q, lz, T0, N1, N2c, B, TI, tpr));
+    else
+      TM_TIMED("fused_pass_a", bytes, s,
+               hipLaunchKernelGGL(k_pass_a<1>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->P, p->ft,
+                                  (const float2*)nullptr, p->Q, lz, T0, N1, N2c, B, TI, tpr));
+    TM_CHECK_LAUNCH();
+  }
+
+  // 6. pass B: inverse y in place
+  rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, 1, 1.0f, s, "fused_pass_b");
+  if (rc != TM_OK) return rc;
+
+  // 7. pass C: inverse x, score, max/argmax
+  {
+    int RL = 4096 / N2;
+    if (RL > 16) RL = 16;
+    const int64_t nrows = (int64_t)N0 * N1;
+    if (nrows < RL) RL = (int)nrows;
+    TM_REQUIRE(RL >= 1 && RL * N2 <= TM_FU_MAXP * TM_FU_THREADS, "x axis too long for pass C");
+    const size_t shm = 2 * (size_t)RL * (N2 + 1) * sizeof(float2);
+    const int64_t blocks = tm_ceil_div(nrows, RL);
+    const double F = (double)nrows * N2;
+    const double bytes = 8.0 * (double)nrows * N2c * p->nq_out * B + 16.0 * F + (p->inv_den ? 4.0 * F * B : 0.0);
+    const float scale = (float)(1.0 / F);
+    const float eps = 1.1920928955078125e-07f;
+    if (flc)
+      TM_TIMED("fused_pass_c", bytes, s,
+               hipLaunchKernelGGL(k_pass_c<3>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->Q, lx, nrows,
+                                  RL, B, (const double*)p->n_obs, (const float*)nullptr, rot_index_dev, scale, eps,
+                                  max_scores, max_rot));
+    else
+      TM_TIMED("fused_pass_c", bytes, s,
+               hipLaunchKernelGGL(k_pass_c<1>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->Q, lx, nrows,
+                                  RL, B, (const double*)nullptr, p->inv_den, rot_index_dev, scale, eps, max_scores,
+                                  max_rot));
+    TM_CHECK_LAUNCH();
+  }
+  return TM_OK;
+}
+
+}  // extern "C"

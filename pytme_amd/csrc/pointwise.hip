@@ -62,6 +62,7 @@ __global__ void k_binary_f32(int op, const float* __restrict__ a, const float* _
 // Reference norm_scores (tme/backends/npfftw_backend.py:537-587), numpy
 // semantics: sigma := 1 where sigma < eps.
 __device__ __forceinline__ float flc_norm(float x, float ex2, float ex, float n, float eps) {
+#pragma clang fp contract(off)
   float m = ex / n;
   m = m * m;
   float v = ex2 / n - m;

@@ -158,6 +158,38 @@ __global__ void k_sum_partials(const double* __restrict__ partials, int nparts, 
   out[b] = s;
 }
 
+// Internal launcher shared with the fused scan: poses are float32 rotations
+// (9 per batch entry) about the geometric box centre.
+int tm_rotate_launch(const float* src, const float* srcm, const int* shape, const float* rots, int nbatch,
+                     float* dst, float* dstm, int64_t ds0, int64_t ds1, int64_t dbatch, double* n_obs,
+                     double* workspace, hipStream_t s) {
+  TM_REQUIRE(nbatch >= 1 && nbatch <= 65535, "rotation batch %d outside [1, 65535]", nbatch);
+  RotBox box;
+  box.s0 = shape[0];
+  box.s1 = shape[1];
+  box.s2 = shape[2];
+  box.ds0 = ds0;
+  box.ds1 = ds1;
+  box.dbatch = dbatch;
+  RotPose pose;
+  for (int k = 0; k < 3; ++k) {
+    pose.center[k] = (shape[k] - 1) / 2.0;
+    pose.translation[k] = 0.0;
+  }
+  const int64_t nvox = (int64_t)box.s0 * box.s1 * box.s2;
+  const int64_t nblk = tm_ceil_div(nvox, TM_ROT_THREADS);
+  TM_TIMED("rotate_trilinear", (srcm ? 8.0 : 4.0) * nvox * nbatch, s,
+           hipLaunchKernelGGL(k_rotate_trilinear, dim3((unsigned)nblk, nbatch), dim3(TM_ROT_THREADS), 0, s, src,
+                              srcm, rots, (const double*)nullptr, pose, box, dst, dstm, n_obs ? workspace : nullptr));
+  TM_CHECK_LAUNCH();
+  if (n_obs) {
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)tm_ceil_div(nbatch, 64)), dim3(64), 0, s, workspace,
+                       (int)nblk, n_obs, nbatch);
+    TM_CHECK_LAUNCH();
+  }
+  return TM_OK;
+}
+
 extern "C" {
 
 // Number of doubles the workspace of tm_rotate_trilinear must hold when
