@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "fft_core.h"
+#include "fft_ct.h"
 #include "fft_plan.h"
 #include "tm_common.h"
 
@@ -53,6 +54,60 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
     const int e = f / ti, l = f - e * ti;
     dst[(int64_t)e * inner + l] = c_scale(R[f], scale);
   }
+}
+
+// Compile-time specialised variant of k_c2c_strided for the common lengths:
+// the tile (16 lines of 128 B for the long axes) goes straight from global
+// memory into registers, LDS only carries the inter-stage exchanges.
+template <int N, int L, int NT, int DIR>
+__global__ void __launch_bounds__(NT)
+k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner,
+         int64_t tiles_per_outer, float scale) {
+  using PL = CtPlan<N, L, NT, true, false>;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int64_t o = blockIdx.x / tiles_per_outer;
+  const int64_t i0 = (blockIdx.x - o * tiles_per_outer) * (int64_t)L;
+  const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
+  const float2* src = in + o * (int64_t)lin * inner + i0;
+  float2* dst = out + o * (int64_t)N * inner + i0;
+  float2 v[PL::E_IN], w[PL::E_OUT];
+#pragma unroll
+  for (int i = 0; i < PL::E_IN / PL::R_IN; ++i)
+#pragma unroll
+    for (int r = 0; r < PL::R_IN; ++r) {
+      int l, e;
+      const bool ok = PL::in_pos(tid, i, r, l, e);
+      v[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * (int)inner + l] : make_float2(0.f, 0.f);
+    }
+  ct_fft<PL, DIR, L>(lds, tw, v, w, tid);
+#pragma unroll
+  for (int i = 0; i < PL::E_OUT / PL::R_OUT; ++i)
+#pragma unroll
+    for (int r = 0; r < PL::R_OUT; ++r) {
+      int l, e;
+      if (PL::out_pos(tid, i, r, l, e) && l < ti) dst[e * (int)inner + l] = c_scale(w[i * PL::R_OUT + r], scale);
+    }
+}
+
+template <int N, int L, int NT>
+static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
+                         int dir, float scale, hipStream_t s, const char* name) {
+  const int64_t tpo = tm_ceil_div(inner, L);
+  const int64_t blocks = outer * tpo;
+  TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
+  const size_t shm = (size_t)N * L * sizeof(float2);
+  const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
+  if (dir < 0)
+    TM_TIMED(name, bytes, s,
+             hipLaunchKernelGGL((k_c2c_ct<N, L, NT, -1>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw,
+                                lin, inner, tpo, scale));
+  else
+    TM_TIMED(name, bytes, s,
+             hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw,
+                                lin, inner, tpo, scale));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
 }
 
 // R2C along contiguous rows: in rows of lin <= n reals (stride lin,
@@ -164,6 +219,15 @@ static int tile_lines(int n, int64_t avail) {
 
 int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
                   int dir, float scale, hipStream_t s, const char* name) {
+  if (inner >= 12) {
+    switch (lp.n) {
+      case 128: return launch_c2c_ct<128, 16, 256>(lp, in, lin, out, outer, inner, dir, scale, s, name);
+      case 256: return launch_c2c_ct<256, 16, 256>(lp, in, lin, out, outer, inner, dir, scale, s, name);
+      case 512: return launch_c2c_ct<512, 16, 512>(lp, in, lin, out, outer, inner, dir, scale, s, name);
+      case 1024: return launch_c2c_ct<1024, 8, 512>(lp, in, lin, out, outer, inner, dir, scale, s, name);
+      default: break;
+    }
+  }
   const int TI = tile_lines(lp.n, inner);
   const int64_t tpo = tm_ceil_div(inner, TI);
   const int64_t blocks = outer * tpo;

@@ -1,0 +1,244 @@
+// Compile-time specialised Stockham FFT for a group of L lines of length N.
+//
+// The line length, line count, thread count and radix sequence are template
+// constants, so every index, stride and loop bound folds at compile time.
+// Data enter and leave in registers; LDS is used only to exchange values
+// between radix stages (one in-place buffer: each stage reads all of its
+// inputs into registers, synchronises, then writes its outputs).  An FFT
+// with S radix stages therefore touches LDS 2(S-1) times per element and
+// global memory only where the caller loads and stores.
+//
+// Ownership: butterfly t = tid + i*NT of a stage with radix R maps to line l
+// and butterfly j (LFAST: l = t % L, j = t / L; otherwise j = t % (N/R),
+// l = t / (N/R)).  The first stage reads elements j + r*N/R and the last
+// stage produces elements j + r*N/R of its own radix, so a forward plan's
+// outputs and the inverse plan with the reversed radix order (REV) have the
+// same per-thread ownership: spectra can be multiplied in registers between
+// a forward and an inverse transform.
+#pragma once
+#include "fft_core.h"
+
+constexpr int ct_factor(int n, int idx) {
+  int m = n, p2 = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++p2;
+  }
+  int n8 = p2 / 3, rem = p2 % 3;
+  if (rem == 1 && n8 > 0) {
+    n8 -= 1;
+    rem = 4;
+  }
+  int k = 0;
+  for (int i = 0; i < n8; ++i, ++k)
+    if (k == idx) return 8;
+  if (rem == 2) {
+    if (k == idx) return 4;
+    ++k;
+  } else if (rem == 4) {
+    if (k == idx) return 4;
+    ++k;
+    if (k == idx) return 4;
+    ++k;
+  } else if (rem == 1) {
+    if (k == idx) return 2;
+    ++k;
+  }
+  const int odd[3] = {5, 3, 7};
+  for (int q = 0; q < 3; ++q) {
+    while (m % odd[q] == 0) {
+      if (k == idx) return odd[q];
+      ++k;
+      m /= odd[q];
+    }
+  }
+  return 0;
+}
+
+constexpr int ct_nstages(int n) {
+  int s = 0;
+  while (ct_factor(n, s) != 0) ++s;
+  return s;
+}
+
+constexpr bool ct_supported(int n) {
+  int p = 1;
+  for (int s = 0; s < ct_nstages(n); ++s) p *= ct_factor(n, s);
+  return p == n;
+}
+
+template <int N_, int L_, int NT_, bool LFAST_, bool REV_>
+struct CtPlan {
+  static constexpr int N = N_, L = L_, NT = NT_;
+  static constexpr bool LFAST = LFAST_, REV = REV_;
+  static constexpr int NST = ct_nstages(N_);
+  static_assert(ct_supported(N_), "length not supported by the compile-time FFT");
+  static constexpr int radix(int s) { return REV ? ct_factor(N_, NST - 1 - s) : ct_factor(N_, s); }
+  static constexpr int ns(int s) {
+    int v = 1;
+    for (int i = 0; i < s; ++i) v *= radix(i);
+    return v;
+  }
+  static constexpr int bpt(int R) { return (N_ / R * L_ + NT_ - 1) / NT_; }
+  static constexpr int R_IN = radix(0), R_OUT = radix(NST - 1);
+  static constexpr int E_IN = bpt(R_IN) * R_IN, E_OUT = bpt(R_OUT) * R_OUT;
+
+  // line / element of register slot (i, r) of a stage with radix R
+  template <int R>
+  __device__ static __forceinline__ bool slot(int tid, int i, int& l, int& j) {
+    constexpr int NB = N_ / R;
+    const int t = tid + i * NT_;
+    if (LFAST) {
+      l = t % L_;
+      j = t / L_;
+    } else {
+      j = t % NB;
+      l = t / NB;
+    }
+    return (NB * L_) % NT_ == 0 || t < NB * L_;
+  }
+  __device__ static __forceinline__ bool in_pos(int tid, int i, int r, int& l, int& e) {
+    int j;
+    const bool ok = slot<R_IN>(tid, i, l, j);
+    e = j + r * (N_ / R_IN);
+    return ok;
+  }
+  __device__ static __forceinline__ bool out_pos(int tid, int i, int r, int& l, int& e) {
+    int j;
+    const bool ok = slot<R_OUT>(tid, i, l, j);
+    e = j + r * (N_ / R_OUT);
+    return ok;
+  }
+};
+
+template <int DIR, int R>
+__device__ __forceinline__ void ct_dft(float2* v) {
+  if constexpr (R == 2) dft2<DIR>(v);
+  else if constexpr (R == 3) dft3<DIR>(v);
+  else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 5) dft5<DIR>(v);
+  else if constexpr (R == 8) dft8<DIR>(v);
+  else {
+    // generic odd prime with compile-time twiddles
+    float2 out[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const int m = (r * q) % R;
+        const double a = (double)DIR * 2.0 * 3.14159265358979323846 * (double)m / (double)R;
+        acc = c_add(acc, c_mul(v[r], make_float2((float)__builtin_cos(a), (float)__builtin_sin(a))));
+      }
+      out[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = out[q];
+  }
+}
+
+// LDS address of (line l, element e): element-major e*LD + l (LFAST) or
+// line-major l*LD + e.
+template <class P, int LD>
+__device__ __forceinline__ int ct_lds(int l, int e) {
+  return P::LFAST ? e * LD + l : l * LD + e;
+}
+
+// twiddle W_N^(idx) (DIR-signed) from a table of a multiple TWS*N length
+template <int DIR, int TWS>
+__device__ __forceinline__ float2 ct_tw(const float2* __restrict__ tw, int idx) {
+  return tw_load<DIR>(tw, idx * TWS);
+}
+
+// v[r] *= W^(r*base) for r = 1..R-1: one table load, powers by recurrence
+// (at most R-2 = 6 extra roundings for radix 8, ~4e-7 relative).
+template <int DIR, int TWS, int R>
+__device__ __forceinline__ void ct_twiddle(float2* v, const float2* __restrict__ tw, int base) {
+  if (base == 0) return;
+  const float2 w1 = ct_tw<DIR, TWS>(tw, base);
+  float2 w = w1;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    v[r] = c_mul(v[r], w);
+    if (r + 1 < R) w = c_mul(w, w1);
+  }
+}
+
+// middle stage s (reads LDS, writes LDS)
+template <class P, int DIR, int LD, int TWS, int S>
+__device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid) {
+  if constexpr (S < P::NST - 1) {
+    constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
+    float2 v[BPT][R];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      int l, j;
+      if (P::template slot<R>(tid, i, l, j)) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      int l, j;
+      if (P::template slot<R>(tid, i, l, j)) {
+        const int k = j % NS;
+        ct_twiddle<DIR, TWS, R>(v[i], tw, k * TS);
+        ct_dft<DIR, R>(v[i]);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[ct_lds<P, LD>(l, base + r * NS)] = v[i][r];
+      }
+    }
+    __syncthreads();
+    ct_mid<P, DIR, LD, TWS, S + 1>(lds, tw, tid);
+  }
+}
+
+// Transform: in[E_IN] (first-stage ownership) -> out[E_OUT] (last-stage
+// ownership).  All threads of the block must call it.  `lds` holds at least
+// the L lines at stride LD.  TWS: the table tw has TWS*N entries of
+// exp(-2 pi i k / (TWS*N)).
+template <class P, int DIR, int LD, int TWS = 1>
+__device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ tw, float2* in, float2* out,
+                                       int tid) {
+  constexpr int R0 = P::radix(0);
+  constexpr int B0 = P::bpt(R0);
+  if constexpr (P::NST == 1) {
+#pragma unroll
+    for (int i = 0; i < B0; ++i) {
+      ct_dft<DIR, R0>(in + i * R0);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) out[i * R0 + r] = in[i * R0 + r];
+    }
+    return;
+  } else {
+    __syncthreads();  // the buffer may still be read by a previous transform
+#pragma unroll
+    for (int i = 0; i < B0; ++i) {
+      int l, j;
+      if (P::template slot<R0>(tid, i, l, j)) {
+        ct_dft<DIR, R0>(in + i * R0);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) lds[ct_lds<P, LD>(l, j * R0 + r)] = in[i * R0 + r];
+      }
+    }
+    __syncthreads();
+    ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid);
+    constexpr int S = P::NST - 1;
+    constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      int l, j;
+      if (P::template slot<R>(tid, i, l, j)) {
+        float2* v = out + i * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = lds[ct_lds<P, LD>(l, j + r * NB)];
+        const int k = j % NS;
+        ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
+        ct_dft<DIR, R>(v);
+      }
+    }
+  }
+}

@@ -30,6 +30,7 @@
 #include <string.h>
 
 #include "fft_core.h"
+#include "fft_ct.h"
 #include "fft_plan.h"
 #include "tm_common.h"
 
@@ -289,6 +290,249 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int RL, int B
 }
 
 // --------------------------------------------------------------------------
+// compile-time specialised pass A / pass C
+
+// Pass A with the z transforms in registers: a block owns L = 16 (128 B)
+// consecutive kx of one ky; spectra are multiplied in registers between the
+// forward (pruned) and inverse transforms.  KIND 1: X = F t (template slot
+// q_in, output slot o_x); KIND 0 (FLC): S1 = F m, S2 = F2 m (mask slot q_in,
+// output slots 0 and 1).  Splitting the FLC work into two launches keeps the
+// resident spectra and transform buffers within the register file.
+template <int N0, int L, int NT, int KIND>
+__global__ void __launch_bounds__(NT)
+k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
+            float2* __restrict__ Q, const float2* __restrict__ tw, int T0, int N1, int N2c, int B, int tiles_per_row,
+            int q_in, int o_x) {
+  using PF = CtPlan<N0, L, NT, true, false>;
+  using PI = CtPlan<N0, L, NT, true, true>;
+  static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
+  constexpr int EI = PF::E_IN, EO = PF::E_OUT, EW = EI > EO ? EI : EO;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int ky = blockIdx.x / tiles_per_row;
+  const int kx0 = (blockIdx.x - ky * tiles_per_row) * L;
+  const int ti = (N2c - kx0) < L ? (N2c - kx0) : L;
+  const int plane = N1 * N2c;  // per-slab offsets fit in 32 bits (< 2^31 / 8 elements)
+  const int64_t col = (int64_t)ky * N2c + kx0;
+  const float2* Ftc = Ft + col;
+  const float2* Ft2c = KIND == 0 ? Ft2 + col : nullptr;
+
+  float2 F[EO], F2[KIND == 0 ? EO : 1];
+#pragma unroll
+  for (int i = 0; i < EO / PF::R_OUT; ++i)
+#pragma unroll
+    for (int r = 0; r < PF::R_OUT; ++r) {
+      int l, e;
+      const bool ok = PF::out_pos(tid, i, r, l, e) && l < ti;
+      const int s = i * PF::R_OUT + r;
+      F[s] = ok ? Ftc[e * plane + l] : make_float2(0.f, 0.f);
+      if constexpr (KIND == 0) F2[s] = ok ? Ft2c[e * plane + l] : make_float2(0.f, 0.f);
+    }
+
+  auto inv_store = [&](int o, int b, float2* w) {
+    ct_fft<PI, 1, L>(lds, tw, w, w, tid);
+    float2* dst = Q + ((int64_t)(o * B + b) * N0) * (int64_t)plane + col;
+#pragma unroll
+    for (int i = 0; i < PI::E_OUT / PI::R_OUT; ++i)
+#pragma unroll
+      for (int r = 0; r < PI::R_OUT; ++r) {
+        int l, e;
+        if (PI::out_pos(tid, i, r, l, e) && l < ti) dst[e * plane + l] = w[i * PI::R_OUT + r];
+      }
+  };
+
+  for (int b = 0; b < B; ++b) {
+    float2 w[EW];
+    const float2* src = P + ((int64_t)(q_in * B + b) * T0) * (int64_t)plane + col;
+#pragma unroll
+    for (int i = 0; i < EI / PF::R_IN; ++i)
+#pragma unroll
+      for (int r = 0; r < PF::R_IN; ++r) {
+        int l, e;
+        const bool ok = PF::in_pos(tid, i, r, l, e) && l < ti && e < T0;
+        w[i * PF::R_IN + r] = ok ? src[e * plane + l] : make_float2(0.f, 0.f);
+      }
+    ct_fft<PF, -1, L>(lds, tw, w, w, tid);
+    if constexpr (KIND == 1) {
+#pragma unroll
+      for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
+      inv_store(o_x, b, w);
+    } else {
+      float2 t[EO];
+#pragma unroll
+      for (int s = 0; s < EO; ++s) {
+        t[s] = c_mul(F2[s], w[s]);
+        w[s] = c_mul(F[s], w[s]);
+      }
+      inv_store(0, b, w);
+      inv_store(1, b, t);
+    }
+  }
+}
+
+// Pass C with the x inverse as a half-length complex transform: the N2 = 2M
+// real outputs of a row are the real/imaginary parts of the M-point inverse
+// of Z[k] = (X[k] + X*[M-k]) + i W^k (X[k] - X*[M-k]), W = exp(2 pi i/N2)
+// (imaginary parts of the DC and Nyquist bins dropped, as numpy's irfft).
+template <int M, int RL, int NT, int NQ>
+__global__ void __launch_bounds__(NT)
+k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_t nrows, int B,
+            const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
+            float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
+  using PI = CtPlan<M, RL, NT, false, true>;
+  constexpr int N2 = 2 * M, N2c = M + 1, LD = M + 1;
+  constexpr int EI = PI::E_IN, EO = PI::E_OUT;
+  extern __shared__ float2 lds[];
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * RL;
+  const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
+
+  float best0[EO], best1[EO], a1[EO], b1[EO], a2[EO], b2[EO];
+  int arg0[EO], arg1[EO];
+#pragma unroll
+  for (int s = 0; s < EO; ++s) {
+    best0[s] = best1[s] = -INFINITY;
+    arg0[s] = arg1[s] = -1;
+    a1[s] = b1[s] = a2[s] = b2[s] = 0.f;
+  }
+  for (int b = 0; b < B; ++b) {
+    const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
+#pragma unroll 1
+    for (int o = 0; o < NQ; ++o) {
+      const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * N2c;
+      __syncthreads();
+      for (int f = tid; f < N2c * rl; f += NT) {
+        const int l = f / N2c, e = f - l * N2c;
+        lds[l * LD + e] = src[f];
+      }
+      __syncthreads();
+      float2 z[EI > EO ? EI : EO];
+      float2* y = z;
+#pragma unroll
+      for (int i = 0; i < EI / PI::R_IN; ++i)
+#pragma unroll
+        for (int r = 0; r < PI::R_IN; ++r) {
+          int l, k;
+          if (!PI::in_pos(tid, i, r, l, k)) {
+            z[i * PI::R_IN + r] = make_float2(0.f, 0.f);
+            continue;
+          }
+          float2 A = lds[l * LD + k];
+          float2 C = lds[l * LD + (M - k)];
+          if (k == 0) {
+            A.y = 0.f;
+            C.y = 0.f;
+          }
+          const float2 E = make_float2(A.x + C.x, A.y - C.y);
+          const float2 D = c_mul(make_float2(A.x - C.x, A.y + C.y), tw_load<1>(tw2, k));
+          z[i * PI::R_IN + r] = make_float2(E.x - D.y, E.y + D.x);
+        }
+      ct_fft<PI, 1, LD, 2>(lds, tw2, z, z, tid);
+#pragma unroll
+      for (int i = 0; i < EO / PI::R_OUT; ++i)
+#pragma unroll
+        for (int r = 0; r < PI::R_OUT; ++r) {
+          int l, m;
+          const int s = i * PI::R_OUT + r;
+          if (!PI::out_pos(tid, i, r, l, m) || l >= rl) continue;
+          const float x0 = y[s].x * scale, x1 = y[s].y * scale;
+          if (o == NQ - 1) {
+            float s0, s1;
+            if (NQ == 3) {
+              s0 = fu_flc_norm(x0, a2[s], a1[s], nf, eps);
+              s1 = fu_flc_norm(x1, b2[s], b1[s], nf, eps);
+            } else if (inv_den) {
+              const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + l) * N2 + 2 * m);
+              s0 = x0 * d.x;
+              s1 = x1 * d.y;
+            } else {
+              s0 = x0;
+              s1 = x1;
+            }
+            if (s0 > best0[s]) {
+              best0[s] = s0;
+              arg0[s] = rot_idx[b];
+            }
+            if (s1 > best1[s]) {
+              best1[s] = s1;
+              arg1[s] = rot_idx[b];
+            }
+          } else if (o == 0) {
+            a1[s] = x0;
+            b1[s] = x1;
+          } else {
+            a2[s] = x0;
+            b2[s] = x1;
+          }
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < EO / PI::R_OUT; ++i)
+#pragma unroll
+    for (int r = 0; r < PI::R_OUT; ++r) {
+      int l, m;
+      const int s = i * PI::R_OUT + r;
+      if (!PI::out_pos(tid, i, r, l, m) || l >= rl) continue;
+      const int64_t g = (r0 + l) * N2 + 2 * m;
+      float2 old = *reinterpret_cast<float2*>(max_scores + g);
+      int2 oldr = *reinterpret_cast<int2*>(max_rot + g);
+      bool upd = false;
+      if (best0[s] > old.x) {
+        old.x = best0[s];
+        oldr.x = arg0[s];
+        upd = true;
+      }
+      if (best1[s] > old.y) {
+        old.y = best1[s];
+        oldr.y = arg1[s];
+        upd = true;
+      }
+      if (upd) {
+        *reinterpret_cast<float2*>(max_scores + g) = old;
+        *reinterpret_cast<int2*>(max_rot + g) = oldr;
+      }
+    }
+}
+
+template <int N0, int L, int NT>
+static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
+                            const float2* tw, int T0, int N1, int N2c, int B, double bytes, hipStream_t s) {
+  const int tpr = (int)tm_ceil_div(N2c, L);
+  const int64_t blocks = (int64_t)N1 * tpr;
+  const size_t shm = (size_t)N0 * L * sizeof(float2);
+  if (flc) {
+    TM_TIMED("fused_pass_a", bytes * 0.6, s,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 0>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft, ft2, Q,
+                                tw, T0, N1, N2c, B, tpr, 0, 0));
+    TM_CHECK_LAUNCH();
+  }
+  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", flc ? bytes * 0.4 : bytes, s,
+           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
+                              (const float2*)nullptr, Q, tw, T0, N1, N2c, B, tpr, flc ? 1 : 0, flc ? 2 : 0));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+template <int M, int RL, int NT>
+static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_t nrows, int B, const double* n_obs,
+                            const float* inv_den, const int* rot_idx, float scale, float eps, float* max_scores,
+                            int* max_rot, double bytes, hipStream_t s) {
+  const int64_t blocks = tm_ceil_div(nrows, RL);
+  const size_t shm = (size_t)RL * (M + 1) * sizeof(float2);
+  if (flc)
+    TM_TIMED("fused_pass_c", bytes, s,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2, nrows,
+                                B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
+  else
+    TM_TIMED("fused_pass_c", bytes, s,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2, nrows,
+                                B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores, max_rot));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+// --------------------------------------------------------------------------
 // plan
 
 struct tm_scan_plan {
@@ -484,7 +728,19 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   if (rc != TM_OK) return rc;
 
   // 5. pass A: z-axis, spectral products
-  {
+  const double C_ = (double)N0 * N1 * N2c;
+  const double bytes_a = 8.0 * C_ * (flc ? 2 : 1) + 8.0 * p->nq_in * B * (double)T0 * N1 * N2c +
+                         8.0 * C_ * p->nq_out * B;
+  bool done_a = true;
+  switch (N0) {
+    case 128: rc = launch_pass_a_ct<128, 16, 256>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
+    case 256: rc = launch_pass_a_ct<256, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
+    case 512: rc = launch_pass_a_ct<512, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
+    case 1024: rc = launch_pass_a_ct<1024, 8, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
+    default: done_a = false; break;
+  }
+  if (done_a && rc != TM_OK) return rc;
+  if (!done_a) {
     int TI = 4096 / N0;
     if (TI > 16) TI = 16;
     if (TI > N2c) TI = N2c;
@@ -512,7 +768,23 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   if (rc != TM_OK) return rc;
 
   // 7. pass C: inverse x, score, max/argmax
+  bool done_c = false;
   {
+    const int64_t nrows = (int64_t)N0 * N1;
+    const double F = (double)nrows * N2;
+    const double bytes = 8.0 * (double)nrows * N2c * p->nq_out * B + 16.0 * F + (p->inv_den ? 4.0 * F * B : 0.0);
+    const float scale = (float)(1.0 / F);
+    const float eps = 1.1920928955078125e-07f;
+    done_c = true;
+    switch (N2) {
+      case 128: rc = launch_pass_c_ct<64, 32, 256>(flc, p->Q, lx.tw, nrows, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 200: rc = launch_pass_c_ct<100, 32, 512>(flc, p->Q, lx.tw, nrows, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 256: rc = launch_pass_c_ct<128, 32, 512>(flc, p->Q, lx.tw, nrows, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      default: done_c = false; break;
+    }
+    if (done_c && rc != TM_OK) return rc;
+  }
+  if (!done_c) {
     int RL = 4096 / N2;
     if (RL > 16) RL = 16;
     const int64_t nrows = (int64_t)N0 * N1;

@@ -122,3 +122,41 @@ def test_filtered_parity(score, fused):
                               return_raw=True, template_filter=tfilt, target_filter=gfilt)
     n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+@pytest.mark.parametrize("shape", [(128, 128, 200), (128, 128, 128)])
+@pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_specialised_sizes_parity(shape, score, fused):
+    """Tomogram sizes that take the compile-time FFT kernels (512/128-point
+    axes, half-length 200/128-point C2R) against the FP64 oracle."""
+    target, template, mask, rots = _workload(seed=31, shape=shape, T=16, n_rot=4, copies=4)
+    if score == "FLCSphericalMask":
+        from pytme_amd.synthetic import spherical_mask
+
+        mask = spherical_mask(16, 7)
+    print(f"specialised {shape} {score} fused={fused}", flush=True)
+    s, off, r, _ = _run(score, target, template, mask, rots, fused, threshold=-1.0)
+    _, _, raws, oc = orc.scan(score, target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+def test_fused_matches_plain_full_size():
+    """Size-independent property at the benchmark size: the fused batched scan
+    and the per-rotation loop give the same maxima and rotation indices on a
+    512x512x200 tomogram with the wedge x CTF filters."""
+    from pytme_amd.synthetic import make_workload
+
+    target, template, mask, rotations, filters, spec = make_workload("flc_512x512x200_t48", copies=20, n_rotations=6)
+    outs = []
+    for fused in (False, True):
+        print(f"full size fused={fused}", flush=True)
+        s, off, r, mapping = _run("FLC", target, template, mask, rotations, fused, threshold=0.0,
+                                  template_filter=filters[0], target_filter=filters[1])
+        outs.append((s, r))
+    (s0, r0), (s1, r1) = outs
+    assert np.abs(s0 - s1).max() < 1e-5
+    agree = (r0 == r1).mean()
+    assert agree > 0.9999, agree
