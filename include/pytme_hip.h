@@ -121,6 +121,11 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* template
                         const float* inv_denominator, int normalize, double mask_sum, int max_batch,
                         tm_scan_plan** plan);
 int tm_scan_plan_destroy(tm_scan_plan* plan);
+/* Refresh the plan's copy of the target spectra (same shapes as at creation;
+ * stream-ordered).  The plan keeps the spectra at a padded row stride, so a
+ * caller whose spectra change while a plan is kept (a new target split in
+ * tme/matching_exhaustive.py:305-327) calls this before the next batch. */
+int tm_scan_plan_set_target(tm_scan_plan* plan, const void* ft_target, const void* ft_target2, void* stream);
 int tm_scan_batch(tm_scan_plan* plan, const float* rotations, const int* rotation_indices, int nbatch,
                   float* max_scores, int* max_rotations, void* stream);
 

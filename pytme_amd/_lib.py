@@ -72,6 +72,7 @@ SIGNATURES = {
         ],
     ),
     "tm_scan_plan_destroy": (c_int, [c_void_p]),
+    "tm_scan_plan_set_target": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_scan_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
 }
 

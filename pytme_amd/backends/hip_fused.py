@@ -138,19 +138,23 @@ def run(backend, score: str, **kw):
         t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,
         tuple(0 if x is None else x.data_ptr() for x in keep),
     )
+    ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
+    stream = backend._stream()
     if _PLAN["key"] != key:
         _free_plan()
         handle = ctypes.c_void_p()
-        ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
         _lib.call(
             "tm_scan_plan_create", mode, _lib.i64_array(fast_shape), _lib.i32_array(tshape), ptr(templ), ptr(mask),
             ptr(filt), ptr(ft_target), ptr(ft2), ptr(inv_den), normalize, mask_sum, B, ctypes.byref(handle),
         )
         _PLAN.update(key=key, handle=handle.value, keep=keep)
+    else:
+        # the plan holds a row-padded copy of the target spectra: refresh it,
+        # the caller's arrays may hold new values at the same addresses
+        _lib.call("tm_scan_plan_set_target", _PLAN["handle"], ptr(ft_target), ptr(ft2), stream)
 
     idx = analyzer.rotation_indices(state, rots)
     idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
-    stream = backend._stream()
     for start in range(0, n, B):
         nb = min(B, n - start)
         _lib.call(

@@ -24,15 +24,15 @@
 // --------------------------------------------------------------------------
 // kernels
 
-// C2C along the middle axis: in is [outer][lin][inner], out is
-// [outer][n][inner] (lin <= n; the missing lin..n-1 inputs are zeros, the
-// pruned transform of a zero-padded line).  in == out (lin == n) is the
-// in-place case.  A block owns TI consecutive inner indices of one outer
+// C2C along the middle axis: in is [outer][lin][ld], out is [outer][n][ld]
+// and the first `inner` of the ld inner indices are transformed (lin <= n;
+// the missing lin..n-1 inputs are zeros, the pruned transform of a
+// zero-padded line).  in == out (lin == n) is the in-place case.  A block owns TI consecutive inner indices of one outer
 // index; the tile is element-major in LDS (element e of line l at e*ti + l)
 // so both the global accesses and the butterflies are conflict-free.
 template <int DIR>
 __global__ void __launch_bounds__(TM_FFT_THREADS)
-k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner, int TI,
+k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner, int64_t ld, int TI,
               int64_t tiles_per_outer, float scale) {
   extern __shared__ float2 lds[];
   const int n = p.n;
@@ -41,18 +41,18 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
   const int ti = (int)((inner - i0) < TI ? (inner - i0) : TI);
   float2* A = lds;
   float2* B = lds + n * ti;
-  const float2* src = in + o * (int64_t)lin * inner + i0;
-  float2* dst = out + o * (int64_t)n * inner + i0;
+  const float2* src = in + o * (int64_t)lin * ld + i0;
+  float2* dst = out + o * (int64_t)n * ld + i0;
   const int tid = threadIdx.x;
   for (int f = tid; f < n * ti; f += TM_FFT_THREADS) {
     const int e = f / ti, l = f - e * ti;
-    A[f] = e < lin ? src[(int64_t)e * inner + l] : make_float2(0.f, 0.f);
+    A[f] = e < lin ? src[(int64_t)e * ld + l] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   float2* R = lds_fft_lines<DIR, true>(A, B, ti, 1, ti, p, tid, TM_FFT_THREADS);
   for (int f = tid; f < n * ti; f += TM_FFT_THREADS) {
     const int e = f / ti, l = f - e * ti;
-    dst[(int64_t)e * inner + l] = c_scale(R[f], scale);
+    dst[(int64_t)e * ld + l] = c_scale(R[f], scale);
   }
 }
 
@@ -61,51 +61,83 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
 // memory into registers, LDS only carries the inter-stage exchanges.
 template <int N, int L, int NT, int DIR>
 __global__ void __launch_bounds__(NT)
-k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner,
-         int64_t tiles_per_outer, float scale) {
+k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner, int ld,
+         int64_t tiles_per_outer, int64_t ntiles, float scale) {
   using PL = CtPlan<N, L, NT, true, false>;
+  constexpr int EI = PL::E_IN, EO = PL::E_OUT, EW = EI > EO ? EI : EO;
   extern __shared__ float2 lds[];
+  __shared__ float2 tws[N];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
-  const int64_t o = blockIdx.x / tiles_per_outer;
-  const int64_t i0 = (blockIdx.x - o * tiles_per_outer) * (int64_t)L;
-  const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
-  const float2* src = in + o * (int64_t)lin * inner + i0;
-  float2* dst = out + o * (int64_t)N * inner + i0;
-  float2 v[PL::E_IN], w[PL::E_OUT];
+  for (int i = tid; i < N; i += NT) tws[i] = tw[i];
+  // persistent blocks: tile t = blockIdx.x + k * gridDim.x; the next tile's
+  // loads are issued before the current tile's transform
+  float2 v[EW], nx[EI];
+  auto load = [&](int64_t t, float2* dstv) {
+    const int64_t o = t / tiles_per_outer;
+    const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
+    const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
+    const float2* src = in + o * (int64_t)lin * ld + i0;
 #pragma unroll
-  for (int i = 0; i < PL::E_IN / PL::R_IN; ++i)
+    for (int i = 0; i < EI / PL::R_IN; ++i)
 #pragma unroll
-    for (int r = 0; r < PL::R_IN; ++r) {
-      int l, e;
-      const bool ok = PL::in_pos(tid, i, r, l, e);
-      v[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * (int)inner + l] : make_float2(0.f, 0.f);
-    }
-  ct_fft<PL, DIR, L>(lds, tw, v, w, tid);
+      for (int r = 0; r < PL::R_IN; ++r) {
+        int l, e;
+        const bool ok = PL::in_pos(tid, i, r, l, e);
+        dstv[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * ld + l] : make_float2(0.f, 0.f);
+      }
+  };
+  int64_t t = blockIdx.x;
+  if (t >= ntiles) return;
+  load(t, nx);
+  for (; t < ntiles; t += gridDim.x) {
 #pragma unroll
-  for (int i = 0; i < PL::E_OUT / PL::R_OUT; ++i)
+    for (int s = 0; s < EI; ++s) v[s] = nx[s];
+    if (t + gridDim.x < ntiles) load(t + gridDim.x, nx);
+    ct_fft<PL, DIR, L>(lds, tws, v, v, tid);
+    const int64_t o = t / tiles_per_outer;
+    const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
+    const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
+    float2* dst = out + o * (int64_t)N * ld + i0;
 #pragma unroll
-    for (int r = 0; r < PL::R_OUT; ++r) {
-      int l, e;
-      if (PL::out_pos(tid, i, r, l, e) && l < ti) dst[e * (int)inner + l] = c_scale(w[i * PL::R_OUT + r], scale);
-    }
+    for (int i = 0; i < EO / PL::R_OUT; ++i)
+#pragma unroll
+      for (int r = 0; r < PL::R_OUT; ++r) {
+        int l, e;
+        if (PL::out_pos(tid, i, r, l, e) && l < ti) dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
+      }
+  }
+}
+
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
 }
 
 template <int N, int L, int NT>
 static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
-                         int dir, float scale, hipStream_t s, const char* name) {
+                         int64_t ld, int dir, float scale, hipStream_t s, const char* name) {
   const int64_t tpo = tm_ceil_div(inner, L);
-  const int64_t blocks = outer * tpo;
-  TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
+  const int64_t ntiles = outer * tpo;
   const size_t shm = (size_t)N * L * sizeof(float2);
+  const int per_cu = (int)(163840 / shm);  // LDS-resident blocks per CU
+  int64_t blocks = (int64_t)device_cus() * (per_cu < 1 ? 1 : per_cu);
+  if (blocks > ntiles) blocks = ntiles;
   const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
   if (dir < 0)
     TM_TIMED(name, bytes, s,
              hipLaunchKernelGGL((k_c2c_ct<N, L, NT, -1>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw,
-                                lin, inner, tpo, scale));
+                                lin, inner, (int)ld, tpo, ntiles, scale));
   else
     TM_TIMED(name, bytes, s,
              hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw,
-                                lin, inner, tpo, scale));
+                                lin, inner, (int)ld, tpo, ntiles, scale));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -113,7 +145,8 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
 // R2C along contiguous rows: in rows of lin <= n reals (stride lin,
 // zero-padded to n), out rows of n/2+1 complex (stride n/2+1).
 __global__ void __launch_bounds__(TM_FFT_THREADS)
-k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, int lin, int64_t nrows, int RL) {
+k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, int lin, int64_t nrows, int RL,
+           int64_t ldo) {
   extern __shared__ float2 lds[];
   const int n = p.n, nc = n / 2 + 1;
   const int ld = n + 1;
@@ -129,10 +162,10 @@ k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, i
   }
   __syncthreads();
   float2* R = lds_fft_lines<-1, false>(A, B, rl, ld, 1, p, tid, TM_FFT_THREADS);
-  float2* dst = out + r0 * nc;
+  float2* dst = out + r0 * ldo;
   for (int f = tid; f < nc * rl; f += TM_FFT_THREADS) {
     const int l = f / nc, e = f - l * nc;
-    dst[f] = R[l * ld + e];
+    dst[l * ldo + e] = R[l * ld + e];
   }
 }
 
@@ -218,13 +251,15 @@ static int tile_lines(int n, int64_t avail) {
 }
 
 int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
-                  int dir, float scale, hipStream_t s, const char* name) {
-  if (inner >= 12) {
+                  int64_t ld, int dir, float scale, hipStream_t s, const char* name) {
+  TM_REQUIRE(inner >= 1 && inner <= ld, "C2C: %lld inner lines at stride %lld", (long long)inner, (long long)ld);
+  // the specialised kernels address a tile with 32-bit offsets
+  if (inner >= 12 && (int64_t)lp.n * ld < (1LL << 31)) {
     switch (lp.n) {
-      case 128: return launch_c2c_ct<128, 16, 256>(lp, in, lin, out, outer, inner, dir, scale, s, name);
-      case 256: return launch_c2c_ct<256, 16, 256>(lp, in, lin, out, outer, inner, dir, scale, s, name);
-      case 512: return launch_c2c_ct<512, 16, 512>(lp, in, lin, out, outer, inner, dir, scale, s, name);
-      case 1024: return launch_c2c_ct<1024, 8, 512>(lp, in, lin, out, outer, inner, dir, scale, s, name);
+      case 128: return launch_c2c_ct<128, 16, 256>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+      case 256: return launch_c2c_ct<256, 16, 256>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+      case 512: return launch_c2c_ct<512, 16, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+      case 1024: return launch_c2c_ct<1024, 8, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
       default: break;
     }
   }
@@ -237,11 +272,11 @@ int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, in
   if (dir < 0)
     TM_TIMED(name, bytes, s,
              hipLaunchKernelGGL(k_c2c_strided<-1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out,
-                                lp, lin, inner, TI, tpo, scale));
+                                lp, lin, inner, ld, TI, tpo, scale));
   else
     TM_TIMED(name, bytes, s,
              hipLaunchKernelGGL(k_c2c_strided<1>, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out,
-                                lp, lin, inner, TI, tpo, scale));
+                                lp, lin, inner, ld, TI, tpo, scale));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -254,15 +289,16 @@ static int rows_per_block(int n, int64_t nrows) {
   return r;
 }
 
-int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, hipStream_t s,
-                       const char* name) {
+int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
+                       hipStream_t s, const char* name) {
+  TM_REQUIRE(ldo >= lp.n / 2 + 1, "R2C: output row stride %lld below %d", (long long)ldo, lp.n / 2 + 1);
   const int RL = rows_per_block(lp.n, nrows);
   const size_t shm = 2 * (size_t)RL * (lp.n + 1) * sizeof(float2);
   const int64_t blocks = tm_ceil_div(nrows, RL);
   TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
   TM_TIMED(name, (4.0 * lin + 8.0 * (lp.n / 2 + 1)) * (double)nrows, s,
            hipLaunchKernelGGL(k_r2c_rows, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out, lp, lin,
-                              nrows, RL));
+                              nrows, RL, ldo));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -285,13 +321,14 @@ int tm_fft_r2c_batch(const tm_fft_plan* p, int64_t batch, const float* in, float
   int64_t nrows = batch;
   for (int i = 0; i < nd - 1; ++i) nrows *= p->shape[i];
   const LinePlan& last = p->lines[nd - 1].lp;
-  int rc = tm_launch_r2c_rows(last, in, last.n, out, nrows, s, "fft_r2c_rows");
+  int rc = tm_launch_r2c_rows(last, in, last.n, out, nrows, last.n / 2 + 1, s, "fft_r2c_rows");
   if (rc != TM_OK) return rc;
   for (int ax = nd - 2; ax >= 0; --ax) {
     int64_t outer = batch, inner = 1;
     for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
     for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
-    rc = tm_launch_c2c(p->lines[ax].lp, out, p->lines[ax].lp.n, out, outer, inner, -1, 1.0f, s, "fft_c2c_axis");
+    rc = tm_launch_c2c(p->lines[ax].lp, out, p->lines[ax].lp.n, out, outer, inner, inner, -1, 1.0f, s,
+                       "fft_c2c_axis");
     if (rc != TM_OK) return rc;
   }
   return TM_OK;
@@ -305,7 +342,8 @@ int tm_fft_c2r_batch(const tm_fft_plan* p, int64_t batch, float2* in, float* out
     int64_t outer = batch, inner = 1;
     for (int i = 0; i < ax; ++i) outer *= p->cshape[i];
     for (int i = ax + 1; i < nd; ++i) inner *= p->cshape[i];
-    int rc = tm_launch_c2c(p->lines[ax].lp, in, p->lines[ax].lp.n, in, outer, inner, 1, 1.0f, s, "fft_c2c_axis");
+    int rc = tm_launch_c2c(p->lines[ax].lp, in, p->lines[ax].lp.n, in, outer, inner, inner, 1, 1.0f, s,
+                           "fft_c2c_axis");
     if (rc != TM_OK) return rc;
   }
   int64_t nrows = batch;

@@ -18,6 +18,14 @@
 #define TM_MAX_STAGES 14
 #define TM_MAX_GENERIC_RADIX 13
 
+// Workgroup barrier ordering LDS only: waits for this wave's LDS operations,
+// not for its outstanding global loads and stores, so prefetched loads and
+// earlier stores stay in flight across the barrier.  (__syncthreads() also
+// drains vector memory.)
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 struct LinePlan {
   int n;
   int nst;

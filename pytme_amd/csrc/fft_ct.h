@@ -150,12 +150,10 @@ __device__ __forceinline__ float2 ct_tw(const float2* __restrict__ tw, int idx) 
   return tw_load<DIR>(tw, idx * TWS);
 }
 
-// v[r] *= W^(r*base) for r = 1..R-1: one table load, powers by recurrence
-// (at most R-2 = 6 extra roundings for radix 8, ~4e-7 relative).
-template <int DIR, int TWS, int R>
-__device__ __forceinline__ void ct_twiddle(float2* v, const float2* __restrict__ tw, int base) {
-  if (base == 0) return;
-  const float2 w1 = ct_tw<DIR, TWS>(tw, base);
+// v[r] *= w1^r for r = 1..R-1, powers by recurrence (at most R-2 = 6 extra
+// roundings for radix 8, ~4e-7 relative).
+template <int R>
+__device__ __forceinline__ void ct_twiddle_w(float2* v, float2 w1) {
   float2 w = w1;
 #pragma unroll
   for (int r = 1; r < R; ++r) {
@@ -164,34 +162,45 @@ __device__ __forceinline__ void ct_twiddle(float2* v, const float2* __restrict__
   }
 }
 
+// v[r] *= W^(r*base) for r = 1..R-1 with W^base from the table.
+template <int DIR, int TWS, int R>
+__device__ __forceinline__ void ct_twiddle(float2* v, const float2* __restrict__ tw, int base) {
+  if (base == 0) return;
+  ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, base));
+}
+
 // middle stage s (reads LDS, writes LDS)
 template <class P, int DIR, int LD, int TWS, int S>
 __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid) {
   if constexpr (S < P::NST - 1) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
-    float2 v[BPT][R];
+    // the twiddle is read with the data, before the barrier: the table is
+    // never written during the transform
+    float2 v[BPT][R], w1[BPT];
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
       int l, j;
       if (P::template slot<R>(tid, i, l, j)) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
+        const int k = j % NS;
+        if (k != 0) w1[i] = ct_tw<DIR, TWS>(tw, k * TS);
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
       int l, j;
       if (P::template slot<R>(tid, i, l, j)) {
         const int k = j % NS;
-        ct_twiddle<DIR, TWS, R>(v[i], tw, k * TS);
+        if (k != 0) ct_twiddle_w<R>(v[i], w1[i]);
         ct_dft<DIR, R>(v[i]);
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) lds[ct_lds<P, LD>(l, base + r * NS)] = v[i][r];
       }
     }
-    __syncthreads();
+    lds_barrier();
     ct_mid<P, DIR, LD, TWS, S + 1>(lds, tw, tid);
   }
 }
@@ -214,7 +223,7 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
     }
     return;
   } else {
-    __syncthreads();  // the buffer may still be read by a previous transform
+    lds_barrier();  // the buffer may still be read by a previous transform
 #pragma unroll
     for (int i = 0; i < B0; ++i) {
       int l, j;
@@ -224,7 +233,7 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
         for (int r = 0; r < R0; ++r) lds[ct_lds<P, LD>(l, j * R0 + r)] = in[i * R0 + r];
       }
     }
-    __syncthreads();
+    lds_barrier();
     ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid);
     constexpr int S = P::NST - 1;
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);

@@ -20,13 +20,15 @@ struct tm_fft_plan {
   tm_line lines[3];
 };
 
-// C2C along the middle axis of [outer][lin][inner] -> [outer][n][inner]
-// (lin <= n zero-padded; dir -1 forward, +1 inverse unnormalised).
+// C2C along the middle axis of [outer][lin][ld] -> [outer][n][ld] for the
+// first `inner` <= ld inner indices (lin <= n zero-padded; dir -1 forward,
+// +1 inverse unnormalised).  Inner indices inner..ld-1 are not touched.
 int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
-                  int dir, float scale, hipStream_t s, const char* name);
-// R2C of nrows contiguous rows of lin <= n reals (zero-padded to n).
-int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, hipStream_t s,
-                       const char* name);
+                  int64_t ld, int dir, float scale, hipStream_t s, const char* name);
+// R2C of nrows contiguous rows of lin <= n reals (zero-padded to n) into
+// rows of n/2+1 bins at a row stride of ldo >= n/2+1 complex.
+int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
+                       hipStream_t s, const char* name);
 // C2R of nrows contiguous half-spectrum rows, times scale.
 int tm_launch_c2r_rows(const LinePlan& lp, const float2* in, float* out, int64_t nrows, float scale, hipStream_t s,
                        const char* name);

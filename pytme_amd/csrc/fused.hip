@@ -113,7 +113,7 @@ __global__ void k_normalize_batched(float* __restrict__ t, const float* __restri
 template <int NQ>
 __global__ void __launch_bounds__(TM_FU_THREADS)
 k_pass_a(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
-         float2* __restrict__ Q, LinePlan lp, int T0, int N1, int N2c, int B, int TI, int tiles_per_row) {
+         float2* __restrict__ Q, LinePlan lp, int T0, int N1, int N2c, int N2s, int B, int TI, int tiles_per_row) {
   extern __shared__ float2 lds[];
   const int N0 = lp.n;
   const int tid = threadIdx.x;
@@ -121,8 +121,8 @@ k_pass_a(const float2* __restrict__ P, const float2* __restrict__ Ft, const floa
   const int kx0 = (blockIdx.x - ky * tiles_per_row) * TI;
   const int ti = (N2c - kx0) < TI ? (N2c - kx0) : TI;
   const int npos = N0 * ti;
-  const int64_t plane = (int64_t)N1 * N2c;
-  const int64_t col = (int64_t)ky * N2c + kx0;
+  const int64_t plane = (int64_t)N1 * N2s;
+  const int64_t col = (int64_t)ky * N2s + kx0;
   float2* A = lds;
   float2* Bf = lds + npos;
 
@@ -215,7 +215,8 @@ __device__ __forceinline__ float fu_flc_norm(float x, float ex2, float ex, float
 
 template <int NQ>
 __global__ void __launch_bounds__(TM_FU_THREADS)
-k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int RL, int B, const double* __restrict__ n_obs,
+k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, int RL, int B,
+         const double* __restrict__ n_obs,
          const float* __restrict__ inv_den, const int* __restrict__ rot_idx, float scale, float eps,
          float* __restrict__ max_scores, int* __restrict__ max_rot) {
   extern __shared__ float2 lds[];
@@ -239,10 +240,10 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int RL, int B
   for (int b = 0; b < B; ++b) {
 #pragma unroll 1
     for (int o = 0; o < NQ; ++o) {
-      const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * N2c;
+      const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;
       for (int f = tid; f < N2c * rl; f += TM_FU_THREADS) {
         const int l = f / N2c, e = f - l * N2c;
-        const float2 v = src[f];
+        const float2 v = src[(int64_t)l * ldq + e];
         A[l * ld + e] = v;
         if (e > 0 && N2 - e >= N2c) A[l * ld + (N2 - e)] = c_conj(v);
       }
@@ -301,19 +302,21 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int RL, int B
 template <int N0, int L, int NT, int KIND>
 __global__ void __launch_bounds__(NT)
 k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
-            float2* __restrict__ Q, const float2* __restrict__ tw, int T0, int N1, int N2c, int B, int tiles_per_row,
-            int q_in, int o_x) {
+            float2* __restrict__ Q, const float2* __restrict__ tw, int T0, int N1, int N2c, int N2s, int B,
+            int tiles_per_row, int q_in, int o_x) {
   using PF = CtPlan<N0, L, NT, true, false>;
   using PI = CtPlan<N0, L, NT, true, true>;
   static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
   constexpr int EI = PF::E_IN, EO = PF::E_OUT, EW = EI > EO ? EI : EO;
   extern __shared__ float2 lds[];
+  __shared__ float2 tws[N0];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
+  for (int i = tid; i < N0; i += NT) tws[i] = tw[i];
   const int ky = blockIdx.x / tiles_per_row;
   const int kx0 = (blockIdx.x - ky * tiles_per_row) * L;
   const int ti = (N2c - kx0) < L ? (N2c - kx0) : L;
-  const int plane = N1 * N2c;  // per-slab offsets fit in 32 bits (< 2^31 / 8 elements)
-  const int64_t col = (int64_t)ky * N2c + kx0;
+  const int plane = N1 * N2s;  // per-slab offsets fit in 32 bits (checked by the launcher)
+  const int64_t col = (int64_t)ky * N2s + kx0;
   const float2* Ftc = Ft + col;
   const float2* Ft2c = KIND == 0 ? Ft2 + col : nullptr;
 
@@ -330,7 +333,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
     }
 
   auto inv_store = [&](int o, int b, float2* w) {
-    ct_fft<PI, 1, L>(lds, tw, w, w, tid);
+    ct_fft<PI, 1, L>(lds, tws, w, w, tid);
     float2* dst = Q + ((int64_t)(o * B + b) * N0) * (int64_t)plane + col;
 #pragma unroll
     for (int i = 0; i < PI::E_OUT / PI::R_OUT; ++i)
@@ -352,7 +355,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
         const bool ok = PF::in_pos(tid, i, r, l, e) && l < ti && e < T0;
         w[i * PF::R_IN + r] = ok ? src[e * plane + l] : make_float2(0.f, 0.f);
       }
-    ct_fft<PF, -1, L>(lds, tw, w, w, tid);
+    ct_fft<PF, -1, L>(lds, tws, w, w, tid);
     if constexpr (KIND == 1) {
 #pragma unroll
       for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
@@ -376,16 +379,21 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
 // (imaginary parts of the DC and Nyquist bins dropped, as numpy's irfft).
 template <int M, int RL, int NT, int NQ>
 __global__ void __launch_bounds__(NT)
-k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_t nrows, int B,
+k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_t nrows, int ldq, int B,
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
             float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
   using PI = CtPlan<M, RL, NT, false, true>;
   constexpr int N2 = 2 * M, N2c = M + 1, LD = M + 1;
-  constexpr int EI = PI::E_IN, EO = PI::E_OUT;
+  constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
+  constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread
   extern __shared__ float2 lds[];
+  __shared__ float2 tws[N2];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
+  for (int i = tid; i < N2; i += NT) tws[i] = tw2[i];
   const int64_t r0 = (int64_t)blockIdx.x * RL;
   const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
+  const int tile_n = rl * N2c;
+  const int iters = B * NQ;
 
   float best0[EO], best1[EO], a1[EO], b1[EO], a2[EO], b2[EO];
   int arg0[EO], arg1[EO];
@@ -395,77 +403,92 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
     arg0[s] = arg1[s] = -1;
     a1[s] = b1[s] = a2[s] = b2[s] = 0.f;
   }
-  for (int b = 0; b < B; ++b) {
-    const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
-#pragma unroll 1
-    for (int o = 0; o < NQ; ++o) {
-      const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * N2c;
-      __syncthreads();
-      for (int f = tid; f < N2c * rl; f += NT) {
-        const int l = f / N2c, e = f - l * N2c;
-        lds[l * LD + e] = src[f];
-      }
-      __syncthreads();
-      float2 z[EI > EO ? EI : EO];
-      float2* y = z;
+  // rows of slab (o, b) are consecutive at stride ldq: one tile = rl rows
+  float2 pf[NPF];
+  auto prefetch = [&](int it) {
+    const int b = it / NQ, o = it - (it / NQ) * NQ;
+    const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;
 #pragma unroll
-      for (int i = 0; i < EI / PI::R_IN; ++i)
-#pragma unroll
-        for (int r = 0; r < PI::R_IN; ++r) {
-          int l, k;
-          if (!PI::in_pos(tid, i, r, l, k)) {
-            z[i * PI::R_IN + r] = make_float2(0.f, 0.f);
-            continue;
-          }
-          float2 A = lds[l * LD + k];
-          float2 C = lds[l * LD + (M - k)];
-          if (k == 0) {
-            A.y = 0.f;
-            C.y = 0.f;
-          }
-          const float2 E = make_float2(A.x + C.x, A.y - C.y);
-          const float2 D = c_mul(make_float2(A.x - C.x, A.y + C.y), tw_load<1>(tw2, k));
-          z[i * PI::R_IN + r] = make_float2(E.x - D.y, E.y + D.x);
-        }
-      ct_fft<PI, 1, LD, 2>(lds, tw2, z, z, tid);
-#pragma unroll
-      for (int i = 0; i < EO / PI::R_OUT; ++i)
-#pragma unroll
-        for (int r = 0; r < PI::R_OUT; ++r) {
-          int l, m;
-          const int s = i * PI::R_OUT + r;
-          if (!PI::out_pos(tid, i, r, l, m) || l >= rl) continue;
-          const float x0 = y[s].x * scale, x1 = y[s].y * scale;
-          if (o == NQ - 1) {
-            float s0, s1;
-            if (NQ == 3) {
-              s0 = fu_flc_norm(x0, a2[s], a1[s], nf, eps);
-              s1 = fu_flc_norm(x1, b2[s], b1[s], nf, eps);
-            } else if (inv_den) {
-              const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + l) * N2 + 2 * m);
-              s0 = x0 * d.x;
-              s1 = x1 * d.y;
-            } else {
-              s0 = x0;
-              s1 = x1;
-            }
-            if (s0 > best0[s]) {
-              best0[s] = s0;
-              arg0[s] = rot_idx[b];
-            }
-            if (s1 > best1[s]) {
-              best1[s] = s1;
-              arg1[s] = rot_idx[b];
-            }
-          } else if (o == 0) {
-            a1[s] = x0;
-            b1[s] = x1;
-          } else {
-            a2[s] = x0;
-            b2[s] = x1;
-          }
-        }
+    for (int p = 0; p < NPF; ++p) {
+      const int f = tid + p * NT;
+      const int l = f / N2c, e = f - l * N2c;
+      pf[p] = f < tile_n ? src[l * ldq + e] : make_float2(0.f, 0.f);
     }
+  };
+  prefetch(0);
+  for (int it = 0; it < iters; ++it) {
+    const int b = it / NQ, o = it - b * NQ;
+    lds_barrier();  // the previous transform has finished reading the tile
+#pragma unroll
+    for (int p = 0; p < NPF; ++p) {
+      const int f = tid + p * NT;
+      if (f < tile_n) {
+        const int l = f / N2c, e = f - l * N2c;
+        lds[l * LD + e] = pf[p];
+      }
+    }
+    if (it + 1 < iters) prefetch(it + 1);
+    lds_barrier();
+    float2 z[EW];
+#pragma unroll
+    for (int i = 0; i < EI / PI::R_IN; ++i)
+#pragma unroll
+      for (int r = 0; r < PI::R_IN; ++r) {
+        int l, k;
+        if (!PI::in_pos(tid, i, r, l, k)) {
+          z[i * PI::R_IN + r] = make_float2(0.f, 0.f);
+          continue;
+        }
+        float2 A = lds[l * LD + k];
+        float2 C = lds[l * LD + (M - k)];
+        if (k == 0) {
+          A.y = 0.f;
+          C.y = 0.f;
+        }
+        const float2 E = make_float2(A.x + C.x, A.y - C.y);
+        const float2 D = c_mul(make_float2(A.x - C.x, A.y + C.y), tw_load<1>(tws, k));
+        z[i * PI::R_IN + r] = make_float2(E.x - D.y, E.y + D.x);
+      }
+    ct_fft<PI, 1, LD, 2>(lds, tws, z, z, tid);
+    const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
+    const int ridx = rot_idx[b];
+#pragma unroll
+    for (int i = 0; i < EO / PI::R_OUT; ++i)
+#pragma unroll
+      for (int r = 0; r < PI::R_OUT; ++r) {
+        int l, m;
+        const int s = i * PI::R_OUT + r;
+        if (!PI::out_pos(tid, i, r, l, m) || l >= rl) continue;
+        const float x0 = z[s].x * scale, x1 = z[s].y * scale;
+        if (o == NQ - 1) {
+          float s0, s1;
+          if (NQ == 3) {
+            s0 = fu_flc_norm(x0, a2[s], a1[s], nf, eps);
+            s1 = fu_flc_norm(x1, b2[s], b1[s], nf, eps);
+          } else if (inv_den) {
+            const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + l) * N2 + 2 * m);
+            s0 = x0 * d.x;
+            s1 = x1 * d.y;
+          } else {
+            s0 = x0;
+            s1 = x1;
+          }
+          if (s0 > best0[s]) {
+            best0[s] = s0;
+            arg0[s] = ridx;
+          }
+          if (s1 > best1[s]) {
+            best1[s] = s1;
+            arg1[s] = ridx;
+          }
+        } else if (o == 0) {
+          a1[s] = x0;
+          b1[s] = x1;
+        } else {
+          a2[s] = x0;
+          b2[s] = x1;
+        }
+      }
   }
 #pragma unroll
   for (int i = 0; i < EO / PI::R_OUT; ++i)
@@ -497,25 +520,27 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
 
 template <int N0, int L, int NT>
 static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
-                            const float2* tw, int T0, int N1, int N2c, int B, double bytes, hipStream_t s) {
+                            const float2* tw, int T0, int N1, int N2c, int N2s, int B, double bytes, hipStream_t s) {
+  TM_REQUIRE((int64_t)N0 * N1 * N2s < (1LL << 31), "pass A: z column stride too large");
   const int tpr = (int)tm_ceil_div(N2c, L);
   const int64_t blocks = (int64_t)N1 * tpr;
   const size_t shm = (size_t)N0 * L * sizeof(float2);
   if (flc) {
     TM_TIMED("fused_pass_a", bytes * 0.6, s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 0>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft, ft2, Q,
-                                tw, T0, N1, N2c, B, tpr, 0, 0));
+                                tw, T0, N1, N2c, N2s, B, tpr, 0, 0));
     TM_CHECK_LAUNCH();
   }
   TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", flc ? bytes * 0.4 : bytes, s,
            hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
-                              (const float2*)nullptr, Q, tw, T0, N1, N2c, B, tpr, flc ? 1 : 0, flc ? 2 : 0));
+                              (const float2*)nullptr, Q, tw, T0, N1, N2c, N2s, B, tpr, flc ? 1 : 0, flc ? 2 : 0));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
 
 template <int M, int RL, int NT>
-static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_t nrows, int B, const double* n_obs,
+static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_t nrows, int ldq, int B,
+                            const double* n_obs,
                             const float* inv_den, const int* rot_idx, float scale, float eps, float* max_scores,
                             int* max_rot, double bytes, hipStream_t s) {
   const int64_t blocks = tm_ceil_div(nrows, RL);
@@ -523,11 +548,11 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_
   if (flc)
     TM_TIMED("fused_pass_c", bytes, s,
              hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2, nrows,
-                                B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
+                                ldq, B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
   else
     TM_TIMED("fused_pass_c", bytes, s,
              hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2, nrows,
-                                B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores, max_rot));
+                                ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores, max_rot));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -538,7 +563,7 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_
 struct tm_scan_plan {
   int mode;  // 0 FLC, 1 CORR
   int nq_in, nq_out;
-  int N[3], N2c;
+  int N[3], N2c, N2s;  // N2s: row stride of the workspace spectra (N2c rounded up to 16 bins = 128 B)
   int T[3], T2c;
   int max_batch;
   int normalize;
@@ -546,8 +571,9 @@ struct tm_scan_plan {
   const float* templ;
   const float* mask;
   const float* filter;
-  const float2* ft;
+  const float2* ft;   // target spectra at row stride N2s (the caller's or padded copies)
   const float2* ft2;
+  float2 *ftp, *ft2p;  // padded copies, owned by the plan
   const float* inv_den;
   tm_fft_plan* fast;   // line plans of N0, N1, N2
   tm_fft_plan* tplan;  // template-size plan (filter)
@@ -562,7 +588,7 @@ extern "C" int tm_fft_plan_destroy(tm_fft_plan* p);
 
 static void scan_plan_free(tm_scan_plan* p) {
   if (!p) return;
-  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->n_obs, p->rot_ws, p->mom_ws, p->ms};
+  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->n_obs, p->rot_ws, p->mom_ws, p->ms, p->ftp, p->ft2p};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (p->fast) tm_fft_plan_destroy(p->fast);
@@ -580,19 +606,42 @@ static void scan_plan_free(tm_scan_plan* p) {
     }                                                      \
   } while (0)
 
+// Workspace spectra rows are padded to a multiple of 16 bins (128 B), so
+// every 16-bin column tile of the z and y passes is one aligned 128-byte
+// segment per row.
+static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
+
+// Copy the caller's target spectra (rows of N2c bins) into the padded copies.
+static int scan_copy_target(tm_scan_plan* p, const void* ft, const void* ft2, hipStream_t s) {
+  if (p->N2s == p->N2c) {
+    p->ft = (const float2*)ft;
+    p->ft2 = (const float2*)ft2;
+    return TM_OK;
+  }
+  const size_t rows = (size_t)p->N[0] * p->N[1];
+  const size_t spitch = sizeof(float2) * p->N2c, dpitch = sizeof(float2) * p->N2s;
+  TM_CHECK_HIP(hipMemcpy2DAsync(p->ftp, dpitch, ft, spitch, spitch, rows, hipMemcpyDeviceToDevice, s));
+  if (p->mode == 0)
+    TM_CHECK_HIP(hipMemcpy2DAsync(p->ft2p, dpitch, ft2, spitch, spitch, rows, hipMemcpyDeviceToDevice, s));
+  p->ft = p->ftp;
+  p->ft2 = p->mode == 0 ? p->ft2p : nullptr;
+  return TM_OK;
+}
+
 extern "C" {
 
 int64_t tm_scan_workspace_bytes(int mode, const int64_t* fast_shape, const int* tshape, int max_batch,
                                 int with_filter) {
-  const int64_t N0 = fast_shape[0], N1 = fast_shape[1], N2c = fast_shape[2] / 2 + 1;
+  const int64_t N0 = fast_shape[0], N1 = fast_shape[1], N2c = fast_shape[2] / 2 + 1, N2s = pad_bins(N2c);
   const int64_t T0 = tshape[0], T1 = tshape[1], T2 = tshape[2];
   const int64_t nq_in = mode == 0 ? 2 : 1, nq_out = mode == 0 ? 3 : 1, B = max_batch;
   int64_t bytes = 0;
   bytes += 4 * B * T0 * T1 * T2 * (mode == 0 ? 2 : 1);
   if (with_filter) bytes += 8 * B * T0 * T1 * (T2 / 2 + 1);
-  bytes += 8 * nq_in * B * T0 * T1 * N2c;
-  bytes += 8 * nq_in * B * T0 * N1 * N2c;
-  bytes += 8 * nq_out * B * N0 * N1 * N2c;
+  bytes += 8 * nq_in * B * T0 * T1 * N2s;
+  bytes += 8 * nq_in * B * T0 * N1 * N2s;
+  bytes += 8 * nq_out * B * N0 * N1 * N2s;
+  if (N2s != N2c) bytes += 8 * (mode == 0 ? 2 : 1) * N0 * N1 * N2s;  // padded target spectra
   return bytes;
 }
 
@@ -622,6 +671,7 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
     p->T[i] = tshape[i];
   }
   p->N2c = p->N[2] / 2 + 1;
+  p->N2s = (int)pad_bins(p->N2c);
   p->T2c = p->T[2] / 2 + 1;
   p->max_batch = max_batch;
   p->normalize = mode == 0 ? 1 : normalize;
@@ -646,19 +696,43 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
     }
   }
   const int64_t B = max_batch, T3 = (int64_t)p->T[0] * p->T[1] * p->T[2];
-  const int64_t N0 = p->N[0], N1 = p->N[1], N2c = p->N2c, T0 = p->T[0], T1 = p->T[1];
+  const int64_t N0 = p->N[0], N1 = p->N[1], N2s = p->N2s, T0 = p->T[0], T1 = p->T[1];
   TM_ALLOC(p->TR, sizeof(float) * B * T3);
   if (mode == 0) TM_ALLOC(p->MR, sizeof(float) * B * T3);
   if (filter) TM_ALLOC(p->specT, sizeof(float2) * B * T0 * T1 * p->T2c);
-  TM_ALLOC(p->Q1, sizeof(float2) * p->nq_in * B * T0 * T1 * N2c);
-  TM_ALLOC(p->P, sizeof(float2) * p->nq_in * B * T0 * N1 * N2c);
-  TM_ALLOC(p->Q, sizeof(float2) * p->nq_out * B * N0 * N1 * N2c);
+  TM_ALLOC(p->Q1, sizeof(float2) * p->nq_in * B * T0 * T1 * N2s);
+  TM_ALLOC(p->P, sizeof(float2) * p->nq_in * B * T0 * N1 * N2s);
+  TM_ALLOC(p->Q, sizeof(float2) * p->nq_out * B * N0 * N1 * N2s);
+  if (p->N2s != p->N2c) {
+    TM_ALLOC(p->ftp, sizeof(float2) * N0 * N1 * N2s);
+    if (mode == 0) TM_ALLOC(p->ft2p, sizeof(float2) * N0 * N1 * N2s);
+  }
   TM_ALLOC(p->n_obs, sizeof(double) * B);
   TM_ALLOC(p->rot_ws, sizeof(double) * (tm_ceil_div(T3, 256) * B + 8));
   TM_ALLOC(p->mom_ws, sizeof(double) * 2 * TM_MOM_BLOCKS * B);
   TM_ALLOC(p->ms, sizeof(double) * 2 * B);
+  // the spectra may still be in flight on the caller's stream
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) {
+    rc = scan_copy_target(p, ft_target, ft_target2, nullptr);
+    if (rc == TM_OK) e = hipStreamSynchronize(nullptr);
+  }
+  if (e != hipSuccess || rc != TM_OK) {
+    if (e != hipSuccess) tm_set_error("copying the target spectra: %s", hipGetErrorString(e));
+    scan_plan_free(p);
+    return rc != TM_OK ? rc : TM_ERR_HIP;
+  }
   *out = p;
   return TM_OK;
+}
+
+// Refresh the plan's copy of the target spectra (same shapes as at
+// creation), ordered on `stream`.  Needed whenever the caller's spectra
+// change while the plan is kept.
+int tm_scan_plan_set_target(tm_scan_plan* p, const void* ft_target, const void* ft_target2, void* stream) {
+  TM_REQUIRE(p && ft_target, "null argument to tm_scan_plan_set_target");
+  TM_REQUIRE(p->mode == 1 || ft_target2, "FLC needs the squared-target spectrum");
+  return scan_copy_target(p, ft_target, ft_target2, (hipStream_t)stream);
 }
 
 int tm_scan_plan_destroy(tm_scan_plan* p) {
@@ -677,7 +751,7 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   const int B = nbatch;
   const int T0 = p->T[0], T1 = p->T[1], T2 = p->T[2];
   const int64_t T3 = (int64_t)T0 * T1 * T2;
-  const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2], N2c = p->N2c;
+  const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2], N2c = p->N2c, N2s = p->N2s;
   const bool flc = p->mode == 0;
   int rc;
 
@@ -717,14 +791,15 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   const LinePlan& lx = p->fast->lines[2].lp;
   const LinePlan& ly = p->fast->lines[1].lp;
   const LinePlan& lz = p->fast->lines[0].lp;
-  const int64_t q1_stride = (int64_t)B * T0 * T1 * N2c;
+  const int64_t q1_stride = (int64_t)B * T0 * T1 * N2s;
   if (flc) {
-    rc = tm_launch_r2c_rows(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, s, "fused_x_pass");
+    rc = tm_launch_r2c_rows(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, N2s, s, "fused_x_pass");
     if (rc != TM_OK) return rc;
   }
-  rc = tm_launch_r2c_rows(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, s, "fused_x_pass");
+  rc = tm_launch_r2c_rows(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, N2s, s,
+                          "fused_x_pass");
   if (rc != TM_OK) return rc;
-  rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, -1, 1.0f, s, "fused_y_pass");
+  rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, N2s, -1, 1.0f, s, "fused_y_pass");
   if (rc != TM_OK) return rc;
 
   // 5. pass A: z-axis, spectral products
@@ -733,10 +808,10 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
                          8.0 * C_ * p->nq_out * B;
   bool done_a = true;
   switch (N0) {
-    case 128: rc = launch_pass_a_ct<128, 16, 256>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
-    case 256: rc = launch_pass_a_ct<256, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
-    case 512: rc = launch_pass_a_ct<512, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
-    case 1024: rc = launch_pass_a_ct<1024, 8, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, B, bytes_a, s); break;
+    case 128: rc = launch_pass_a_ct<128, 16, 256>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
+    case 256: rc = launch_pass_a_ct<256, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
+    case 512: rc = launch_pass_a_ct<512, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
+    case 1024: rc = launch_pass_a_ct<1024, 8, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
     default: done_a = false; break;
   }
   if (done_a && rc != TM_OK) return rc;
@@ -755,16 +830,16 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
     if (flc)
       TM_TIMED("fused_pass_a", bytes, s,
                hipLaunchKernelGGL(k_pass_a<3>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->P, p->ft,
-                                  p->ft2, p->Q, lz, T0, N1, N2c, B, TI, tpr));
+                                  p->ft2, p->Q, lz, T0, N1, N2c, N2s, B, TI, tpr));
     else
       TM_TIMED("fused_pass_a", bytes, s,
                hipLaunchKernelGGL(k_pass_a<1>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->P, p->ft,
-                                  (const float2*)nullptr, p->Q, lz, T0, N1, N2c, B, TI, tpr));
+                                  (const float2*)nullptr, p->Q, lz, T0, N1, N2c, N2s, B, TI, tpr));
     TM_CHECK_LAUNCH();
   }
 
   // 6. pass B: inverse y in place
-  rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, 1, 1.0f, s, "fused_pass_b");
+  rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, N2s, 1, 1.0f, s, "fused_pass_b");
   if (rc != TM_OK) return rc;
 
   // 7. pass C: inverse x, score, max/argmax
@@ -777,9 +852,9 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
     const float eps = 1.1920928955078125e-07f;
     done_c = true;
     switch (N2) {
-      case 128: rc = launch_pass_c_ct<64, 32, 256>(flc, p->Q, lx.tw, nrows, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 200: rc = launch_pass_c_ct<100, 32, 512>(flc, p->Q, lx.tw, nrows, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 256: rc = launch_pass_c_ct<128, 32, 512>(flc, p->Q, lx.tw, nrows, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 128: rc = launch_pass_c_ct<64, 16, 256>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 200: rc = launch_pass_c_ct<100, 16, 256>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 256: rc = launch_pass_c_ct<128, 16, 256>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
       default: done_c = false; break;
     }
     if (done_c && rc != TM_OK) return rc;
@@ -799,12 +874,12 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
     if (flc)
       TM_TIMED("fused_pass_c", bytes, s,
                hipLaunchKernelGGL(k_pass_c<3>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->Q, lx, nrows,
-                                  RL, B, (const double*)p->n_obs, (const float*)nullptr, rot_index_dev, scale, eps,
+                                  (int64_t)N2s, RL, B, (const double*)p->n_obs, (const float*)nullptr, rot_index_dev, scale, eps,
                                   max_scores, max_rot));
     else
       TM_TIMED("fused_pass_c", bytes, s,
                hipLaunchKernelGGL(k_pass_c<1>, dim3((unsigned)blocks), dim3(TM_FU_THREADS), shm, s, p->Q, lx, nrows,
-                                  RL, B, (const double*)nullptr, p->inv_den, rot_index_dev, scale, eps, max_scores,
+                                  (int64_t)N2s, RL, B, (const double*)nullptr, p->inv_den, rot_index_dev, scale, eps, max_scores,
                                   max_rot));
     TM_CHECK_LAUNCH();
   }

@@ -160,3 +160,55 @@ def test_fused_matches_plain_full_size():
     assert np.abs(s0 - s1).max() < 1e-5
     agree = (r0 == r1).mean()
     assert agree > 0.9999, agree
+
+
+@pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
+def test_kept_plan_sees_new_target_values(score):
+    """The fused plan is kept across calls with the same arrays; it holds a
+    row-padded copy of the target spectra, so new values written into the
+    caller's arrays at the same addresses must reach the next call."""
+    import torch
+
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+    from pytme_amd.backends import backend as be
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+
+    be.change_backend("hip", device="cuda", fused=True)
+    target_a, template, mask, rots = _workload(seed=41, n_rot=6)
+    target_b, _, _, _ = _workload(seed=42, n_rot=6)
+    if score == "FLCSphericalMask":
+        from pytme_amd.synthetic import spherical_mask
+
+        mask = spherical_mask(12, 5)
+    setup_fn, scoring = MATCHING_EXHAUSTIVE_REGISTER[score]
+
+    def prepare(target):
+        md = MatchingData(target.copy(), template.copy(), template_mask=mask.copy(), rotations=rots)
+        setup, cb_args, chunks = prepare_scan(md, setup_fn, n_jobs=1, callback_class_args={"score_threshold": -1.0},
+                                              pad_target=False, pad_template_filter=False, interpolation_order=1)
+        return setup, cb_args, chunks[0]
+
+    def run(setup, cb_args, chunk):
+        proxy = SharedAnalyzerProxy(MaxScoreOverRotations, cb_args)
+        scoring(rotations=chunk, callback=proxy, **setup)
+        s, _, r, _ = proxy.result(**cb_args)
+        return be.to_numpy_array(s).copy(), be.to_numpy_array(r).copy()
+
+    setup_a, cb_a, chunk_a = prepare(target_a)
+    setup_b, cb_b, chunk_b = prepare(target_b)
+    s_b, r_b = run(setup_b, cb_b, chunk_b)
+    run(setup_a, cb_a, chunk_a)  # creates (or re-keys) the plan on target a's arrays
+    copied = 0
+    for key in setup_a:
+        if key.startswith("ft_target") or key == "inv_denominator":
+            dst, src = be.from_sharedarr(setup_a[key]), be.from_sharedarr(setup_b[key])
+            assert torch.is_tensor(dst) and dst.shape == src.shape
+            dst.copy_(src)
+            copied += 1
+    assert copied == 2
+    torch.cuda.synchronize()
+    s_a2, r_a2 = run(setup_a, cb_a, chunk_a)  # same arrays and plan, target b's values
+    assert np.abs(s_a2 - s_b).max() < 1e-6
+    assert (r_a2 == r_b).mean() > 0.9999
