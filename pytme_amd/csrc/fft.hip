@@ -59,7 +59,11 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
 // Compile-time specialised variant of k_c2c_strided for the common lengths:
 // the tile (16 lines of 128 B for the long axes) goes straight from global
 // memory into registers, LDS only carries the inter-stage exchanges.
-template <int N, int L, int NT, int DIR>
+// FULL: lin == N and every tile lies inside the row stride ld (lines past
+// `inner` are padding that may be transformed), so no load or store is
+// predicated; the compiler can then wait for the prefetched tile without
+// draining the previous tile's stores.
+template <int N, int L, int NT, int DIR, bool FULL>
 __global__ void __launch_bounds__(NT)
 k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner, int ld,
          int64_t tiles_per_outer, int64_t ntiles, float scale) {
@@ -83,16 +87,21 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
       for (int r = 0; r < PL::R_IN; ++r) {
         int l, e;
         const bool ok = PL::in_pos(tid, i, r, l, e);
-        dstv[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * ld + l] : make_float2(0.f, 0.f);
+        if (FULL)
+          dstv[i * PL::R_IN + r] = src[e * ld + l];
+        else
+          dstv[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * ld + l] : make_float2(0.f, 0.f);
       }
   };
   int64_t t = blockIdx.x;
   if (t >= ntiles) return;
-  load(t, nx);
+  load(t, v);
+  ct_ready<EI>(v);
   for (; t < ntiles; t += gridDim.x) {
-#pragma unroll
-    for (int s = 0; s < EI; ++s) v[s] = nx[s];
-    if (t + gridDim.x < ntiles) load(t + gridDim.x, nx);
+    // on the last tile the current one is reloaded and discarded: the loop
+    // carries no predicated loads, so the wait for the prefetch at the loop
+    // end does not also wait for this tile's stores
+    load(t + gridDim.x < ntiles ? t + gridDim.x : t, nx);
     ct_fft<PL, DIR, L>(lds, tws, v, v, tid);
     const int64_t o = t / tiles_per_outer;
     const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
@@ -103,8 +112,12 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
 #pragma unroll
       for (int r = 0; r < PL::R_OUT; ++r) {
         int l, e;
-        if (PL::out_pos(tid, i, r, l, e) && l < ti) dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
+        const bool ok = PL::out_pos(tid, i, r, l, e);
+        if (FULL || (ok && l < ti)) dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
       }
+#pragma unroll
+    for (int s = 0; s < EI; ++s) v[s] = nx[s];
+    ct_ready<EI>(v);
   }
 }
 
@@ -130,14 +143,21 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   int64_t blocks = (int64_t)device_cus() * (per_cu < 1 ? 1 : per_cu);
   if (blocks > ntiles) blocks = ntiles;
   const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
-  if (dir < 0)
-    TM_TIMED(name, bytes, s,
-             hipLaunchKernelGGL((k_c2c_ct<N, L, NT, -1>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw,
-                                lin, inner, (int)ld, tpo, ntiles, scale));
-  else
-    TM_TIMED(name, bytes, s,
-             hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw,
-                                lin, inner, (int)ld, tpo, ntiles, scale));
+  using PL = CtPlan<N, L, NT, true, false>;
+  // unpredicated form: all slots valid, no pruning, tiles inside the stride
+  const bool full = (N / PL::R_IN * L) % NT == 0 && (N / PL::R_OUT * L) % NT == 0 && lin == N && tpo * L <= ld;
+#define TM_C2C_LAUNCH(D, F)                                                                                      \
+  TM_TIMED(name, bytes, s,                                                                                      \
+           hipLaunchKernelGGL((k_c2c_ct<N, L, NT, D, F>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw, \
+                              lin, inner, (int)ld, tpo, ntiles, scale))
+  if (dir < 0) {
+    if (full) TM_C2C_LAUNCH(-1, true);
+    else TM_C2C_LAUNCH(-1, false);
+  } else {
+    if (full) TM_C2C_LAUNCH(1, true);
+    else TM_C2C_LAUNCH(1, false);
+  }
+#undef TM_C2C_LAUNCH
   TM_CHECK_LAUNCH();
   return TM_OK;
 }

@@ -137,6 +137,16 @@ __device__ __forceinline__ void ct_dft(float2* v) {
   }
 }
 
+// Consume registers here: the compiler waits for their loads at this point
+// (after whatever was issued since) instead of at their first arithmetic use
+// in the next loop iteration, where merging the loop-entry path would make it
+// wait for every outstanding store as well.
+template <int E>
+__device__ __forceinline__ void ct_ready(const float2* v) {
+#pragma unroll
+  for (int s = 0; s < E; ++s) asm volatile("" ::"v"(v[s].x), "v"(v[s].y));
+}
+
 // LDS address of (line l, element e): element-major e*LD + l (LFAST) or
 // line-major l*LD + e.
 template <class P, int LD>

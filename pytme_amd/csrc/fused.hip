@@ -293,28 +293,41 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
 // --------------------------------------------------------------------------
 // compile-time specialised pass A / pass C
 
-// Pass A with the z transforms in registers: a block owns L = 16 (128 B)
-// consecutive kx of one ky; spectra are multiplied in registers between the
-// forward (pruned) and inverse transforms.  KIND 1: X = F t (template slot
-// q_in, output slot o_x); KIND 0 (FLC): S1 = F m, S2 = F2 m (mask slot q_in,
-// output slots 0 and 1).  Splitting the FLC work into two launches keeps the
+// Pass A with the z transforms in registers: a block owns L (128 B) kx
+// columns of one ky; spectra are multiplied in registers between the forward
+// (pruned) and inverse transforms.  KIND 1: X = F t (template slot q_in,
+// output slot o_x); KIND 0 (FLC): S1 = F m, S2 = F2 m (mask slot q_in, output
+// slots 0 and 1).  Splitting the FLC work into two launches keeps the
 // resident spectra and transform buffers within the register file.
-template <int N0, int L, int NT, int KIND>
-__global__ void __launch_bounds__(NT)
+//
+// The spectra rows are padded to a multiple of L bins, so every load and
+// store is unpredicated (padding columns are transformed too and never
+// read).  PR: T0 <= N0 / R_IN, so only the first element of each first-stage
+// butterfly can be non-zero and the others are compile-time zeros.  The next
+// rotation's input columns are loaded before this rotation's transforms.
+// The X kernel (KIND 1) is held to 128 VGPRs for the 512-point case so two
+// blocks share a CU (a couple of spilled registers are cheaper than half the
+// occupancy).
+template <int N0, int L, int NT, int KIND, bool PR>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((KIND == 1 && PR && N0 <= 512) ? 4 : 1, 8)))
 k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
-            float2* __restrict__ Q, const float2* __restrict__ tw, int T0, int N1, int N2c, int N2s, int B,
+            float2* __restrict__ Q, const float2* __restrict__ tw, int T0, int N1, int N2s, int B,
             int tiles_per_row, int q_in, int o_x) {
   using PF = CtPlan<N0, L, NT, true, false>;
   using PI = CtPlan<N0, L, NT, true, true>;
   static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
+  static_assert((N0 / PF::R_IN * L) % NT == 0 && (N0 / PF::R_OUT * L) % NT == 0 &&
+                    (N0 / PI::R_OUT * L) % NT == 0,
+                "every butterfly slot must be valid");
   constexpr int EI = PF::E_IN, EO = PF::E_OUT, EW = EI > EO ? EI : EO;
+  constexpr int BI = EI / PF::R_IN;      // first-stage butterflies per thread
+  constexpr int NX = PR ? BI : EI;       // input loads per thread
   extern __shared__ float2 lds[];
   __shared__ float2 tws[N0];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
   for (int i = tid; i < N0; i += NT) tws[i] = tw[i];
   const int ky = blockIdx.x / tiles_per_row;
   const int kx0 = (blockIdx.x - ky * tiles_per_row) * L;
-  const int ti = (N2c - kx0) < L ? (N2c - kx0) : L;
   const int plane = N1 * N2s;  // per-slab offsets fit in 32 bits (checked by the launcher)
   const int64_t col = (int64_t)ky * N2s + kx0;
   const float2* Ftc = Ft + col;
@@ -326,11 +339,33 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
 #pragma unroll
     for (int r = 0; r < PF::R_OUT; ++r) {
       int l, e;
-      const bool ok = PF::out_pos(tid, i, r, l, e) && l < ti;
+      PF::out_pos(tid, i, r, l, e);
       const int s = i * PF::R_OUT + r;
-      F[s] = ok ? Ftc[e * plane + l] : make_float2(0.f, 0.f);
-      if constexpr (KIND == 0) F2[s] = ok ? Ft2c[e * plane + l] : make_float2(0.f, 0.f);
+      F[s] = Ftc[e * plane + l];
+      if constexpr (KIND == 0) F2[s] = Ft2c[e * plane + l];
     }
+
+  // input element of load slot k: rows e >= T0 are clamped to a valid row
+  // (loaded, then replaced by zero)
+  auto in_elem = [&](int k, int& l, int& e) {
+    if constexpr (PR) {
+      int j;
+      PF::template slot<PF::R_IN>(tid, k, l, j);
+      e = j;
+    } else {
+      PF::in_pos(tid, k / PF::R_IN, k % PF::R_IN, l, e);
+    }
+  };
+  float2 x[NX], xn[NX];
+  auto load_in = [&](int b, float2* dst) {
+    const float2* src = P + ((int64_t)(q_in * B + b) * T0) * (int64_t)plane + col;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      int l, e;
+      in_elem(k, l, e);
+      dst[k] = src[(e < T0 ? e : T0 - 1) * plane + l];
+    }
+  };
 
   auto inv_store = [&](int o, int b, float2* w) {
     ct_fft<PI, 1, L>(lds, tws, w, w, tid);
@@ -340,21 +375,30 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
 #pragma unroll
       for (int r = 0; r < PI::R_OUT; ++r) {
         int l, e;
-        if (PI::out_pos(tid, i, r, l, e) && l < ti) dst[e * plane + l] = w[i * PI::R_OUT + r];
+        PI::out_pos(tid, i, r, l, e);
+        dst[e * plane + l] = w[i * PI::R_OUT + r];
       }
   };
 
+  load_in(0, x);
+  ct_ready<EO>(F);
+  if constexpr (KIND == 0) ct_ready<EO>(F2);
+  ct_ready<NX>(x);
   for (int b = 0; b < B; ++b) {
+    load_in(b + 1 < B ? b + 1 : b, xn);  // the last rotation reloads itself, keeping the loop unpredicated
     float2 w[EW];
-    const float2* src = P + ((int64_t)(q_in * B + b) * T0) * (int64_t)plane + col;
 #pragma unroll
-    for (int i = 0; i < EI / PF::R_IN; ++i)
+    for (int k = 0; k < NX; ++k) {
+      int l, e;
+      in_elem(k, l, e);
+      const float2 v = e < T0 ? x[k] : make_float2(0.f, 0.f);
+      if constexpr (PR) {
 #pragma unroll
-      for (int r = 0; r < PF::R_IN; ++r) {
-        int l, e;
-        const bool ok = PF::in_pos(tid, i, r, l, e) && l < ti && e < T0;
-        w[i * PF::R_IN + r] = ok ? src[e * plane + l] : make_float2(0.f, 0.f);
+        for (int r = 0; r < PF::R_IN; ++r) w[k * PF::R_IN + r] = r == 0 ? v : make_float2(0.f, 0.f);
+      } else {
+        w[k] = v;
       }
+    }
     ct_fft<PF, -1, L>(lds, tws, w, w, tid);
     if constexpr (KIND == 1) {
 #pragma unroll
@@ -370,6 +414,9 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       inv_store(0, b, w);
       inv_store(1, b, t);
     }
+#pragma unroll
+    for (int k = 0; k < NX; ++k) x[k] = xn[k];
+    ct_ready<NX>(x);
   }
 }
 
@@ -518,24 +565,35 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
     }
 }
 
+template <int N0, int L, int NT, bool PR>
+static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
+                               const float2* tw, int T0, int N1, int N2s, int tpr, int B, double bytes,
+                               hipStream_t s) {
+  const int64_t blocks = (int64_t)N1 * tpr;
+  const size_t shm = (size_t)N0 * L * sizeof(float2);
+  if (flc) {
+    TM_TIMED("fused_pass_a", bytes * 0.6, s,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 0, PR>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft, ft2,
+                                Q, tw, T0, N1, N2s, B, tpr, 0, 0));
+    TM_CHECK_LAUNCH();
+  }
+  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", flc ? bytes * 0.4 : bytes, s,
+           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
+                              (const float2*)nullptr, Q, tw, T0, N1, N2s, B, tpr, flc ? 1 : 0, flc ? 2 : 0));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
 template <int N0, int L, int NT>
 static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
                             const float2* tw, int T0, int N1, int N2c, int N2s, int B, double bytes, hipStream_t s) {
   TM_REQUIRE((int64_t)N0 * N1 * N2s < (1LL << 31), "pass A: z column stride too large");
   const int tpr = (int)tm_ceil_div(N2c, L);
-  const int64_t blocks = (int64_t)N1 * tpr;
-  const size_t shm = (size_t)N0 * L * sizeof(float2);
-  if (flc) {
-    TM_TIMED("fused_pass_a", bytes * 0.6, s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 0>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft, ft2, Q,
-                                tw, T0, N1, N2c, N2s, B, tpr, 0, 0));
-    TM_CHECK_LAUNCH();
-  }
-  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", flc ? bytes * 0.4 : bytes, s,
-           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
-                              (const float2*)nullptr, Q, tw, T0, N1, N2c, N2s, B, tpr, flc ? 1 : 0, flc ? 2 : 0));
-  TM_CHECK_LAUNCH();
-  return TM_OK;
+  TM_REQUIRE((int64_t)tpr * L <= N2s, "pass A: spectra rows not padded to whole tiles");
+  using PF = CtPlan<N0, L, NT, true, false>;
+  if (T0 <= N0 / PF::R_IN)
+    return launch_pass_a_ct_pr<N0, L, NT, true>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
+  return launch_pass_a_ct_pr<N0, L, NT, false>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
 }
 
 template <int M, int RL, int NT>
