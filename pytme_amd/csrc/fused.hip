@@ -213,6 +213,18 @@ __device__ __forceinline__ float fu_flc_norm(float x, float ex2, float ex, float
   return x / (sd * n);
 }
 
+// The denominator sd * n of fu_flc_norm (x / denominator is fu_flc_norm).
+__device__ __forceinline__ float fu_flc_denominator(float ex2, float ex, float n, float eps) {
+#pragma clang fp contract(off)
+  float m = ex / n;
+  m = m * m;
+  float v = ex2 / n - m;
+  v = fmaxf(v, 0.f);
+  float sd = sqrtf(v);
+  sd = sd < eps ? 1.f : sd;
+  return sd * n;
+}
+
 template <int NQ>
 __global__ void __launch_bounds__(TM_FU_THREADS)
 k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, int RL, int B,
@@ -424,7 +436,16 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
 // real outputs of a row are the real/imaginary parts of the M-point inverse
 // of Z[k] = (X[k] + X*[M-k]) + i W^k (X[k] - X*[M-k]), W = exp(2 pi i/N2)
 // (imaginary parts of the DC and Nyquist bins dropped, as numpy's irfft).
-template <int M, int RL, int NT, int NQ>
+//
+// A block owns RL rows and walks the (rotation, spectrum) tiles of the batch
+// (FLC: S1, S2, X per rotation), keeping the running max/argmax of its rows
+// in registers and read-modify-writing the state once at the end.  For FLC
+// the local sum S1 is held only until S2 arrives; from then on the thread
+// holds the denominator sd * n (the same float operations as the reference's
+// normalisation, in the same order).  Tiles are prefetched PD deep into
+// registers; every prefetch is issued (the tail re-reads the last tile), so
+// waiting for the oldest one never waits for the younger ones.
+template <int M, int RL, int NT, int NQ, int PD>
 __global__ void __launch_bounds__(NT)
 k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_t nrows, int ldq, int B,
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
@@ -432,7 +453,8 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
   using PI = CtPlan<M, RL, NT, false, true>;
   constexpr int N2 = 2 * M, N2c = M + 1, LD = M + 1;
   constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
-  constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread
+  constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread and tile
+  static_assert(PD == 1 || PD == 2, "prefetch depth 1 or 2");
   extern __shared__ float2 lds[];
   __shared__ float2 tws[N2];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
@@ -442,28 +464,27 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
   const int tile_n = rl * N2c;
   const int iters = B * NQ;
 
-  float best0[EO], best1[EO], a1[EO], b1[EO], a2[EO], b2[EO];
+  float best0[EO], best1[EO], d0[EO], d1[EO];
   int arg0[EO], arg1[EO];
 #pragma unroll
   for (int s = 0; s < EO; ++s) {
     best0[s] = best1[s] = -INFINITY;
     arg0[s] = arg1[s] = -1;
-    a1[s] = b1[s] = a2[s] = b2[s] = 0.f;
+    d0[s] = d1[s] = 0.f;
   }
   // rows of slab (o, b) are consecutive at stride ldq: one tile = rl rows
-  float2 pf[NPF];
-  auto prefetch = [&](int it) {
+  auto prefetch = [&](int it, float2* pf) {
     const int b = it / NQ, o = it - (it / NQ) * NQ;
     const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
-      const int f = tid + p * NT;
+      int f = tid + p * NT;
+      f = f < tile_n ? f : tile_n - 1;  // unpredicated: out-of-tile slots re-read a valid bin
       const int l = f / N2c, e = f - l * N2c;
-      pf[p] = f < tile_n ? src[l * ldq + e] : make_float2(0.f, 0.f);
+      pf[p] = src[l * ldq + e];
     }
   };
-  prefetch(0);
-  for (int it = 0; it < iters; ++it) {
+  auto step = [&](int it, float2* pf) {
     const int b = it / NQ, o = it - b * NQ;
     lds_barrier();  // the previous transform has finished reading the tile
 #pragma unroll
@@ -474,7 +495,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
         lds[l * LD + e] = pf[p];
       }
     }
-    if (it + 1 < iters) prefetch(it + 1);
+    prefetch(it + PD < iters ? it + PD : iters - 1, pf);
     lds_barrier();
     float2 z[EW];
 #pragma unroll
@@ -510,8 +531,8 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
         if (o == NQ - 1) {
           float s0, s1;
           if (NQ == 3) {
-            s0 = fu_flc_norm(x0, a2[s], a1[s], nf, eps);
-            s1 = fu_flc_norm(x1, b2[s], b1[s], nf, eps);
+            s0 = x0 / d0[s];
+            s1 = x1 / d1[s];
           } else if (inv_den) {
             const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + l) * N2 + 2 * m);
             s0 = x0 * d.x;
@@ -529,13 +550,27 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
             arg1[s] = ridx;
           }
         } else if (o == 0) {
-          a1[s] = x0;
-          b1[s] = x1;
+          d0[s] = x0;  // S1 until S2 arrives
+          d1[s] = x1;
         } else {
-          a2[s] = x0;
-          b2[s] = x1;
+          d0[s] = fu_flc_denominator(x0, d0[s], nf, eps);
+          d1[s] = fu_flc_denominator(x1, d1[s], nf, eps);
         }
       }
+  };
+
+  float2 pfa[NPF], pfb[PD == 2 ? NPF : 1];
+  prefetch(0, pfa);
+  if constexpr (PD == 1) {
+    for (int it = 0; it < iters; ++it) step(it, pfa);
+  } else {
+    prefetch(iters > 1 ? 1 : 0, pfb);
+    int it = 0;
+    for (; it + 1 < iters; it += 2) {
+      step(it, pfa);
+      step(it + 1, pfb);
+    }
+    if (it < iters) step(it, pfa);
   }
 #pragma unroll
   for (int i = 0; i < EO / PI::R_OUT; ++i)
@@ -596,21 +631,22 @@ static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const f
   return launch_pass_a_ct_pr<N0, L, NT, false>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
 }
 
-template <int M, int RL, int NT>
+template <int M, int RL, int NT, int PD>
 static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_t nrows, int ldq, int B,
-                            const double* n_obs,
-                            const float* inv_den, const int* rot_idx, float scale, float eps, float* max_scores,
-                            int* max_rot, double bytes, hipStream_t s) {
+                            const double* n_obs, const float* inv_den, const int* rot_idx, float scale, float eps,
+                            float* max_scores, int* max_rot, double bytes, hipStream_t s) {
   const int64_t blocks = tm_ceil_div(nrows, RL);
+  TM_REQUIRE(blocks <= 0x7fffffff, "pass C grid too large");
   const size_t shm = (size_t)RL * (M + 1) * sizeof(float2);
   if (flc)
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2, nrows,
-                                ldq, B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
+                                nrows, ldq, B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
   else
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2, nrows,
-                                ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores, max_rot));
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
+                                nrows, ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores,
+                                max_rot));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -910,9 +946,9 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
     const float eps = 1.1920928955078125e-07f;
     done_c = true;
     switch (N2) {
-      case 128: rc = launch_pass_c_ct<64, 16, 256>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 200: rc = launch_pass_c_ct<100, 16, 256>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 256: rc = launch_pass_c_ct<128, 16, 256>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 128: rc = launch_pass_c_ct<64, 32, 256, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 200: rc = launch_pass_c_ct<100, 5, 128, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 256: rc = launch_pass_c_ct<128, 16, 256, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
       default: done_c = false; break;
     }
     if (done_c && rc != TM_OK) return rc;
