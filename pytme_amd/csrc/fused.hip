@@ -213,16 +213,17 @@ __device__ __forceinline__ float fu_flc_norm(float x, float ex2, float ex, float
   return x / (sd * n);
 }
 
-// The denominator sd * n of fu_flc_norm (x / denominator is fu_flc_norm).
-__device__ __forceinline__ float fu_flc_denominator(float ex2, float ex, float n, float eps) {
-#pragma clang fp contract(off)
-  float m = ex / n;
-  m = m * m;
-  float v = ex2 / n - m;
+// The factor r with fu_flc_norm(x * scale, ex2 * scale, ex * scale, n, eps)
+// ~= x * r, from the unscaled inverse-transform outputs and sn = scale / n.
+// The hardware square root and reciprocal (<= 1 ulp each) and the folded
+// scale replace the correctly rounded divisions: a few ulp in the score.
+__device__ __forceinline__ float fu_flc_factor(float ex2, float ex, float sn, float n, float scale, float eps) {
+  const float m = ex * sn;
+  float v = ex2 * sn - m * m;
   v = fmaxf(v, 0.f);
-  float sd = sqrtf(v);
+  float sd = __builtin_amdgcn_sqrtf(v);
   sd = sd < eps ? 1.f : sd;
-  return sd * n;
+  return scale * __builtin_amdgcn_rcpf(sd * n);
 }
 
 template <int NQ>
@@ -441,10 +442,10 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
 // (FLC: S1, S2, X per rotation), keeping the running max/argmax of its rows
 // in registers and read-modify-writing the state once at the end.  For FLC
 // the local sum S1 is held only until S2 arrives; from then on the thread
-// holds the denominator sd * n (the same float operations as the reference's
-// normalisation, in the same order).  Tiles are prefetched PD deep into
-// registers; every prefetch is issued (the tail re-reads the last tile), so
-// waiting for the oldest one never waits for the younger ones.
+// holds the factor scale / (sd * n), so the score is one multiply.  Tiles
+// are prefetched PD deep into registers; every prefetch is issued (the tail
+// re-reads the last tile), so waiting for the oldest one never waits for the
+// younger ones.
 template <int M, int RL, int NT, int NQ, int PD>
 __global__ void __launch_bounds__(NT)
 k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_t nrows, int ldq, int B,
@@ -519,6 +520,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
       }
     ct_fft<PI, 1, LD, 2>(lds, tws, z, z, tid);
     const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
+    const float sn = NQ == 3 ? scale / nf : 0.f;
     const int ridx = rot_idx[b];
 #pragma unroll
     for (int i = 0; i < EO / PI::R_OUT; ++i)
@@ -527,12 +529,14 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
         int l, m;
         const int s = i * PI::R_OUT + r;
         if (!PI::out_pos(tid, i, r, l, m) || l >= rl) continue;
-        const float x0 = z[s].x * scale, x1 = z[s].y * scale;
+        // FLC works on the unscaled outputs (the scale is in sn and in the
+        // held factor); the correlation scores scale first
+        const float x0 = NQ == 3 ? z[s].x : z[s].x * scale, x1 = NQ == 3 ? z[s].y : z[s].y * scale;
         if (o == NQ - 1) {
           float s0, s1;
           if (NQ == 3) {
-            s0 = x0 / d0[s];
-            s1 = x1 / d1[s];
+            s0 = x0 * d0[s];
+            s1 = x1 * d1[s];
           } else if (inv_den) {
             const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + l) * N2 + 2 * m);
             s0 = x0 * d.x;
@@ -553,8 +557,8 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
           d0[s] = x0;  // S1 until S2 arrives
           d1[s] = x1;
         } else {
-          d0[s] = fu_flc_denominator(x0, d0[s], nf, eps);
-          d1[s] = fu_flc_denominator(x1, d1[s], nf, eps);
+          d0[s] = fu_flc_factor(x0, d0[s], sn, nf, scale, eps);
+          d1[s] = fu_flc_factor(x1, d1[s], sn, nf, scale, eps);
         }
       }
   };
