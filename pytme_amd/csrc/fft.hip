@@ -139,8 +139,18 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   const int64_t tpo = tm_ceil_div(inner, L);
   const int64_t ntiles = outer * tpo;
   const size_t shm = (size_t)N * L * sizeof(float2);
-  const int per_cu = (int)(163840 / shm);  // LDS-resident blocks per CU
-  int64_t blocks = (int64_t)device_cus() * (per_cu < 1 ? 1 : per_cu);
+  // persistent grid: exactly the blocks that are resident at once (LDS and
+  // register limits), so every block runs from the start
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_ct<N, L, NT, 1, true>, NT, shm) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    per_cu = nb;
+  }
+  int64_t blocks = (int64_t)device_cus() * per_cu;
   if (blocks > ntiles) blocks = ntiles;
   const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
   using PL = CtPlan<N, L, NT, true, false>;
@@ -278,7 +288,11 @@ int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, in
     switch (lp.n) {
       case 128: return launch_c2c_ct<128, 16, 256>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
       case 256: return launch_c2c_ct<256, 16, 256>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
-      case 512: return launch_c2c_ct<512, 16, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+      case 512:
+        // unpredicated 8-line tiles when 16-line tiles would not fit the stride
+        if (lin == 512 && tm_ceil_div(inner, 16) * 16 > ld && tm_ceil_div(inner, 8) * 8 <= ld)
+          return launch_c2c_ct<512, 8, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+        return launch_c2c_ct<512, 16, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
       case 1024: return launch_c2c_ct<1024, 8, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
       default: break;
     }

@@ -707,6 +707,8 @@ static void scan_plan_free(tm_scan_plan* p) {
 // Workspace spectra rows are padded to a multiple of 16 bins (128 B), so
 // every 16-bin column tile of the z and y passes is one aligned 128-byte
 // segment per row.
+// (8-bin rows would save the padding traffic but 64-byte row segments
+// stream measurably slower than 128-byte ones on MI355X.)
 static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
 
 // Copy the caller's target spectra (rows of N2c bins) into the padded copies.
@@ -905,11 +907,16 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   const double bytes_a = 8.0 * C_ * (flc ? 2 : 1) + 8.0 * p->nq_in * B * (double)T0 * N1 * N2c +
                          8.0 * C_ * p->nq_out * B;
   bool done_a = true;
+  // 16-bin column tiles when they fit the padded rows, else 8-bin tiles
+  const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s;
+#define TM_PASS_A(N0_, L_, NT_) \
+  launch_pass_a_ct<N0_, L_, NT_>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s)
   switch (N0) {
-    case 128: rc = launch_pass_a_ct<128, 16, 256>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
-    case 256: rc = launch_pass_a_ct<256, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
-    case 512: rc = launch_pass_a_ct<512, 16, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
-    case 1024: rc = launch_pass_a_ct<1024, 8, 512>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s); break;
+    case 128: rc = l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128); break;
+    case 256: rc = l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256); break;
+    case 512: rc = l16 ? TM_PASS_A(512, 16, 512) : TM_PASS_A(512, 8, 512); break;
+    case 1024: rc = TM_PASS_A(1024, 8, 512); break;
+#undef TM_PASS_A
     default: done_a = false; break;
   }
   if (done_a && rc != TM_OK) return rc;
