@@ -82,6 +82,16 @@ struct CtPlan {
   static constexpr int bpt(int R) { return (N_ / R * L_ + NT_ - 1) / NT_; }
   static constexpr int R_IN = radix(0), R_OUT = radix(NST - 1);
   static constexpr int E_IN = bpt(R_IN) * R_IN, E_OUT = bpt(R_OUT) * R_OUT;
+  static constexpr bool all_valid() {
+    for (int s = 0; s < NST; ++s)
+      if ((N_ / radix(s) * L_) % NT_ != 0) return false;
+    return true;
+  }
+  // Line-major plans with idle butterfly slots send those slots to a junk
+  // line L (the LDS buffer then holds L + 1 lines), so no LDS access and no
+  // butterfly is predicated; element-major plans with idle slots predicate.
+  static constexpr bool JUNK = !LFAST_ && !all_valid();
+  static constexpr int LDS_LINES = JUNK ? L_ + 1 : L_;
 
   // line / element of register slot (i, r) of a stage with radix R
   template <int R>
@@ -147,6 +157,19 @@ __device__ __forceinline__ void ct_ready(const float2* v) {
   for (int s = 0; s < E; ++s) asm volatile("" ::"v"(v[s].x), "v"(v[s].y));
 }
 
+// Slot (i) of a radix-R stage: true if the thread works on it.  For JUNK
+// plans every slot is worked on and idle ones address the junk line.
+template <class P, int R>
+__device__ __forceinline__ bool ct_slot(int tid, int i, int& l, int& j) {
+  const bool ok = P::template slot<R>(tid, i, l, j);
+  if constexpr (P::JUNK) {
+    if (!ok) l = P::L;
+    return true;
+  } else {
+    return ok;
+  }
+}
+
 // LDS address of (line l, element e): element-major e*LD + l (LFAST) or
 // line-major l*LD + e.
 template <class P, int LD>
@@ -190,20 +213,23 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
       int l, j;
-      if (P::template slot<R>(tid, i, l, j)) {
+      if (ct_slot<P, R>(tid, i, l, j)) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
         const int k = j % NS;
-        if (k != 0) w1[i] = ct_tw<DIR, TWS>(tw, k * TS);
+        if (P::JUNK || k != 0) w1[i] = ct_tw<DIR, TWS>(tw, k * TS);
       }
     }
     lds_barrier();
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
       int l, j;
-      if (P::template slot<R>(tid, i, l, j)) {
+      if (ct_slot<P, R>(tid, i, l, j)) {
         const int k = j % NS;
-        if (k != 0) ct_twiddle_w<R>(v[i], w1[i]);
+        if (P::JUNK)
+          ct_twiddle_w<R>(v[i], w1[i]);  // k = 0 reads W^0 = 1: exact, and no branch
+        else if (k != 0)
+          ct_twiddle_w<R>(v[i], w1[i]);
         ct_dft<DIR, R>(v[i]);
         const int base = (j - k) * R + k;
 #pragma unroll
@@ -237,7 +263,7 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
 #pragma unroll
     for (int i = 0; i < B0; ++i) {
       int l, j;
-      if (P::template slot<R0>(tid, i, l, j)) {
+      if (ct_slot<P, R0>(tid, i, l, j)) {
         ct_dft<DIR, R0>(in + i * R0);
 #pragma unroll
         for (int r = 0; r < R0; ++r) lds[ct_lds<P, LD>(l, j * R0 + r)] = in[i * R0 + r];
@@ -250,12 +276,15 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
       int l, j;
-      if (P::template slot<R>(tid, i, l, j)) {
+      if (ct_slot<P, R>(tid, i, l, j)) {
         float2* v = out + i * R;
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = lds[ct_lds<P, LD>(l, j + r * NB)];
         const int k = j % NS;
-        ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
+        if (P::JUNK)
+          ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, k * TS));
+        else
+          ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
         ct_dft<DIR, R>(v);
       }
     }

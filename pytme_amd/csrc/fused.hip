@@ -303,6 +303,18 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block i on XCD i % 8).
+// Renumbering so that consecutive logical tiles run on one XCD keeps the
+// neighbouring column tiles of a row (which share DRAM pages) together.
+#ifndef TM_XCD_REMAP
+#define TM_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int tm_xcd_block() {
+  const int n = gridDim.x, b = blockIdx.x;
+  if (!TM_XCD_REMAP || (n & 7)) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // --------------------------------------------------------------------------
 // compile-time specialised pass A / pass C
 
@@ -339,8 +351,9 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   __shared__ float2 tws[N0];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
   for (int i = tid; i < N0; i += NT) tws[i] = tw[i];
-  const int ky = blockIdx.x / tiles_per_row;
-  const int kx0 = (blockIdx.x - ky * tiles_per_row) * L;
+  const int bid = tm_xcd_block();
+  const int ky = bid / tiles_per_row;
+  const int kx0 = (bid - ky * tiles_per_row) * L;
   const int plane = N1 * N2s;  // per-slab offsets fit in 32 bits (checked by the launcher)
   const int64_t col = (int64_t)ky * N2s + kx0;
   const float2* Ftc = Ft + col;
@@ -433,6 +446,9 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   }
 }
 
+// LDS row stride (complex elements) of a pass C tile row of M + 1 bins.
+constexpr int pass_c_ld(int M) { return M + 1; }
+
 // Pass C with the x inverse as a half-length complex transform: the N2 = 2M
 // real outputs of a row are the real/imaginary parts of the M-point inverse
 // of Z[k] = (X[k] + X*[M-k]) + i W^k (X[k] - X*[M-k]), W = exp(2 pi i/N2)
@@ -452,10 +468,13 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
             float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
   using PI = CtPlan<M, RL, NT, false, true>;
-  constexpr int N2 = 2 * M, N2c = M + 1, LD = M + 1;
+  constexpr int N2 = 2 * M, N2c = M + 1, LD = pass_c_ld(M);
   constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
   constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread and tile
-  static_assert(PD == 1 || PD == 2, "prefetch depth 1 or 2");
+  static_assert(PD >= 1 && PD <= 3, "prefetch depth 1 to 3");
+  // the tile loop is unrolled by U so that the spectrum index o and the
+  // prefetch buffer of every step are compile-time constants
+  constexpr int U = NQ % PD == 0 ? NQ : (PD % NQ == 0 ? PD : NQ * PD);
   extern __shared__ float2 lds[];
   __shared__ float2 tws[N2];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
@@ -485,16 +504,14 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
       pf[p] = src[l * ldq + e];
     }
   };
-  auto step = [&](int it, float2* pf) {
-    const int b = it / NQ, o = it - b * NQ;
+  auto step = [&](int it, int o, float2* pf) {
+    const int b = it / NQ;
     lds_barrier();  // the previous transform has finished reading the tile
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
       const int f = tid + p * NT;
-      if (f < tile_n) {
-        const int l = f / N2c, e = f - l * N2c;
-        lds[l * LD + e] = pf[p];
-      }
+      const int l = f / N2c, e = f - l * N2c;
+      lds[f < tile_n ? l * LD + e : RL * LD] = pf[p];  // out-of-tile slots write the junk line
     }
     prefetch(it + PD < iters ? it + PD : iters - 1, pf);
     lds_barrier();
@@ -504,16 +521,11 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
 #pragma unroll
       for (int r = 0; r < PI::R_IN; ++r) {
         int l, k;
-        if (!PI::in_pos(tid, i, r, l, k)) {
-          z[i * PI::R_IN + r] = make_float2(0.f, 0.f);
-          continue;
-        }
+        if (!PI::in_pos(tid, i, r, l, k)) l = RL;  // idle slot: junk line (never stored)
         float2 A = lds[l * LD + k];
         float2 C = lds[l * LD + (M - k)];
-        if (k == 0) {
-          A.y = 0.f;
-          C.y = 0.f;
-        }
+        A.y = k == 0 ? 0.f : A.y;
+        C.y = k == 0 ? 0.f : C.y;
         const float2 E = make_float2(A.x + C.x, A.y - C.y);
         const float2 D = c_mul(make_float2(A.x - C.x, A.y + C.y), tw_load<1>(tws, k));
         z[i * PI::R_IN + r] = make_float2(E.x - D.y, E.y + D.x);
@@ -522,28 +534,30 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
     const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
     const float sn = NQ == 3 ? scale / nf : 0.f;
     const int ridx = rot_idx[b];
+    // Every output slot is updated (idle slots and rows past rl hold junk
+    // that is never stored); the branch on o is uniform.  FLC works on the
+    // unscaled outputs (the scale is in sn and in the held factor); the
+    // correlation scores scale first.
+    if (o == NQ - 1) {
 #pragma unroll
-    for (int i = 0; i < EO / PI::R_OUT; ++i)
+      for (int i = 0; i < EO / PI::R_OUT; ++i)
 #pragma unroll
-      for (int r = 0; r < PI::R_OUT; ++r) {
-        int l, m;
-        const int s = i * PI::R_OUT + r;
-        if (!PI::out_pos(tid, i, r, l, m) || l >= rl) continue;
-        // FLC works on the unscaled outputs (the scale is in sn and in the
-        // held factor); the correlation scores scale first
-        const float x0 = NQ == 3 ? z[s].x : z[s].x * scale, x1 = NQ == 3 ? z[s].y : z[s].y * scale;
-        if (o == NQ - 1) {
+        for (int r = 0; r < PI::R_OUT; ++r) {
+          int l, m;
+          const int s = i * PI::R_OUT + r;
+          const bool ok = PI::out_pos(tid, i, r, l, m) && l < rl;
           float s0, s1;
           if (NQ == 3) {
-            s0 = x0 * d0[s];
-            s1 = x1 * d1[s];
-          } else if (inv_den) {
-            const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + l) * N2 + 2 * m);
-            s0 = x0 * d.x;
-            s1 = x1 * d.y;
+            s0 = z[s].x * d0[s];
+            s1 = z[s].y * d1[s];
           } else {
-            s0 = x0;
-            s1 = x1;
+            s0 = z[s].x * scale;
+            s1 = z[s].y * scale;
+            if (inv_den) {
+              const float2 d = *reinterpret_cast<const float2*>(inv_den + (r0 + (ok ? l : 0)) * N2 + 2 * m);
+              s0 *= d.x;
+              s1 *= d.y;
+            }
           }
           if (s0 > best0[s]) {
             best0[s] = s0;
@@ -553,28 +567,28 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
             best1[s] = s1;
             arg1[s] = ridx;
           }
-        } else if (o == 0) {
-          d0[s] = x0;  // S1 until S2 arrives
-          d1[s] = x1;
-        } else {
-          d0[s] = fu_flc_factor(x0, d0[s], sn, nf, scale, eps);
-          d1[s] = fu_flc_factor(x1, d1[s], sn, nf, scale, eps);
         }
+    } else if (o == 0) {
+#pragma unroll
+      for (int s = 0; s < EO; ++s) {
+        d0[s] = z[s].x;  // S1 until S2 arrives
+        d1[s] = z[s].y;
       }
+    } else {
+#pragma unroll
+      for (int s = 0; s < EO; ++s) {
+        d0[s] = fu_flc_factor(z[s].x, d0[s], sn, nf, scale, eps);
+        d1[s] = fu_flc_factor(z[s].y, d1[s], sn, nf, scale, eps);
+      }
+    }
   };
 
-  float2 pfa[NPF], pfb[PD == 2 ? NPF : 1];
-  prefetch(0, pfa);
-  if constexpr (PD == 1) {
-    for (int it = 0; it < iters; ++it) step(it, pfa);
-  } else {
-    prefetch(iters > 1 ? 1 : 0, pfb);
-    int it = 0;
-    for (; it + 1 < iters; it += 2) {
-      step(it, pfa);
-      step(it + 1, pfb);
-    }
-    if (it < iters) step(it, pfa);
+  float2 pf[PD][NPF];
+#pragma unroll
+  for (int q = 0; q < PD; ++q) prefetch(q < iters ? q : iters - 1, pf[q]);
+  for (int it0 = 0; it0 < iters; it0 += U) {  // iters is a multiple of U (checked by the launcher)
+#pragma unroll
+    for (int u = 0; u < U; ++u) step(it0 + u, u % NQ, pf[u % PD]);
   }
 #pragma unroll
   for (int i = 0; i < EO / PI::R_OUT; ++i)
@@ -604,7 +618,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
     }
 }
 
-template <int N0, int L, int NT, bool PR>
+template <int N0, int L, int NT, int NT0, bool PR>
 static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
                                const float2* tw, int T0, int N1, int N2s, int tpr, int B, double bytes,
                                hipStream_t s) {
@@ -612,7 +626,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   const size_t shm = (size_t)N0 * L * sizeof(float2);
   if (flc) {
     TM_TIMED("fused_pass_a", bytes * 0.6, s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 0, PR>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft, ft2,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
                                 Q, tw, T0, N1, N2s, B, tpr, 0, 0));
     TM_CHECK_LAUNCH();
   }
@@ -623,7 +637,8 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   return TM_OK;
 }
 
-template <int N0, int L, int NT>
+// NT: threads of the X kernel, NT0: threads of the FLC mask kernel
+template <int N0, int L, int NT, int NT0 = NT>
 static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
                             const float2* tw, int T0, int N1, int N2c, int N2s, int B, double bytes, hipStream_t s) {
   TM_REQUIRE((int64_t)N0 * N1 * N2s < (1LL << 31), "pass A: z column stride too large");
@@ -631,26 +646,35 @@ static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const f
   TM_REQUIRE((int64_t)tpr * L <= N2s, "pass A: spectra rows not padded to whole tiles");
   using PF = CtPlan<N0, L, NT, true, false>;
   if (T0 <= N0 / PF::R_IN)
-    return launch_pass_a_ct_pr<N0, L, NT, true>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
-  return launch_pass_a_ct_pr<N0, L, NT, false>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
+    return launch_pass_a_ct_pr<N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
+  return launch_pass_a_ct_pr<N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
 }
 
-template <int M, int RL, int NT, int PD>
+template <int M, int RL, int NT, int PD3, int PD1>
 static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_t nrows, int ldq, int B,
                             const double* n_obs, const float* inv_den, const int* rot_idx, float scale, float eps,
                             float* max_scores, int* max_rot, double bytes, hipStream_t s) {
+  // PD3 / PD1: prefetch depth of the FLC (3 spectra) / correlation kernels;
+  // the step count B * NQ must be a multiple of the unroll lcm(NQ, PD)
   const int64_t blocks = tm_ceil_div(nrows, RL);
   TM_REQUIRE(blocks <= 0x7fffffff, "pass C grid too large");
-  const size_t shm = (size_t)RL * (M + 1) * sizeof(float2);
-  if (flc)
+  const size_t shm = (size_t)(RL + 1) * pass_c_ld(M) * sizeof(float2);  // + the junk line
+  if (flc) {
+    static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
                                 nrows, ldq, B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
-  else
+  } else if (B % PD1 == 0) {
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
                                 nrows, ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores,
                                 max_rot));
+  } else {
+    TM_TIMED("fused_pass_c", bytes, s,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
+                                nrows, ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores,
+                                max_rot));
+  }
   TM_CHECK_LAUNCH();
   return TM_OK;
 }
@@ -909,8 +933,8 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   bool done_a = true;
   // 16-bin column tiles when they fit the padded rows, else 8-bin tiles
   const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s;
-#define TM_PASS_A(N0_, L_, NT_) \
-  launch_pass_a_ct<N0_, L_, NT_>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s)
+#define TM_PASS_A(N0_, L_, ...) \
+  launch_pass_a_ct<N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s)
   switch (N0) {
     case 128: rc = l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128); break;
     case 256: rc = l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256); break;
@@ -957,9 +981,9 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
     const float eps = 1.1920928955078125e-07f;
     done_c = true;
     switch (N2) {
-      case 128: rc = launch_pass_c_ct<64, 32, 256, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 200: rc = launch_pass_c_ct<100, 5, 128, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 256: rc = launch_pass_c_ct<128, 16, 256, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 128: rc = launch_pass_c_ct<64, 32, 256, 1, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 200: rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 256: rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
       default: done_c = false; break;
     }
     if (done_c && rc != TM_OK) return rc;
