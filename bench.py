@@ -24,6 +24,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the default workload (scaled by
+# a 1 GiB copy); its traffic_bytes_per_launch line feeds roofline.traffic
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48_b16.md")
+
+
+def measured_traffic(kernel, workload, batch, path):
+    """HBM bytes per launch of `kernel` from the committed counter summary,
+    or None when the run is not the profiled configuration."""
+    if workload != "flc_512x512x200_t48" or batch != 16 or path != "fused":
+        return None
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                line = line.strip()
+                if line.startswith("traffic_bytes_per_launch = "):
+                    return json.loads(line.split("=", 1)[1]).get(kernel)
+    except OSError:
+        return None
+    return None
 
 
 def parse():
@@ -203,7 +222,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(name, args.workload, batch, args.path),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "launches": count,
             "avg_launch_us": round(ms * 1e3 / max(count, 1), 2),
             "bytes_per_launch": by / max(count, 1),
