@@ -129,6 +129,28 @@ int tm_scan_plan_set_target(tm_scan_plan* plan, const void* ft_target, const voi
 int tm_scan_batch(tm_scan_plan* plan, const float* rotations, const int* rotation_indices, int nbatch,
                   float* max_scores, int* max_rotations, void* stream);
 
+/* ---- local FLC refinement --------------------------------------------------
+ * tm_flc_local_fwd replaces the score evaluation of the derivative-free
+ * refinement, FLC.__call__ after _MatchDensityToDensity.score
+ * (tme/matching_optimization.py:488-548, 198-238), for a batch of poses: it
+ * returns per pose the six moments (n, sum g m, sum g^2 m, sum g m f,
+ * sum m f, sum m f^2) of the rotated standardised template g and mask m over
+ * the template box with corner v in the target f; the host assembles
+ * overlap / (n sd_f).  The rotated mask goes into the mask (the reference
+ * writes it into the template buffer, :162-163).  tm_flc_local_bwd has no
+ * counterpart in the reference (which is derivative-free): given
+ * w = dScore/d(moment) it returns dScore/d(R^T) and dScore/ds.
+ * poses: float32 [n][12] = R^T row-major (9) and the sub-voxel shift s (3);
+ * corners: int32 [n][3]; moments float64 [n][6]; grad float64 [n][12]. */
+typedef struct tm_flc_local_plan tm_flc_local_plan;
+int tm_flc_local_plan_create(const float* template_std, const float* mask, const int* template_shape, int max_poses,
+                             tm_flc_local_plan** plan);
+int tm_flc_local_plan_destroy(tm_flc_local_plan* plan);
+int tm_flc_local_fwd(tm_flc_local_plan* plan, const float* target, const int64_t* target_shape, const float* poses,
+                     const int* corners, int n_poses, double* moments, void* stream);
+int tm_flc_local_bwd(tm_flc_local_plan* plan, const float* target, const int64_t* target_shape, const float* poses,
+                     const int* corners, int n_poses, const double* weights, double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

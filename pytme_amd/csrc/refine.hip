@@ -1,0 +1,302 @@
+// Local FLC score moments and their gradient for batches of poses (the
+// refinement objective of tme/matching_optimization.py:488-548 evaluated at
+// many candidate peaks at once, plus its derivative).
+//
+// A pose p places the standardised template g and its mask m with the
+// corner at the integer voxel v_p of the target f and samples them at
+//     q(o) = R_p^T (o - c) + c + s_p,   c = floor(T / 2),
+// for every template voxel o (order-1 interpolation, zero outside
+// [0, T-1] per axis, as scipy's map_coordinates in constant mode).  The
+// forward kernel returns the six moments
+//     n = sum m,  sum g m,  sum g^2 m,  sum g m f,  sum m f,  sum m f^2
+// (f = 0 where the template box leaves the target) from which the host
+// assembles the score; the backward kernel takes the score's derivatives
+// w_k = dS/d(moment k) and returns dS/d(R^T) (3x3) and dS/ds (3),
+// recomputing the interpolation weights instead of storing them.
+//
+// Layout: template and mask interleaved as float2 [T0][T1][T2] (x fastest,
+// L2-resident: 2 MB for 64^3); the target is the caller's float32
+// [N0][N1][N2] volume in HBM.  Grid: (blocks per pose, poses); each block
+// owns a contiguous range of template voxels, every thread accumulates its
+// voxels (moments in FP64), the block reduces in LDS and writes one partial
+// per pose and block; a second kernel sums the partials in block order, so
+// results are deterministic.
+#include <math.h>
+#include <stdlib.h>
+
+#include "tm_common.h"
+
+#define TM_RF_THREADS 256
+
+struct tm_flc_local_plan {
+  int T[3];
+  int64_t nvox;
+  int max_poses;
+  int nblk;  // blocks per pose
+  float2* gm;
+  double* partial;  // [max_poses][nblk][12]
+};
+
+namespace {
+
+struct Sample {
+  float g, m;
+  float dg[3], dm[3];  // d/dq (z, y, x); filled when GRAD
+};
+
+template <bool GRAD>
+__device__ __forceinline__ Sample trilinear(const float2* __restrict__ gm, int T0, int T1, int T2, float qz, float qy,
+                                            float qx) {
+  Sample r;
+  const bool valid = qz >= 0.f && qy >= 0.f && qx >= 0.f && qz <= (float)(T0 - 1) && qy <= (float)(T1 - 1) &&
+                     qx <= (float)(T2 - 1);
+  const float fz0 = floorf(qz), fy0 = floorf(qy), fx0 = floorf(qx);
+  int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
+  const float wz = qz - fz0, wy = qy - fy0, wx = qx - fx0;
+  // out-of-range samples read voxel 0 and are zeroed afterwards
+  z0 = valid ? z0 : 0;
+  y0 = valid ? y0 : 0;
+  x0 = valid ? x0 : 0;
+  const int z1 = min(z0 + 1, T0 - 1), y1 = min(y0 + 1, T1 - 1), x1 = min(x0 + 1, T2 - 1);
+  const int r00 = (z0 * T1 + y0) * T2, r01 = (z0 * T1 + y1) * T2, r10 = (z1 * T1 + y0) * T2,
+            r11 = (z1 * T1 + y1) * T2;
+  const float2 v000 = gm[r00 + x0], v001 = gm[r00 + x1], v010 = gm[r01 + x0], v011 = gm[r01 + x1];
+  const float2 v100 = gm[r10 + x0], v101 = gm[r10 + x1], v110 = gm[r11 + x0], v111 = gm[r11 + x1];
+  auto lerp = [](float a, float b, float t) { return a + t * (b - a); };
+  // along x
+  const float g00 = lerp(v000.x, v001.x, wx), g01 = lerp(v010.x, v011.x, wx);
+  const float g10 = lerp(v100.x, v101.x, wx), g11 = lerp(v110.x, v111.x, wx);
+  const float m00 = lerp(v000.y, v001.y, wx), m01 = lerp(v010.y, v011.y, wx);
+  const float m10 = lerp(v100.y, v101.y, wx), m11 = lerp(v110.y, v111.y, wx);
+  const float g0 = lerp(g00, g01, wy), g1 = lerp(g10, g11, wy);
+  const float m0 = lerp(m00, m01, wy), m1 = lerp(m10, m11, wy);
+  r.g = valid ? lerp(g0, g1, wz) : 0.f;
+  r.m = valid ? lerp(m0, m1, wz) : 0.f;
+  if (GRAD) {
+    const float s = valid ? 1.f : 0.f;
+    r.dg[0] = s * (g1 - g0);
+    r.dm[0] = s * (m1 - m0);
+    r.dg[1] = s * lerp(g01 - g00, g11 - g10, wz);
+    r.dm[1] = s * lerp(m01 - m00, m11 - m10, wz);
+    const float gx0 = lerp(v001.x - v000.x, v011.x - v010.x, wy), gx1 = lerp(v101.x - v100.x, v111.x - v110.x, wy);
+    const float mx0 = lerp(v001.y - v000.y, v011.y - v010.y, wy), mx1 = lerp(v101.y - v100.y, v111.y - v110.y, wy);
+    r.dg[2] = s * lerp(gx0, gx1, wz);
+    r.dm[2] = s * lerp(mx0, mx1, wz);
+  }
+  return r;
+}
+
+// NV = 6 (moments) or 12 (gradient): block-reduce acc[NV] in FP64 and store
+template <int NV>
+__device__ __forceinline__ void block_store(double* acc, double* out) {
+  __shared__ double red[TM_RF_THREADS / 64][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < TM_RF_THREADS / 64; ++w) v += red[w][threadIdx.x];
+    out[threadIdx.x] = v;
+  }
+}
+
+template <bool BWD>
+__global__ void __launch_bounds__(TM_RF_THREADS)
+k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const float2* __restrict__ gm, int T0, int T1,
+            int T2, const float* __restrict__ poses, const int* __restrict__ corners,
+            const double* __restrict__ weights, int64_t nvox, int nblk, double* __restrict__ partial) {
+  const int p = blockIdx.y, blk = blockIdx.x;
+  const float* P = poses + 12 * p;  // R^T row-major (9), s (3)
+  const float r00 = P[0], r01 = P[1], r02 = P[2], r10 = P[3], r11 = P[4], r12 = P[5], r20 = P[6], r21 = P[7],
+              r22 = P[8];
+  const float cz = (float)(T0 / 2), cy = (float)(T1 / 2), cx = (float)(T2 / 2);
+  const float bz = cz + P[9], by = cy + P[10], bx = cx + P[11];
+  const int vz = corners[3 * p], vy = corners[3 * p + 1], vx = corners[3 * p + 2];
+  float w[6];
+  if (BWD) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = (float)weights[6 * p + k];
+  }
+  const int64_t per = (nvox + nblk - 1) / nblk;
+  const int64_t o0 = per * blk, o1 = min(o0 + per, nvox);
+  double acc[BWD ? 12 : 6];
+  float ga[12];
+#pragma unroll
+  for (int k = 0; k < (BWD ? 12 : 6); ++k) acc[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) ga[k] = 0.f;
+  int nacc = 0;
+  for (int64_t o = o0 + threadIdx.x; o < o1; o += TM_RF_THREADS) {
+    const int ix = (int)(o % T2);
+    const int64_t zy = o / T2;
+    const int iy = (int)(zy % T1), iz = (int)(zy / T1);
+    const float dz = (float)iz - cz, dy = (float)iy - cy, dx = (float)ix - cx;
+    const float qz = r00 * dz + r01 * dy + r02 * dx + bz;
+    const float qy = r10 * dz + r11 * dy + r12 * dx + by;
+    const float qx = r20 * dz + r21 * dy + r22 * dx + bx;
+    const Sample sm = trilinear<BWD>(gm, T0, T1, T2, qz, qy, qx);
+    const int pz = vz + iz, py = vy + iy, px = vx + ix;
+    const bool inside = pz >= 0 && py >= 0 && px >= 0 && pz < N0 && py < N1 && px < N2;
+    const float f = inside ? target[((int64_t)pz * N1 + py) * N2 + px] : 0.f;
+    if (!BWD) {
+      const float gmv = sm.g * sm.m, mf = sm.m * f;
+      acc[0] += (double)sm.m;
+      acc[1] += (double)gmv;
+      acc[2] += (double)(gmv * sm.g);
+      acc[3] += (double)(gmv * f);
+      acc[4] += (double)mf;
+      acc[5] += (double)(mf * f);
+    } else {
+      const float ag = w[1] * sm.m + 2.f * w[2] * sm.g * sm.m + w[3] * sm.m * f;
+      const float am = w[0] + w[1] * sm.g + w[2] * sm.g * sm.g + w[3] * sm.g * f + w[4] * f + w[5] * f * f;
+      const float gq0 = ag * sm.dg[0] + am * sm.dm[0];
+      const float gq1 = ag * sm.dg[1] + am * sm.dm[1];
+      const float gq2 = ag * sm.dg[2] + am * sm.dm[2];
+      // q = R^T d + c + s: d q_a / d (R^T)_ab = d_b, d q / d s = I
+      ga[0] += gq0 * dz;
+      ga[1] += gq0 * dy;
+      ga[2] += gq0 * dx;
+      ga[3] += gq1 * dz;
+      ga[4] += gq1 * dy;
+      ga[5] += gq1 * dx;
+      ga[6] += gq2 * dz;
+      ga[7] += gq2 * dy;
+      ga[8] += gq2 * dx;
+      ga[9] += gq0;
+      ga[10] += gq1;
+      ga[11] += gq2;
+      // flush the float partial sums into FP64 every 32 voxels
+      if (++nacc == 32) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          acc[k] += (double)ga[k];
+          ga[k] = 0.f;
+        }
+        nacc = 0;
+      }
+    }
+  }
+  if (BWD) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] += (double)ga[k];
+  }
+  block_store<BWD ? 12 : 6>(acc, partial + ((int64_t)p * nblk + blk) * 12);
+}
+
+__global__ void k_flc_local_reduce(const double* __restrict__ partial, int nposes, int nblk, int nv,
+                                   double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nposes * nv) return;
+  const int p = i / nv, k = i - p * nv;
+  double v = 0.0;
+  for (int b = 0; b < nblk; ++b) v += partial[((int64_t)p * nblk + b) * 12 + k];
+  out[i] = v;
+}
+
+__global__ void k_interleave(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
+                             float2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(a[i], b[i]);
+}
+
+int local_launch(tm_flc_local_plan* p, bool bwd, const float* target, const int64_t* tshape, const float* poses,
+                 const int* corners, int nposes, const double* weights, double* out, hipStream_t s) {
+  TM_REQUIRE(p && target && tshape && poses && corners && out, "null argument to the local FLC kernels");
+  TM_REQUIRE(nposes >= 1 && nposes <= p->max_poses, "%d poses outside [1, %d]", nposes, p->max_poses);
+  TM_REQUIRE(!bwd || weights, "the gradient needs the score weights");
+  for (int i = 0; i < 3; ++i) TM_REQUIRE(tshape[i] >= 1 && tshape[i] < (1 << 30), "target axis %d out of range", i);
+  const dim3 grid((unsigned)p->nblk, (unsigned)nposes);
+  const double bytes = (double)nposes * p->nvox * 4.0;  // target window (template brick is L2-resident)
+  if (bwd)
+    TM_TIMED("flc_local_bwd", bytes, s,
+             hipLaunchKernelGGL(k_flc_local<true>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
+                                (int)tshape[1], (int)tshape[2], p->gm, p->T[0], p->T[1], p->T[2], poses, corners,
+                                weights, p->nvox, p->nblk, p->partial));
+  else
+    TM_TIMED("flc_local_fwd", bytes, s,
+             hipLaunchKernelGGL(k_flc_local<false>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
+                                (int)tshape[1], (int)tshape[2], p->gm, p->T[0], p->T[1], p->T[2], poses, corners,
+                                (const double*)nullptr, p->nvox, p->nblk, p->partial));
+  TM_CHECK_LAUNCH();
+  const int nv = bwd ? 12 : 6;
+  hipLaunchKernelGGL(k_flc_local_reduce, dim3((unsigned)tm_ceil_div((int64_t)nposes * nv, 256)), dim3(256), 0, s,
+                     p->partial, nposes, p->nblk, nv, out);
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Plan for poses of one standardised template and its mask (T^3 float32,
+// device).  The template and mask are copied (interleaved) into the plan.
+int tm_flc_local_plan_create(const float* template_std, const float* mask, const int* tshape, int max_poses,
+                             tm_flc_local_plan** out) {
+  TM_REQUIRE(template_std && mask && tshape && out, "null argument to tm_flc_local_plan_create");
+  TM_REQUIRE(max_poses >= 1 && max_poses <= (1 << 20), "max_poses %d out of range", max_poses);
+  for (int i = 0; i < 3; ++i) TM_REQUIRE(tshape[i] >= 1 && tshape[i] <= 1024, "template axis %d out of range", i);
+  tm_flc_local_plan* p = (tm_flc_local_plan*)calloc(1, sizeof(tm_flc_local_plan));
+  TM_REQUIRE(p, "host allocation failed");
+  for (int i = 0; i < 3; ++i) p->T[i] = tshape[i];
+  p->nvox = (int64_t)tshape[0] * tshape[1] * tshape[2];
+  p->max_poses = max_poses;
+  // ~64 voxels per thread, at least one block per pose
+  int64_t nb = tm_ceil_div(p->nvox, (int64_t)TM_RF_THREADS * 64);
+  p->nblk = (int)(nb < 1 ? 1 : (nb > 64 ? 64 : nb));
+  hipError_t e = hipMalloc((void**)&p->gm, sizeof(float2) * p->nvox);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->partial, sizeof(double) * 12 * (size_t)max_poses * p->nblk);
+  if (e != hipSuccess) {
+    tm_set_error("hipMalloc failed: %s", hipGetErrorString(e));
+    if (p->gm) hipFree(p->gm);
+    if (p->partial) hipFree(p->partial);
+    free(p);
+    return e == hipErrorOutOfMemory ? TM_ERR_OOM : TM_ERR_HIP;
+  }
+  hipLaunchKernelGGL(k_interleave, dim3((unsigned)tm_ceil_div(p->nvox, 256)), dim3(256), 0, 0, template_std, mask,
+                     p->nvox, p->gm);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    tm_set_error("preparing the template brick: %s", hipGetErrorString(e));
+    hipFree(p->gm);
+    hipFree(p->partial);
+    free(p);
+    return TM_ERR_HIP;
+  }
+  *out = p;
+  return TM_OK;
+}
+
+int tm_flc_local_plan_destroy(tm_flc_local_plan* p) {
+  if (p) {
+    hipFree(p->gm);
+    hipFree(p->partial);
+    free(p);
+  }
+  return TM_OK;
+}
+
+// Moments of n_poses poses: poses float32 [n][12] = R^T row-major, s;
+// corners int32 [n][3] = v; out float64 [n][6].
+int tm_flc_local_fwd(tm_flc_local_plan* p, const float* target, const int64_t* target_shape, const float* poses,
+                     const int* corners, int n_poses, double* moments, void* stream) {
+  return local_launch(p, false, target, target_shape, poses, corners, n_poses, nullptr, moments,
+                      (hipStream_t)stream);
+}
+
+// Gradient: weights float64 [n][6] = dS/d(moment); out float64 [n][12] =
+// dS/d(R^T) row-major (9), dS/ds (3).
+int tm_flc_local_bwd(tm_flc_local_plan* p, const float* target, const int64_t* target_shape, const float* poses,
+                     const int* corners, int n_poses, const double* weights, double* grad, void* stream) {
+  return local_launch(p, true, target, target_shape, poses, corners, n_poses, weights, grad, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+"""Local FLC refinement on the GPU: moments and scores vs the FP64 oracle,
+the backward kernel vs the CPU FP64 autograd reference and finite
+differences, and batched gradient refinement vs derivative-free refinement."""
+
+import numpy as np
+import pytest
+from scipy.ndimage import gaussian_filter
+
+from oracle import refine as orf
+
+pytestmark = pytest.mark.gpu
+
+
+def _template(seed, T):
+    rng = np.random.default_rng(seed)
+    t = gaussian_filter(rng.normal(size=(T,) * 3), 1.2)
+    c = T // 2
+    zz, yy, xx = np.indices((T,) * 3)
+    mask = ((zz - c) ** 2 + (yy - c) ** 2 + (xx - c) ** 2 <= (c - 0.5) ** 2).astype(np.float32)
+    return t.astype(np.float32), mask
+
+
+def _poses(rng, n, tshape, T, margin=3):
+    lo = -margin
+    hi = np.asarray(tshape) - T + margin
+    t = rng.uniform(lo, hi, size=(n, 3))
+    a = rng.uniform(-180, 180, size=(n, 3))
+    return t, a
+
+
+def _gpu_R(a_deg):
+    import torch
+
+    from pytme_amd.refine import euler_zyz_matrix
+
+    Rt = euler_zyz_matrix(torch.deg2rad(torch.as_tensor(a_deg, dtype=torch.float64))).transpose(-1, -2)
+    return Rt.to(torch.float32).to(torch.float64)  # the matrix the kernel uses
+
+
+def test_moments_and_scores_match_oracle():
+    import torch
+
+    from pytme_amd.refine import LocalFLC, score_from_moments
+
+    rng = np.random.default_rng(7)
+    target = rng.normal(size=(30, 28, 26)).astype(np.float32)
+    template, mask = _template(8, 9)
+    ts = orf.prepare_template(template, mask)
+    t, a = _poses(rng, 40, target.shape, 9)
+    local = LocalFLC(target, template, mask, max_poses=64)
+    print("local moments 40 poses, 9^3 template", flush=True)
+    Rt = _gpu_R(a).cuda()
+    v = np.trunc(t).astype(np.int32)
+    s = torch.as_tensor(t - v, dtype=torch.float64).cuda()
+    got = local.moments(Rt, s, v).cpu().numpy()
+    sc = score_from_moments(torch.as_tensor(got)).numpy()
+    worst = 0.0
+    for i in range(t.shape[0]):
+        R = Rt[i].cpu().numpy().T
+        want = orf.local_sums(target, ts, mask, t[i], R, "reference")  # the kernel adds s as given
+        scale = np.maximum(np.abs(want), np.array([1.0, 1.0, want[2], 1.0, 1.0, want[5]]))
+        err = np.abs(got[i] - want) / np.maximum(scale, 1e-6)
+        worst = max(worst, float(err.max()))
+        assert err.max() < 5e-5, (i, got[i], want)
+        ref_score = orf.score_from_sums(want)
+        assert abs(sc[i] - ref_score) < 2e-5, (i, sc[i], ref_score)
+    print(f"  worst relative moment error {worst:.2e}", flush=True)
+    # the rigid sub-voxel model through the host helper
+    from pytme_amd.refine import score_poses
+
+    got = score_poses(local, t, a, "rigid").cpu().numpy()
+    for i in range(t.shape[0]):
+        R = Rt[i].cpu().numpy().T
+        want = orf.score_from_sums(orf.local_sums(target, ts, mask, t[i], R, "rigid"))
+        assert abs(got[i] - want) < 2e-5, (i, got[i], want)
+
+
+def _autograd_reference(target, ts, mask, t, a_rad, subvoxel):
+    import torch
+
+    euler_zyz, _, sums, score = orf.torch_reference()
+    v, s = orf.split_translation(t)
+    st = torch.tensor(s, requires_grad=True)
+    at = torch.tensor(a_rad, requires_grad=True)
+    val = score(sums(target, ts, mask, v, st, euler_zyz(at), subvoxel))
+    val.backward()
+    return float(val), st.grad.numpy(), at.grad.numpy()
+
+
+def _gpu_score_and_grad(local, t, a_rad, subvoxel="rigid"):
+    import torch
+
+    from pytme_amd.refine import pose_moments, score_from_moments
+
+    tt = torch.tensor(t, dtype=torch.float64, device="cuda", requires_grad=True)
+    at = torch.tensor(a_rad, dtype=torch.float64, device="cuda", requires_grad=True)
+    sc = score_from_moments(pose_moments(local, tt, at, subvoxel))
+    sc.sum().backward()
+    return sc.detach().cpu().numpy(), tt.grad.cpu().numpy(), at.grad.cpu().numpy()
+
+
+@pytest.mark.parametrize("subvoxel", ["rigid", "reference"])
+def test_backward_matches_cpu_fp64_autograd(subvoxel):
+    from pytme_amd.refine import LocalFLC
+
+    rng = np.random.default_rng(11)
+    target = gaussian_filter(rng.normal(size=(26, 26, 26)), 0.7).astype(np.float32)
+    template, mask = _template(12, 11)
+    ts = orf.prepare_template(template, mask)
+    t = rng.uniform(2, 13, size=(6, 3))
+    a = np.deg2rad(rng.uniform(-60, 60, size=(6, 3)))
+    local = LocalFLC(target, template, mask, max_poses=8)
+    print("local backward vs FP64 autograd, 6 poses, 11^3 template", flush=True)
+    sc, gt, ga = _gpu_score_and_grad(local, t, a, subvoxel)
+    for i in range(t.shape[0]):
+        ref, rt, ra = _autograd_reference(target, ts, mask, t[i], a[i], subvoxel)
+        scale = max(np.abs(rt).max(), np.abs(ra).max(), 1e-3)
+        assert abs(sc[i] - ref) < 2e-5
+        assert np.abs(gt[i] - rt).max() < 2e-3 * scale, (i, gt[i], rt)
+        assert np.abs(ga[i] - ra).max() < 2e-3 * scale, (i, ga[i], ra)
+
+
+def test_backward_matches_central_differences():
+    """h = 1e-2 voxel / 1e-3 rad on FP32 GPU scores; the trilinear interpolant
+    is piecewise linear, so the differences average slopes over +-h."""
+    from pytme_amd.refine import LocalFLC
+
+    rng = np.random.default_rng(13)
+    target = gaussian_filter(rng.normal(size=(28, 28, 28)), 0.7).astype(np.float32)
+    template, mask = _template(14, 13)
+    t = np.array([[6.3, 7.6, 5.45], [8.2, 4.7, 9.35]])
+    a = np.deg2rad(np.array([[15.0, 30.0, -20.0], [-40.0, 75.0, 110.0]]))
+    local = LocalFLC(target, template, mask, max_poses=64)
+    print("local backward vs central differences", flush=True)
+    _, gt, ga = _gpu_score_and_grad(local, t, a)
+
+    def scores(tt, aa):
+        s, _, _ = _gpu_score_and_grad(local, tt, aa)
+        return s
+
+    for k in range(3):
+        e = np.zeros((1, 3))
+        e[0, k] = 1e-2
+        fd_t = (scores(t + e, a) - scores(t - e, a)) / 2e-2
+        e[0, k] = 1e-3
+        fd_a = (scores(t, a + e) - scores(t, a - e)) / 2e-3
+        for i in range(t.shape[0]):
+            scale = max(np.abs(gt[i]).max(), np.abs(ga[i]).max())
+            assert abs(fd_t[i] - gt[i, k]) < 5e-2 * scale, ("t", i, k, fd_t[i], gt[i, k])
+            assert abs(fd_a[i] - ga[i, k]) < 5e-2 * scale, ("a", i, k, fd_a[i], ga[i, k])
+
+
+def _embedded_target(seed, shape, template, poses, noise=0.3):
+    rng = np.random.default_rng(seed)
+    target = rng.normal(scale=noise, size=shape)
+    T = template.shape[0]
+    for t, a in poses:
+        v, s = orf.split_translation(t)
+        R = orf.euler_to_matrix(a)
+        block = orf.sample(template, R, orf.sampling_shift(R, s, "rigid"))
+        target[v[0]:v[0] + T, v[1]:v[1] + T, v[2]:v[2] + T] += block
+    return target.astype(np.float32)
+
+
+def test_gradient_refinement_vs_derivative_free():
+    """Perturbed starts (N(0, 4 deg) per angle, N(0, 1.5 voxel) per axis)
+    refined by batched Adam reach the derivative-free (basinhopping/COBYLA,
+    scripts/postprocess.py:588-613 settings) final score with at most 1/5 of
+    its score evaluations."""
+    from pytme_amd.matching_optimization import FLC, optimize_match
+    from pytme_amd.refine import LocalFLC, refine_peaks, score_poses
+
+    from pytme_amd.synthetic import blob_template, threshold_mask
+
+    template = blob_template(20, n_blobs=12, seed=21)
+    mask = threshold_mask(template, 0.1, 2)
+    rng = np.random.default_rng(22)
+    truth = [(np.array([6.0, 7.0, 8.0]) + rng.uniform(0, 1, 3), rng.uniform(-90, 90, 3)),
+             (np.array([36.0, 9.0, 37.0]) + rng.uniform(0, 1, 3), rng.uniform(-90, 90, 3)),
+             (np.array([10.0, 38.0, 34.0]) + rng.uniform(0, 1, 3), rng.uniform(-90, 90, 3)),
+             (np.array([37.0, 36.0, 7.0]) + rng.uniform(0, 1, 3), rng.uniform(-90, 90, 3))]
+    target = _embedded_target(23, (64, 64, 64), template, truth, noise=0.2)
+    t_true = np.array([p[0] for p in truth])
+    a_true = np.array([p[1] for p in truth])
+    t0 = t_true + rng.normal(0, 1.5, size=t_true.shape)
+    a0 = a_true + rng.normal(0, 4.0, size=a_true.shape)
+    local = LocalFLC(target, template, mask, max_poses=16)
+    s_true = score_poses(local, t_true, a_true).cpu().numpy()
+    s_start = score_poses(local, t0, a0).cpu().numpy()
+    print(f"refinement: true {np.round(s_true, 3)} start {np.round(s_start, 3)}", flush=True)
+    t1, a1, s1, info = refine_peaks(local, t0, a0, steps=50, lr_translation=0.1, lr_angle_deg=1.0)
+    print(f"  gradient  {np.round(s1, 3)} ({info['score_evaluations_per_peak']} evaluations per peak)", flush=True)
+    assert np.all(s1 >= s_start)
+    assert np.all(s1 >= 0.98 * s_true), (s1, s_true)
+
+    evals, s_df = [], []
+    for i in range(2):
+        obj = FLC(target, template, mask)
+        tr, rot, fun = optimize_match(obj, optimization_method="basinhopping", maxiter=3,
+                                      bounds_translation=tuple((x - 5, x + 5) for x in t0[i]), x0=[*t0[i], *a0[i]])
+        evals.append(obj.evaluations)
+        s_df.append(-fun)
+    print(f"  derivative-free {np.round(s_df, 3)} ({evals} evaluations)", flush=True)
+    for i in range(2):
+        assert s1[i] >= s_df[i] - 1e-3, (s1[i], s_df[i])
+        assert 5 * info["score_evaluations_per_peak"] <= evals[i], (info, evals)
