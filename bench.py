@@ -189,6 +189,7 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps):
     a0 = a_true + rng.normal(0.0, 4.0, size=a_true.shape)
     local = LocalFLC(target, template, mask, max_poses=n_peaks)
     s0 = score_poses(local, t0, a0).cpu().numpy()
+    s_true = score_poses(local, t_true, a_true).cpu().numpy()
     refine_peaks(local, t0[:64], a0[:64], steps=2)  # warm-up
     torch.cuda.synchronize()
     _lib.timing_enable(True)
@@ -210,6 +211,7 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps):
         "ms_per_step": round(elapsed / steps * 1e3, 3),
         "score_start_mean": round(float(np.mean(s0)), 4),
         "score_final_mean": round(float(np.mean(s1)), 4),
+        "score_embedded_pose_mean": round(float(np.mean(s_true)), 4),
         "score_evaluations_per_peak": info["score_evaluations_per_peak"],
         "kernels": kern,
         "start_poses": "embedded copies, N(0,4 deg) per zyz angle, N(0,1.5 voxel) per axis",
