@@ -212,3 +212,39 @@ def test_kept_plan_sees_new_target_values(score):
     s_a2, r_a2 = run(setup_a, cb_a, chunk_a)  # same arrays and plan, target b's values
     assert np.abs(s_a2 - s_b).max() < 1e-6
     assert (r_a2 == r_b).mean() > 0.9999
+
+
+def test_c0_flcsph_parity():
+    """FLCSphericalMask on the 128^3 / 32^3 workload (spherical mask radius
+    14, 5 embedded copies at SNR 0.5): fused and plain paths against the FP64
+    oracle on a rotation subset."""
+    from pytme_amd.synthetic import make_workload
+
+    target, template, mask, rotations, _, spec = make_workload("flcsph_128_t32", n_rotations=6)
+    _, _, raws, oc = orc.scan("FLCSphericalMask", target, template, rotations, template_mask=mask,
+                              score_threshold=0.0, return_raw=True)
+    for fused in (False, True):
+        print(f"C0 FLCSphericalMask fused={fused}", flush=True)
+        s, off, r, _ = _run("FLCSphericalMask", target, template, mask, rotations, fused, threshold=0.0)
+        n_diff, err = assert_scan_parity(s, r, raws, oc, 0.0)
+        print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+def test_c4_fused_matches_plain_full_size():
+    """Size-independent property at 1024x1024x256 with the 64^3 template and
+    the wedge x CTF filters (20 embedded copies): the fused batched scan and
+    the per-rotation loop agree on every maximum and rotation index."""
+    from pytme_amd.synthetic import make_workload
+
+    target, template, mask, rotations, filters, spec = make_workload("flc_1024x1024x256_t64", copies=20,
+                                                                     n_rotations=4)
+    outs = []
+    for fused in (False, True):
+        print(f"C4 full size fused={fused}", flush=True)
+        s, off, r, mapping = _run("FLC", target, template, mask, rotations, fused, threshold=0.0,
+                                  template_filter=filters[0], target_filter=filters[1])
+        outs.append((s, r))
+    (s0, r0), (s1, r1) = outs
+    assert s0.shape == (1024, 1024, 256)
+    assert np.abs(s0 - s1).max() < 1e-5
+    assert (r0 == r1).mean() > 0.9999
