@@ -12,7 +12,7 @@ import warnings
 from functools import wraps
 from itertools import product
 from math import prod
-from typing import Callable, Dict, Optional, Tuple
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 from joblib import Parallel, delayed
 
@@ -29,6 +29,7 @@ __all__ = [
     "prepare_scan",
     "register_matching_exhaustive",
     "MATCHING_EXHAUSTIVE_REGISTER",
+    "scan_templates",
 ]
 
 
@@ -301,6 +302,51 @@ def scan_subsets(
     if callback_class is not None:
         return callback_class.merge(results, **callback_class_args)
     return None
+
+
+def scan_templates(
+    target,
+    templates: Sequence[Dict],
+    score: str = "FLC",
+    callback_class=None,
+    callback_class_args: Dict = None,
+    **scan_kwargs,
+) -> List[Optional[Tuple]]:
+    """Several templates of uneven size against one target, one result per
+    template (the multi-template schedule the survey's config C2 describes;
+    the reference runs one template per invocation of
+    scripts/match_template.py).
+
+    Each entry of ``templates`` is a dict with ``template`` and optional
+    ``template_mask``, ``rotations``, ``template_filter`` and
+    ``target_filter`` (a ``Compose``).  Every template runs through
+    :func:`scan_subsets` with its own analyzer; on the hip backend each
+    template's fused plan is created once and kept for all its rotations.
+    ``scan_kwargs`` are passed to :func:`scan_subsets`.
+    """
+    from .analyzer import MaxScoreOverRotations
+    from .matching_data import MatchingData
+
+    if score not in MATCHING_EXHAUSTIVE_REGISTER:
+        raise ValueError(f"score has to be one of {', '.join(MATCHING_EXHAUSTIVE_REGISTER.keys())}")
+    setup, scoring = MATCHING_EXHAUSTIVE_REGISTER[score]
+    callback_class = MaxScoreOverRotations if callback_class is None else callback_class
+    callback_class_args = {} if callback_class_args is None else callback_class_args
+    results = []
+    for entry in templates:
+        if "template" not in entry:
+            raise ValueError("every template entry needs a 'template'")
+        md = MatchingData(target, entry["template"], template_mask=entry.get("template_mask"),
+                          rotations=entry.get("rotations"))
+        if entry.get("template_filter") is not None:
+            md.template_filter = entry["template_filter"]
+        if entry.get("target_filter") is not None:
+            md.target_filter = entry["target_filter"]
+        results.append(
+            scan_subsets(matching_data=md, matching_setup=setup, matching_score=scoring,
+                         callback_class=callback_class, callback_class_args=callback_class_args, **scan_kwargs)
+        )
+    return results
 
 
 def register_matching_exhaustive(

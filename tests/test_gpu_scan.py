@@ -248,3 +248,51 @@ def test_c4_fused_matches_plain_full_size():
     assert s0.shape == (1024, 1024, 256)
     assert np.abs(s0 - s1).max() < 1e-5
     assert (r0 == r1).mean() > 0.9999
+
+
+def test_multi_template_schedule():
+    """Four templates of uneven size (24^3, 32^3, 48^3, 64^3 blob densities),
+    each with its own mask and CTF defocus, against one tomogram: every
+    result equals the template's own single-template scan, and each
+    template's strongest peaks sit on its embedded copies."""
+    from pytme_amd.filters import Compose, CTFReconstructed
+    from pytme_amd.matching_exhaustive import scan_templates
+    from pytme_amd.rotations import uniform_rotations
+    from pytme_amd.synthetic import blob_template, threshold_mask
+
+    rng = np.random.default_rng(41)
+    shape = (160, 160, 100)
+    target = rng.standard_normal(shape).astype(np.float32)
+    entries, centres = [], []
+    for k, (T, dz) in enumerate(((24, 20000.0), (32, 25000.0), (48, 30000.0), (64, 35000.0))):
+        tmpl = blob_template(T, n_blobs=10, seed=50 + k)
+        m = threshold_mask(tmpl)
+        cs = []
+        for c in range(2):
+            lo = np.array([T // 2] * 3)
+            hi = np.asarray(shape) - (T - T // 2)
+            ctr = np.array([rng.integers(lo[i], hi[i] + 1) for i in range(3)])
+            box = tuple(slice(int(x - T // 2), int(x - T // 2) + T) for x in ctr)
+            target[box] += 2.0 * tmpl
+            cs.append(ctr)
+        centres.append(cs)
+        ctf = CTFReconstructed(defocus_x=dz, sampling_rate=6.8, acceleration_voltage=300e3,
+                               spherical_aberration=2.7e7, amplitude_contrast=0.07, flip_phase=True)
+        entries.append(dict(template=tmpl, template_mask=m, template_filter=Compose((ctf,)),
+                            rotations=uniform_rotations(8, seed=60 + k, include_identity=True)))
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.backends import backend as be
+
+    be.change_backend("hip", device="cuda", fused=True)
+    print("multi-template schedule, 4 templates", flush=True)
+    kw = dict(callback_class=MaxScoreOverRotations, callback_class_args={"score_threshold": 0.0},
+              job_schedule=(1, 1), interpolation_order=1, pad_template_filter=False)
+    multi = scan_templates(target, entries, score="FLC", **kw)
+    for k, entry in enumerate(entries):
+        single = _run("FLC", target, entry["template"], entry["template_mask"], entry["rotations"], True,
+                      threshold=0.0, template_filter=entry["template_filter"])
+        assert np.array_equal(multi[k][0], single[0]) and np.array_equal(multi[k][2], single[2])
+        s = multi[k][0]
+        for ctr in centres[k]:
+            win = tuple(slice(int(x) - 2, int(x) + 3) for x in ctr)
+            assert s[win].max() >= np.sort(s.ravel())[-200], (k, ctr)
