@@ -296,3 +296,16 @@ def test_multi_template_schedule():
         for ctr in centres[k]:
             win = tuple(slice(int(x) - 2, int(x) + 3) for x in ctr)
             assert s[win].max() >= np.sort(s.ravel())[-200], (k, ctr)
+
+
+def test_fused_partial_batches_and_plan_reuse():
+    """20 rotations in batches of 8 (8, 8, 4): the kept plan, the partial
+    last batch and the rotation index assignment against the FP64 oracle."""
+    target, template, mask, rots = _workload(seed=51, n_rot=20)
+    print("fused partial batches 8+8+4", flush=True)
+    s, off, r, mapping = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=8)
+    _, _, raws, oc = orc.scan("FLC", target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+    assert len(mapping) == 20
