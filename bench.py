@@ -287,6 +287,12 @@ def main():
                for k, v in sorted(report.items(), key=lambda kv: -kv[1][0])} if report else {}
 
     extra = {}
+    if args.path == "fused":
+        # per-kernel events route each batch through eager launches; this
+        # pass times the same steps without them (graph replay when the
+        # backend's scan_graph option is on)
+        g_el, _ = time_steps(scoring, setup, proxy, rot_dev, batch, args.steps, 1)
+        extra["untimed_kernels_rotations_per_s"] = round(n_rot / g_el, 2)
     if args.compare_plain and args.path == "fused":
         from pytme_amd.backends import backend as be
 

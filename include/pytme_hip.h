@@ -128,6 +128,11 @@ int tm_scan_plan_destroy(tm_scan_plan* plan);
 int tm_scan_plan_set_target(tm_scan_plan* plan, const void* ft_target, const void* ft_target2, void* stream);
 int tm_scan_batch(tm_scan_plan* plan, const float* rotations, const int* rotation_indices, int nbatch,
                   float* max_scores, int* max_rotations, void* stream);
+/* No counterpart in the reference: replay the batch's kernel sequence as a
+ * hipGraph on a plan-owned stream (event-ordered with the caller's stream);
+ * the first batch of a (batch size, state arrays) configuration runs eagerly,
+ * the second is captured.  Off by default; per-kernel timing forces eager. */
+int tm_scan_plan_use_graph(tm_scan_plan* plan, int enable);
 
 /* ---- local FLC refinement --------------------------------------------------
  * tm_flc_local_fwd replaces the score evaluation of the derivative-free

@@ -88,6 +88,8 @@ class HipBackend(MatchingBackend):
         forces the plain per-rotation loop.
     rotation_batch : int, optional
         Rotations per fused launch; chosen automatically by default.
+    scan_graph : bool, optional
+        Replay each fused batch's kernel sequence as a hipGraph (default True).
     """
 
     #: spline order the rotation kernel implements (reference CLI default for
@@ -103,6 +105,7 @@ class HipBackend(MatchingBackend):
         overflow_safe_dtype=None,
         fused: bool = True,
         rotation_batch: int = None,
+        scan_graph: bool = True,
         **kwargs,
     ):
         import torch
@@ -119,7 +122,9 @@ class HipBackend(MatchingBackend):
         self._gpu = self.device.type == "cuda"
         self.fused = bool(fused)
         self.rotation_batch = rotation_batch
-        self._init_kwargs = dict(device=device, fused=fused, rotation_batch=rotation_batch, **kwargs)
+        self.scan_graph = bool(scan_graph)
+        self._init_kwargs = dict(device=device, fused=fused, rotation_batch=rotation_batch, scan_graph=scan_graph,
+                                 **kwargs)
 
     # ------------------------------------------------------------------ infra
     def _require_gpu(self, what: str):

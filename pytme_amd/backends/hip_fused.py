@@ -136,7 +136,7 @@ def run(backend, score: str, **kw):
     keep = (templ, mask, filt, ft_target, ft2, inv_den)
     key = (
         t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,
-        tuple(0 if x is None else x.data_ptr() for x in keep),
+        tuple(0 if x is None else x.data_ptr() for x in keep), bool(getattr(backend, "scan_graph", True)),
     )
     ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
     stream = backend._stream()
@@ -148,6 +148,7 @@ def run(backend, score: str, **kw):
             ptr(filt), ptr(ft_target), ptr(ft2), ptr(inv_den), normalize, mask_sum, B, ctypes.byref(handle),
         )
         _PLAN.update(key=key, handle=handle.value, keep=keep)
+        _lib.call("tm_scan_plan_use_graph", _PLAN["handle"], int(getattr(backend, "scan_graph", True)))
     else:
         # the plan holds a row-padded copy of the target spectra: refresh it,
         # the caller's arrays may hold new values at the same addresses

@@ -703,6 +703,16 @@ struct tm_scan_plan {
   float *TR, *MR;
   float2 *specT, *Q1, *P, *Q;
   double *n_obs, *rot_ws, *mom_ws, *ms;
+  // graph replay of the batch sequence (kernels only) on a plan stream
+  int use_graph;
+  hipStream_t gs;
+  hipEvent_t ev_in, ev_out;
+  float* rbuf;  // rotations and analyzer indices of the batch, read by the graph
+  int* ibuf;
+  hipGraphExec_t gexec;
+  int g_nb, g_warm;
+  float* g_ms;
+  int* g_mr;
 };
 
 extern "C" int tm_fft_plan_create(int ndim, const int64_t* shape, int64_t batch, tm_fft_plan** out);
@@ -710,9 +720,14 @@ extern "C" int tm_fft_plan_destroy(tm_fft_plan* p);
 
 static void scan_plan_free(tm_scan_plan* p) {
   if (!p) return;
-  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->n_obs, p->rot_ws, p->mom_ws, p->ms, p->ftp, p->ft2p};
+  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->n_obs, p->rot_ws, p->mom_ws, p->ms, p->ftp, p->ft2p,
+                  p->rbuf, p->ibuf};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (p->gexec) hipGraphExecDestroy(p->gexec);
+  if (p->ev_in) hipEventDestroy(p->ev_in);
+  if (p->ev_out) hipEventDestroy(p->ev_out);
+  if (p->gs) hipStreamDestroy(p->gs);
   if (p->fast) tm_fft_plan_destroy(p->fast);
   if (p->tplan) tm_fft_plan_destroy(p->tplan);
   free(p);
@@ -867,11 +882,90 @@ int tm_scan_plan_destroy(tm_scan_plan* p) {
 // Score rotations rots_dev[0..nbatch) (float32 3x3 each, device) with
 // analyzer indices rot_index_dev[0..nbatch) (device int32) and fold them into
 // max_scores / max_rot (fast_shape, device) with the strict '>' rule.
+static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch,
+                             float* max_scores, int* max_rot, hipStream_t s);
+
+// Replay the batch's kernel sequence as a hipGraph on the plan's stream,
+// ordered after and before the caller's stream by events.  The graph reads
+// the rotations and indices from plan buffers, so only the state arrays and
+// the batch size key it; the first batch of a configuration runs eagerly
+// (it also settles every cached launch attribute), the second captures.
+static int scan_batch_graph(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch,
+                            float* max_scores, int* max_rot, hipStream_t s) {
+  TM_CHECK_HIP(hipEventRecord(p->ev_in, s));
+  TM_CHECK_HIP(hipStreamWaitEvent(p->gs, p->ev_in, 0));
+  TM_CHECK_HIP(hipMemcpyAsync(p->rbuf, rots_dev, sizeof(float) * 9 * nbatch, hipMemcpyDeviceToDevice, p->gs));
+  TM_CHECK_HIP(hipMemcpyAsync(p->ibuf, rot_index_dev, sizeof(int) * nbatch, hipMemcpyDeviceToDevice, p->gs));
+  const bool same = p->gexec && p->g_nb == nbatch && p->g_ms == max_scores && p->g_mr == max_rot;
+  int rc = TM_OK;
+  if (!same) {
+    if (p->gexec) {
+      TM_CHECK_HIP(hipGraphExecDestroy(p->gexec));
+      p->gexec = nullptr;
+    }
+    if (!p->g_warm || p->g_nb != nbatch || p->g_ms != max_scores || p->g_mr != max_rot) {
+      p->g_warm = 1;
+      p->g_nb = nbatch;
+      p->g_ms = max_scores;
+      p->g_mr = max_rot;
+      rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch, max_scores, max_rot, p->gs);
+    } else {
+      hipGraph_t g = nullptr;
+      TM_CHECK_HIP(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
+      rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch, max_scores, max_rot, p->gs);
+      const hipError_t e = hipStreamEndCapture(p->gs, &g);
+      if (rc == TM_OK && e != hipSuccess) {
+        tm_set_error("capturing the batch graph: %s", hipGetErrorString(e));
+        rc = TM_ERR_HIP;
+      }
+      if (rc == TM_OK) {
+        const hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+          tm_set_error("instantiating the batch graph: %s", hipGetErrorString(ie));
+          p->gexec = nullptr;
+          rc = TM_ERR_HIP;
+        }
+      }
+      if (g) hipGraphDestroy(g);
+      if (rc == TM_OK) rc = hipGraphLaunch(p->gexec, p->gs) == hipSuccess ? TM_OK : TM_ERR_HIP;
+    }
+  } else {
+    TM_CHECK_HIP(hipGraphLaunch(p->gexec, p->gs));
+  }
+  if (rc != TM_OK) return rc;
+  TM_CHECK_HIP(hipEventRecord(p->ev_out, p->gs));
+  TM_CHECK_HIP(hipStreamWaitEvent(s, p->ev_out, 0));
+  return TM_OK;
+}
+
 int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch, float* max_scores,
                   int* max_rot, void* stream) {
   TM_REQUIRE(p && rots_dev && rot_index_dev && max_scores && max_rot, "null argument to tm_scan_batch");
   TM_REQUIRE(nbatch >= 1 && nbatch <= p->max_batch, "batch %d outside [1, %d]", nbatch, p->max_batch);
-  hipStream_t s = (hipStream_t)stream;
+  // per-kernel event timing records events inside the sequence: run eagerly
+  if (p->use_graph && !tm_timing_on())
+    return scan_batch_graph(p, rots_dev, rot_index_dev, nbatch, max_scores, max_rot, (hipStream_t)stream);
+  return scan_batch_launch(p, rots_dev, rot_index_dev, nbatch, max_scores, max_rot, (hipStream_t)stream);
+}
+
+// Enable (1) or disable (0) the graph replay of the batch sequence.
+int tm_scan_plan_use_graph(tm_scan_plan* p, int enable) {
+  TM_REQUIRE(p, "null plan");
+  if (enable && !p->gs) {
+    TM_CHECK_HIP(hipStreamCreateWithFlags(&p->gs, hipStreamNonBlocking));
+    TM_CHECK_HIP(hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming));
+    TM_CHECK_HIP(hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming));
+    TM_CHECK_HIP(hipMalloc((void**)&p->rbuf, sizeof(float) * 9 * p->max_batch));
+    TM_CHECK_HIP(hipMalloc((void**)&p->ibuf, sizeof(int) * p->max_batch));
+  }
+  p->use_graph = enable ? 1 : 0;
+  return TM_OK;
+}
+
+}  // extern "C"
+
+static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch,
+                             float* max_scores, int* max_rot, hipStream_t s) {
   const int B = nbatch;
   const int T0 = p->T[0], T1 = p->T[1], T2 = p->T[2];
   const int64_t T3 = (int64_t)T0 * T1 * T2;
@@ -889,7 +983,7 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
     rc = tm_fft_r2c_batch(p->tplan, B, p->TR, p->specT, s);
     if (rc != TM_OK) return rc;
     const int64_t nf = (int64_t)T0 * T1 * p->T2c;
-    rc = tm_cmul_cr(p->specT, p->filter, p->specT, nf * B, nf, stream);
+    rc = tm_cmul_cr(p->specT, p->filter, p->specT, nf * B, nf, (void*)s);
     if (rc != TM_OK) return rc;
     rc = tm_fft_c2r_batch(p->tplan, B, p->specT, p->TR, s);
     if (rc != TM_OK) return rc;
@@ -1014,5 +1108,3 @@ int tm_scan_batch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_d
   }
   return TM_OK;
 }
-
-}  // extern "C"

@@ -309,3 +309,60 @@ def test_fused_partial_batches_and_plan_reuse():
     n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
     assert len(mapping) == 20
+
+
+def test_graph_replays_match_eager_batches():
+    """The fused batch replayed as a hipGraph: every replay (batch 3 onwards;
+    batch 1 runs eagerly, batch 2 is captured) equals the eager batch, with
+    eager torch work and host synchronisation between the replays."""
+    import ctypes
+
+    import torch
+
+    from pytme_amd import _lib
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+    from pytme_amd.backends import backend as be
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+
+    be.change_backend("hip", device="cuda", fused=True)
+    target, template, mask, rots = _workload(seed=61, shape=(48, 44, 40), n_rot=40)
+    setup_fn, _ = MATCHING_EXHAUSTIVE_REGISTER["FLC"]
+    md = MatchingData(target, template, template_mask=mask, rotations=rots)
+    setup, cb_args, chunks = prepare_scan(md, setup_fn, n_jobs=1, callback_class_args={"score_threshold": -1.0},
+                                          pad_target=False, pad_template_filter=False, interpolation_order=1)
+    f = lambda k: be.from_sharedarr(setup[k])  # noqa: E731
+    templ = f("template").float().contiguous()
+    msk = f("template_mask").float().contiguous()
+    ft, ft2 = f("ft_target").contiguous(), f("ft_target2").contiguous()
+    fast = tuple(int(x) for x in setup["fast_shape"])
+    B = 8
+    plans = []
+    for graph in (0, 1):
+        h = ctypes.c_void_p()
+        _lib.call("tm_scan_plan_create", 0, _lib.i64_array(fast), _lib.i32_array(tuple(templ.shape)),
+                  templ.data_ptr(), msk.data_ptr(), None, ft.data_ptr(), ft2.data_ptr(), None, 0, 0.0, B,
+                  ctypes.byref(h))
+        _lib.call("tm_scan_plan_use_graph", h.value, graph)
+        plans.append(h.value)
+    r = chunks[0].float().contiguous()
+    idx = torch.arange(r.shape[0], dtype=torch.int32, device="cuda")
+    ms = torch.empty(fast, device="cuda")
+    mr = torch.empty(fast, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    try:
+        for k in range(r.shape[0] // B):
+            outs = []
+            for plan in plans:
+                ms.fill_(-1.0)
+                mr.fill_(-1)
+                _lib.call("tm_scan_batch", plan, r[k * B].data_ptr(), idx[k * B].data_ptr(), B, ms.data_ptr(),
+                          mr.data_ptr(), stream)
+                torch.cuda.synchronize()
+                outs.append((ms.cpu().numpy().copy(), mr.cpu().numpy().copy()))
+            print(f"batch {k}: graph vs eager", flush=True)
+            assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1]), k
+    finally:
+        for plan in plans:
+            _lib.call("tm_scan_plan_destroy", plan)
