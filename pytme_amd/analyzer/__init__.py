@@ -4,6 +4,7 @@ from .base import AbstractAnalyzer
 from .aggregation import MaxScoreOverRotations
 from .proxy import SharedAnalyzerProxy, StatelessSharedAnalyzerProxy
 from ._utils import cart_to_score, score_to_cart
+from .peaks import PeakCaller, PeakCallerMaximumFilter, PeakCallerSort, filter_points, filter_points_indices
 
 __all__ = [
     "AbstractAnalyzer",
@@ -12,4 +13,9 @@ __all__ = [
     "StatelessSharedAnalyzerProxy",
     "cart_to_score",
     "score_to_cart",
+    "PeakCaller",
+    "PeakCallerSort",
+    "PeakCallerMaximumFilter",
+    "filter_points",
+    "filter_points_indices",
 ]

@@ -17,6 +17,9 @@ backend method                  kernel(s)
 ``norm_scores``                 ``tm_norm_scores``
 ``max_score_over_rotations``    ``tm_max_update``
 ``fill``                        ``tm_fill_f32`` / ``tm_fill_i32``
+``max_filter_coordinates``      ``tm_peaks_max_filter`` + ``tm_peaks_write``
+``topk_indices``                ``tm_topk_f32``
+``greedy_distance_filter``      ``tm_peak_filter``
 ==============================  ==============================================
 
 The backend mirrors ``tme/backends/pytorch_backend.py`` in its interface
@@ -759,6 +762,94 @@ class HipBackend(MatchingBackend):
             scores.numel(), self._stream(),
         )
         return max_scores, rotations
+
+    # --------------------------------------------------------- peak extraction
+    def max_filter_coordinates(self, score_space, min_distance):
+        """Coordinates (n, 3) int64, in C order, of the voxels equal to the
+        maximum of their box of edge ``min_distance`` with zero outside the
+        volume (npfftw_backend.py:200-206: scipy ``maximum_filter`` with
+        ``mode="constant"``, then ``nonzero``); ``tm_peaks_max_filter`` and
+        ``tm_peaks_write``.  3-D float32 score volumes."""
+        t = self._torch
+        self._require_gpu("max_filter_coordinates")
+        s = self.to_backend_array(score_space)
+        if s.ndim != 3:
+            raise ValueError(f"hip max_filter_coordinates takes a 3-D score volume, got {s.ndim} dimensions")
+        if s.dtype != t.float32 or not s.is_contiguous():
+            s = s.to(t.float32).contiguous()
+        # scipy treats an edge of 0 as the identity filter, the same as 1
+        size = max(1, int(np.max(min_distance)))
+        F = s.numel()
+        nblk = -(-F // 256)
+        tmp = t.empty(F, dtype=t.float32, device=s.device)
+        mask = t.empty(4 * nblk, dtype=t.int64, device=s.device)
+        cnt = t.empty(nblk, dtype=t.int64, device=s.device)
+        off = t.empty(nblk + 1, dtype=t.int64, device=s.device)
+        shape = _lib.i64_array(s.shape)
+        _lib.call("tm_peaks_max_filter", s.data_ptr(), shape, size, tmp.data_ptr(), mask.data_ptr(), cnt.data_ptr(),
+                  off.data_ptr(), self._stream())
+        n = int(off[nblk].item())
+        coords = t.empty((n, 3), dtype=t.int64, device=s.device)
+        if n:
+            _lib.call("tm_peaks_write", mask.data_ptr(), off.data_ptr(), shape, coords.data_ptr(), self._stream())
+        return coords
+
+    def unravel_index(self, indices, shape):
+        return self._torch.unravel_index(self.to_backend_array(indices), tuple(int(x) for x in shape))
+
+    def topk_indices(self, arr, k: int):
+        """Indices of the k largest entries, largest first, as unravelled
+        index arrays (npfftw_backend.py:176-181); ties go to the lower flat
+        index and NaN ranks above every number; ``tm_topk_f32``."""
+        t = self._torch
+        self._require_gpu("topk_indices")
+        flat = self.to_backend_array(arr).reshape(-1)
+        if flat.dtype != t.float32 or not flat.is_contiguous():
+            flat = flat.to(t.float32).contiguous()
+        k = int(min(int(k), flat.numel()))
+        if k <= 0:
+            return self.unravel_index(t.empty(0, dtype=t.int64, device=flat.device), arr.shape)
+        nbytes = int(_lib.lib().tm_topk_workspace_bytes(flat.numel(), k))
+        key = (t.cuda.current_device(), "topk")
+        ws = _WORKSPACES.get(key)
+        if ws is None or ws.numel() < nbytes:
+            ws = t.empty(nbytes, dtype=t.uint8, device=flat.device)
+            _WORKSPACES[key] = ws
+        out = t.empty(k, dtype=t.int64, device=flat.device)
+        _lib.call("tm_topk_f32", flat.data_ptr(), flat.numel(), k, out.data_ptr(), ws.data_ptr(), nbytes,
+                  self._stream())
+        return self.unravel_index(out, arr.shape)
+
+    #: kept points the greedy distance filter holds on chip
+    max_filter_keep = 4096
+
+    def greedy_distance_filter(self, coordinates, min_distance: int, max_keep: int = None):
+        """Row indices kept by the greedy minimum-distance pass over integer
+        coordinates taken in order (the int64 ``find_candidate_indices`` of
+        tme/external/bindings.cpp:84-124: a row is dropped when
+        ``trunc(sqrt(d2)) <= min_distance`` for a kept row); at most
+        ``max_keep`` (<= 4096) rows; ``tm_peak_filter``."""
+        t = self._torch
+        self._require_gpu("greedy_distance_filter")
+        c = self.to_backend_array(coordinates)
+        if c.ndim != 2 or c.shape[1] != 3:
+            raise ValueError("greedy_distance_filter takes (n, 3) coordinates")
+        c = c.to(t.int64).contiguous()
+        n = int(c.shape[0])
+        cap = self.max_filter_keep
+        limit = cap if max_keep is None else int(max_keep)
+        if limit > cap:
+            raise ValueError(f"the hip distance filter keeps at most {cap} points, asked for {limit}")
+        keep = t.empty(max(1, min(n, limit)), dtype=t.int64, device=c.device)
+        nkeep = t.zeros(1, dtype=t.int64, device=c.device)
+        if n == 0:
+            return keep[:0]
+        _lib.call("tm_peak_filter", c.data_ptr(), n, int(min_distance), min(n, limit), keep.data_ptr(),
+                  nkeep.data_ptr(), self._stream())
+        k = int(nkeep.item())
+        if max_keep is None and k == cap and int(keep[k - 1].item()) < n - 1:
+            raise ValueError(f"more than {cap} points would be kept; pass max_keep")
+        return keep[:k]
 
     # ------------------------------------------------------------- fused scan
     def fused_scoring(self, **kwargs):
