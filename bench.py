@@ -60,6 +60,9 @@ def parse():
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a 1 GiB device copy first (reference dispatch for HBM byte counters)")
     p.add_argument("--no-refinement", action="store_true", help="skip the gradient-refinement measurement")
+    p.add_argument("--no-peaks", action="store_true", help="skip the peak-calling measurement")
+    p.add_argument("--num-peaks", type=int, default=1000)
+    p.add_argument("--min-distance", type=int, default=5)
     p.add_argument("--refine-peaks", type=int, default=4096)
     p.add_argument("--refine-steps", type=int, default=50)
     return p.parse_args()
@@ -218,6 +221,67 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps):
     }
 
 
+def bench_peaks(scores, num_peaks, min_distance, reps, with_cpu):
+    """Peak calling on the scan's max-score volume: PeakCallerMaximumFilter
+    (box maximum filter + peak compaction, top-2k preselection, greedy
+    distance filter) then result(); ms per tomogram with the kernels' HIP
+    event times, and the oracle on the same volume (one host core) as the
+    CPU baseline and parity check."""
+    import torch
+
+    from pytme_amd import _lib
+    from pytme_amd.analyzer import PeakCallerMaximumFilter
+
+    kw = dict(shape=tuple(scores.shape), num_peaks=num_peaks, min_distance=min_distance)
+    pc = PeakCallerMaximumFilter(**kw)
+    res = pc.result(pc(pc.init_state(), scores, rotation_matrix=np.eye(3)))  # warm-up
+    torch.cuda.synchronize()
+    _lib.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = pc.result(pc(pc.init_state(), scores, rotation_matrix=np.eye(3)))
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / reps
+    report = _lib.timing_report()
+    _lib.timing_enable(False)
+    kernels = {k: {"us_per_launch": round(v[0] * 1e3 / max(v[1], 1), 1), "launches": v[1],
+                   "GB/s": round(v[2] / max(v[0], 1e-9) / 1e6, 1)}
+               for k, v in sorted(report.items(), key=lambda kv: -kv[1][0])}
+    name = "peaks_max_filter_z"
+    roofline = None
+    if name in report:
+        ms, count, by = report[name]
+        achieved = by / (ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": by / max(count, 1)}
+    rec = {
+        "metric": "peak calling ms per tomogram (PeakCallerMaximumFilter on the max-score volume)",
+        "value": round(elapsed * 1e3, 3),
+        "unit": "ms",
+        "higher_is_better": False,
+        "volume": list(scores.shape),
+        "num_peaks": num_peaks,
+        "min_distance": min_distance,
+        "peaks_found": int(res[0].shape[0]),
+        "roofline": roofline,
+        "kernels": kernels,
+    }
+    if with_cpu:
+        from oracle import peaks as op
+
+        host = scores.cpu().numpy()
+        ref = op.PeakCallerOracle("maximum_filter", **kw)
+        t0 = time.perf_counter()
+        want = ref.result(ref(ref.init_state(), host))
+        cpu_s = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(cpu_s * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the same volume, scipy maximum_filter + numpy restatement of the "
+                               "reference numpy path"}
+        rec["matches_oracle"] = bool(all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(res, want)))
+    return rec
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -331,6 +395,9 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
+    if not args.no_peaks and args.path == "fused":
+        out["peak_calling"] = bench_peaks(proxy._state[0], args.num_peaks, args.min_distance, 5,
+                                          not args.no_cpu_baseline)
     if not args.no_refinement and args.workload.startswith("flc_512"):
         out["refinement"] = bench_refinement(target, template, mask, spec, args.refine_peaks, args.refine_steps)
     out.update(extra)

@@ -162,7 +162,7 @@ int tm_flc_local_bwd(tm_flc_local_plan* plan, const float* target, const int64_t
  * primitive (tme/analyzer/peaks.py:662-670): scipy maximum_filter of edge
  * size (1..64) with mode="constant" (zero borders), peaks where the score
  * equals the filtered score, as int64 (z, y, x) rows in C order.  Buffers of
- * a volume of F voxels, nblk = ceil(F / 256): tmp F floats, mask 4 nblk
+ * a volume of F voxels, nblk = ceil(F / 4096): tmp F floats, mask 64 nblk
  * words, cnt nblk and off nblk + 1 int64; off[nblk] is the peak count.
  * tm_topk_f32 replaces topk_indices (npfftw_backend.py:176-181, used by
  * PeakCaller._top_peaks and PeakCallerSort, peaks.py:501-529, 644-659):
@@ -171,8 +171,9 @@ int tm_flc_local_bwd(tm_flc_local_plan* plan, const float* target, const int64_t
  * tm_peak_filter replaces the greedy pass of filter_points_indices
  * (peaks.py:68-106 with find_candidate_indices, tme/external/bindings.cpp:
  * 84-124, int64 coordinates): rows taken in the given order are kept unless
- * trunc(sqrt(d2)) <= min_distance to a kept row; stops at max_keep (<= 4096)
- * kept rows. */
+ * trunc(sqrt(d2)) <= min_distance to a kept row; returns the first max_keep
+ * kept rows (n <= 16384: any max_keep, workspace from
+ * tm_peak_filter_workspace_bytes; larger n: max_keep <= 4096). */
 int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, float* tmp, unsigned long long* mask,
                         int64_t* cnt, int64_t* off, void* stream);
 int tm_peaks_write(const unsigned long long* mask, const int64_t* off, const int64_t* shape, int64_t* coords,
@@ -180,8 +181,9 @@ int tm_peaks_write(const unsigned long long* mask, const int64_t* off, const int
 int64_t tm_topk_workspace_bytes(int64_t n, int64_t k);
 int tm_topk_f32(const float* values, int64_t n, int64_t k, int64_t* out, void* work, int64_t work_bytes,
                 void* stream);
+int64_t tm_peak_filter_workspace_bytes(int64_t n);
 int tm_peak_filter(const int64_t* coords, int64_t n, int64_t min_distance, int64_t max_keep, int64_t* keep,
-                   int64_t* n_keep, void* stream);
+                   int64_t* n_keep, void* work, int64_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -87,7 +87,10 @@ SIGNATURES = {
     "tm_peaks_write": (c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p]),
     "tm_topk_workspace_bytes": (c_int64, [c_int64, c_int64]),
     "tm_topk_f32": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64, c_void_p]),
-    "tm_peak_filter": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p]),
+    "tm_peak_filter_workspace_bytes": (c_int64, [c_int64]),
+    "tm_peak_filter": (
+        c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]
+    ),
 }
 
 _LIB = None

@@ -780,9 +780,9 @@ class HipBackend(MatchingBackend):
         # scipy treats an edge of 0 as the identity filter, the same as 1
         size = max(1, int(np.max(min_distance)))
         F = s.numel()
-        nblk = -(-F // 256)
+        nblk = -(-F // 4096)
         tmp = t.empty(F, dtype=t.float32, device=s.device)
-        mask = t.empty(4 * nblk, dtype=t.int64, device=s.device)
+        mask = t.empty(64 * nblk, dtype=t.int64, device=s.device)
         cnt = t.empty(nblk, dtype=t.int64, device=s.device)
         off = t.empty(nblk + 1, dtype=t.int64, device=s.device)
         shape = _lib.i64_array(s.shape)
@@ -820,15 +820,17 @@ class HipBackend(MatchingBackend):
                   self._stream())
         return self.unravel_index(out, arr.shape)
 
-    #: kept points the greedy distance filter holds on chip
-    max_filter_keep = 4096
+    #: rows the distance filter resolves in parallel; beyond this it keeps at
+    #: most ``sequential_keep`` rows
+    parallel_filter_rows = 16384
+    sequential_keep = 4096
 
     def greedy_distance_filter(self, coordinates, min_distance: int, max_keep: int = None):
         """Row indices kept by the greedy minimum-distance pass over integer
         coordinates taken in order (the int64 ``find_candidate_indices`` of
         tme/external/bindings.cpp:84-124: a row is dropped when
-        ``trunc(sqrt(d2)) <= min_distance`` for a kept row); at most
-        ``max_keep`` (<= 4096) rows; ``tm_peak_filter``."""
+        ``trunc(sqrt(d2)) <= min_distance`` for a kept row), the first
+        ``max_keep`` of them; ``tm_peak_filter``."""
         t = self._torch
         self._require_gpu("greedy_distance_filter")
         c = self.to_backend_array(coordinates)
@@ -836,20 +838,25 @@ class HipBackend(MatchingBackend):
             raise ValueError("greedy_distance_filter takes (n, 3) coordinates")
         c = c.to(t.int64).contiguous()
         n = int(c.shape[0])
-        cap = self.max_filter_keep
-        limit = cap if max_keep is None else int(max_keep)
-        if limit > cap:
-            raise ValueError(f"the hip distance filter keeps at most {cap} points, asked for {limit}")
-        keep = t.empty(max(1, min(n, limit)), dtype=t.int64, device=c.device)
-        nkeep = t.zeros(1, dtype=t.int64, device=c.device)
         if n == 0:
-            return keep[:0]
-        _lib.call("tm_peak_filter", c.data_ptr(), n, int(min_distance), min(n, limit), keep.data_ptr(),
-                  nkeep.data_ptr(), self._stream())
-        k = int(nkeep.item())
-        if max_keep is None and k == cap and int(keep[k - 1].item()) < n - 1:
-            raise ValueError(f"more than {cap} points would be kept; pass max_keep")
-        return keep[:k]
+            return t.empty(0, dtype=t.int64, device=c.device)
+        limit = n if max_keep is None else min(n, int(max_keep))
+        if n > self.parallel_filter_rows and limit > self.sequential_keep:
+            raise ValueError(
+                f"the hip distance filter keeps at most {self.sequential_keep} of more than "
+                f"{self.parallel_filter_rows} rows; pass max_keep"
+            )
+        nbytes = int(_lib.lib().tm_peak_filter_workspace_bytes(n))
+        key = (t.cuda.current_device(), "peak_filter")
+        ws = _WORKSPACES.get(key)
+        if ws is None or ws.numel() < nbytes:
+            ws = t.empty(nbytes, dtype=t.uint8, device=c.device)
+            _WORKSPACES[key] = ws
+        keep = t.empty(limit, dtype=t.int64, device=c.device)
+        nkeep = t.zeros(1, dtype=t.int64, device=c.device)
+        _lib.call("tm_peak_filter", c.data_ptr(), n, int(min_distance), limit, keep.data_ptr(), nkeep.data_ptr(),
+                  ws.data_ptr(), nbytes, self._stream())
+        return keep[: int(nkeep.item())]
 
     # ------------------------------------------------------------- fused scan
     def fused_scoring(self, **kwargs):

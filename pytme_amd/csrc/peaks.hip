@@ -30,35 +30,43 @@
 #define PK_SORT_MAX 65536              // larger sorts run on a global buffer
 #define PK_KEEP_MAX 4096               // kept points of the distance filter
 #define PK_TOPK_ITEMS 16               // elements per thread in the top-k scans
+#define PK_MF_ITEMS 16                 // voxels per thread in the peak test and write
+#define PK_MF_CHUNK (PK_THREADS * PK_MF_ITEMS)
+#define PK_CF_MAX 8                    // earlier conflicts listed per candidate
+#define PK_PAR_MAX 16384               // candidates the parallel distance filter takes
 #define PK_TOPK_CHUNK (PK_THREADS * PK_TOPK_ITEMS)
 
 // ---------------------------------------------------------------------------
 // shared helpers
 
 // exclusive scan of n int64 counts into off[0..n], off[n] = total; one block
-// of 1024 threads, each owning a contiguous segment
+// of 1024 threads walking the counts in coalesced tiles of 1024
 __global__ __launch_bounds__(1024) void k_pk_scan(const int64_t* __restrict__ cnt, int64_t* __restrict__ off,
                                                   int64_t n) {
-  __shared__ int64_t part[1024];
-  const int t = threadIdx.x;
-  const int64_t seg = (n + 1023) / 1024;
-  const int64_t lo = min(n, seg * t), hi = min(n, lo + seg);
-  int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += cnt[i];
-  part[t] = s;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    int64_t v = t >= d ? part[t - d] : 0;
+  __shared__ int64_t wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int64_t run = 0;
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t i = base + t;
+    const int64_t v = i < n ? cnt[i] : 0;
+    int64_t x = v;  // inclusive scan within the wave
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t y = __shfl_up(x, d);
+      if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    part[t] += v;
+    int64_t wb = 0, tot = 0;
+    for (int k = 0; k < 16; ++k) {
+      const int64_t q = wsum[k];
+      if (k < w) wb += q;
+      tot += q;
+    }
+    if (i < n) off[i] = run + wb + x - v;
+    run += tot;
     __syncthreads();
   }
-  int64_t run = part[t] - s;  // exclusive prefix of this segment
-  for (int64_t i = lo; i < hi; ++i) {
-    off[i] = run;
-    run += cnt[i];
-  }
-  if (t == 1023) off[n] = part[1023];
+  if (t == 0) off[n] = run;
 }
 
 // rank of this lane's flag among the set flags of the block (in thread
@@ -117,39 +125,42 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
   }
 }
 
-// z maximum of the xy maxima, the peak test, one ballot word per 64 voxels
+// z maximum of the xy maxima and the peak test over a block of 4096
+// consecutive voxels: one ballot word per 64 voxels (word index = voxel / 64)
 // and the block's peak count
 __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s, const float* __restrict__ mxy,
                                                      unsigned long long* __restrict__ mask,
                                                      int64_t* __restrict__ cnt, int n0, int n1, int n2, int d) {
   __shared__ int wsum[4];
   const int64_t F = (int64_t)n0 * n1 * n2, plane = (int64_t)n1 * n2;
-  const int64_t L = (int64_t)blockIdx.x * PK_THREADS + threadIdx.x;
-  bool peak = false;
-  if (L < F) {
-    const int64_t z = L / plane;
-    const int lo = d / 2;
-    float m = 0.f;
-    bool any_out = false;
-    float mi = -INFINITY;
-    for (int k = 0; k < d; ++k) {
-      const int64_t zz = z - lo + k;
-      if (zz < 0 || zz >= n0) {
-        any_out = true;
-        continue;
-      }
-      mi = fmaxf(mi, mxy[L + (zz - z) * plane]);
-    }
-    m = any_out ? fmaxf(mi, 0.f) : mi;
-    const float v = s[L];
-    peak = (v == m);
-  }
-  const unsigned long long b = __ballot(peak);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    mask[(int64_t)blockIdx.x * 4 + w] = b;
-    wsum[w] = __popcll(b);
+  const int lo = d / 2;
+  int total = 0;
+  for (int it = 0; it < PK_MF_ITEMS; ++it) {
+    const int64_t L = (int64_t)blockIdx.x * PK_MF_CHUNK + it * PK_THREADS + threadIdx.x;
+    bool peak = false;
+    if (L < F) {
+      const int64_t z = L / plane;
+      bool any_out = false;
+      float mi = -INFINITY;
+      for (int k = 0; k < d; ++k) {
+        const int64_t zz = z - lo + k;
+        if (zz < 0 || zz >= n0) {
+          any_out = true;
+          continue;
+        }
+        mi = fmaxf(mi, mxy[L + (zz - z) * plane]);
+      }
+      const float m = any_out ? fmaxf(mi, 0.f) : mi;
+      peak = (s[L] == m);
+    }
+    const unsigned long long b = __ballot(peak);
+    if (lane == 0) {
+      mask[((int64_t)blockIdx.x * PK_MF_CHUNK + it * PK_THREADS) / 64 + w] = b;
+      total += __popcll(b);
+    }
   }
+  if (lane == 0) wsum[w] = total;
   __syncthreads();
   if (threadIdx.x == 0) cnt[blockIdx.x] = (int64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
@@ -160,20 +171,27 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_write(const unsigned long lon
                                                          int n1, int n2, int64_t F) {
   __shared__ int wsum[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const unsigned long long b = mask[(int64_t)blockIdx.x * 4 + w];
-  if (lane == 0) wsum[w] = __popcll(b);
-  __syncthreads();
-  if (!((b >> lane) & 1ull)) return;
-  int base = 0;
-  for (int i = 0; i < w; ++i) base += wsum[i];
-  const int64_t L = (int64_t)blockIdx.x * PK_THREADS + threadIdx.x;
-  if (L >= F) return;
-  const int64_t o = off[blockIdx.x] + base + __popcll(b & ((1ull << lane) - 1ull));
   const int64_t plane = (int64_t)n1 * n2;
-  const int64_t z = L / plane, r = L - z * plane, y = r / n2, x = r - y * n2;
-  xyz[3 * o + 0] = z;
-  xyz[3 * o + 1] = y;
-  xyz[3 * o + 2] = x;
+  int64_t base = off[blockIdx.x];
+  for (int it = 0; it < PK_MF_ITEMS; ++it) {
+    const int64_t L0 = (int64_t)blockIdx.x * PK_MF_CHUNK + it * PK_THREADS;
+    const unsigned long long b = L0 < F ? mask[L0 / 64 + w] : 0ull;
+    if (lane == 0) wsum[w] = __popcll(b);
+    __syncthreads();
+    int wb = 0;
+    for (int i = 0; i < w; ++i) wb += wsum[i];
+    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int64_t L = L0 + threadIdx.x;
+    if (((b >> lane) & 1ull) && L < F) {
+      const int64_t o = base + wb + __popcll(b & ((1ull << lane) - 1ull));
+      const int64_t z = L / plane, r = L - z * plane, y = r / n2, x = r - y * n2;
+      xyz[3 * o + 0] = z;
+      xyz[3 * o + 1] = y;
+      xyz[3 * o + 2] = x;
+    }
+    base += tot;
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -216,19 +234,28 @@ __global__ __launch_bounds__(PK_THREADS) void k_topk_hist(const float* __restric
   if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// picks the digit bin holding the k_rem-th largest remaining key
-__global__ void k_topk_pick(pk_sel_state* st, const unsigned* __restrict__ hist, int shift) {
-  if (threadIdx.x != 0) return;
-  int64_t above = 0;
-  int b = 255;
-  for (; b > 0; --b) {
-    if (above + (int64_t)hist[b] >= st->k_rem) break;
-    above += hist[b];
+// picks the digit bin holding the k_rem-th largest remaining key: the one
+// bin b with (keys in bins above b) < k_rem <= (keys in bins b and above)
+__global__ __launch_bounds__(256) void k_topk_pick(pk_sel_state* st, const unsigned* __restrict__ hist, int shift) {
+  __shared__ int64_t suf[256];
+  const int b = threadIdx.x;
+  const int64_t k_rem = st->k_rem;
+  const int64_t h = hist[b];
+  suf[b] = h;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const int64_t v = b + d < 256 ? suf[b + d] : 0;
+    __syncthreads();
+    suf[b] += v;
+    __syncthreads();
   }
-  st->n_gt += above;
-  st->k_rem -= above;
-  st->prefix |= (unsigned)b << shift;
-  st->pmask |= 255u << shift;
+  const int64_t above = suf[b] - h;
+  if (above < k_rem && k_rem <= suf[b]) {
+    st->n_gt += above;
+    st->k_rem = k_rem - above;
+    st->prefix |= (unsigned)b << shift;
+    st->pmask |= 255u << shift;
+  }
 }
 
 // per-chunk counts of keys above the threshold and equal to it
@@ -325,6 +352,95 @@ __global__ __launch_bounds__(1024) void k_topk_sort(unsigned long long* __restri
 }
 
 // ---------------------------------------------------------------------------
+// distance filter, parallel form.  Greedy selection in order is the fixed
+// point "i is kept iff no kept j < i lies within the distance": a candidate
+// without earlier neighbours is kept; the others are decided in rounds once
+// a neighbour is known kept (drop) or all are known dropped (keep).  Each
+// round decides at least the lowest undecided candidate, so at most n rounds.
+
+__device__ __forceinline__ bool pk_near(const int64_t* __restrict__ a, const int64_t* __restrict__ b, int64_t md,
+                                        int64_t r2) {
+  const int64_t dz = llabs(a[0] - b[0]), dy = llabs(a[1] - b[1]), dx = llabs(a[2] - b[2]);
+  return dz <= md && dy <= md && dx <= md && dz * dz + dy * dy + dx * dx < r2;
+}
+
+// earlier candidates within the distance of each candidate (first PK_CF_MAX
+// listed, all counted)
+__global__ __launch_bounds__(256) void k_pf_conflicts(const int64_t* __restrict__ xyz, int64_t n, int64_t md,
+                                                      int* __restrict__ ncf, int* __restrict__ cf) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int64_t r2 = (md + 1) * (md + 1);
+  const int64_t c[3] = {xyz[3 * i + 0], xyz[3 * i + 1], xyz[3 * i + 2]};
+  int k = 0;
+  for (int64_t j = 0; j < i; ++j) {
+    if (pk_near(c, xyz + 3 * j, md, r2)) {
+      if (k < PK_CF_MAX) cf[i * PK_CF_MAX + k] = (int)j;
+      ++k;
+    }
+  }
+  ncf[i] = k;
+}
+
+// rounds of decisions in LDS (0 open, 1 kept, 2 dropped), then the kept rows
+// in order, at most max_keep
+__global__ __launch_bounds__(1024) void k_pf_resolve(const int64_t* __restrict__ xyz, int64_t n, int64_t md,
+                                                     const int* __restrict__ ncf, const int* __restrict__ cf,
+                                                     int64_t max_keep, int64_t* __restrict__ keep,
+                                                     int64_t* __restrict__ nkeep) {
+  __shared__ unsigned char st[PK_PAR_MAX];
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t r2 = (md + 1) * (md + 1);
+  for (int64_t i = t; i < n; i += 1024) st[i] = ncf[i] == 0 ? 1 : 0;
+  __syncthreads();
+  for (int64_t round = 0; round <= n; ++round) {
+    bool open = false;
+    for (int64_t i = t; i < n; i += 1024) {
+      if (st[i] != 0) continue;
+      const int k = ncf[i];
+      bool kept_nb = false, all_dropped = true;
+      if (k <= PK_CF_MAX) {
+        for (int q = 0; q < k; ++q) {
+          const unsigned char v = st[cf[i * PK_CF_MAX + q]];
+          kept_nb |= v == 1;
+          all_dropped &= v == 2;
+        }
+      } else {
+        for (int64_t j = 0; j < i && !kept_nb; ++j) {
+          if (!pk_near(xyz + 3 * i, xyz + 3 * j, md, r2)) continue;
+          const unsigned char v = st[j];
+          kept_nb |= v == 1;
+          all_dropped &= v == 2;
+        }
+      }
+      if (kept_nb) st[i] = 2;
+      else if (all_dropped) st[i] = 1;
+      else open = true;
+    }
+    if (!__syncthreads_or(open)) break;
+  }
+  int64_t base = 0;
+  for (int64_t b0 = 0; b0 < n && base < max_keep; b0 += 1024) {
+    const int64_t i = b0 + t;
+    const bool kp = i < n && st[i] == 1;
+    const unsigned long long b = __ballot(kp);
+    if (lane == 0) wsum[w] = __popcll(b);
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < w) wb += wsum[q];
+      tot += wsum[q];
+    }
+    const int64_t o = base + wb + __popcll(b & ((1ull << lane) - 1ull));
+    if (kp && o < max_keep) keep[o] = i;
+    base += tot;
+    __syncthreads();
+  }
+  if (t == 0) *nkeep = base < max_keep ? base : max_keep;
+}
+
+// ---------------------------------------------------------------------------
 // greedy distance filter: candidates in the given order; the kept list is
 // spread over the threads (slot j belongs to thread j % 256), so the only
 // cross-thread exchange per candidate is one barrier-or
@@ -369,8 +485,8 @@ extern "C" {
 
 // Peaks of a 3-D float32 volume under a box maximum filter of edge `size`
 // (constant-zero borders), first half: filters, marks and counts.  Buffers:
-// tmp F floats, mask 4*nblk words, off nblk+1 int64 with nblk = ceil(F/256).
-// off[nblk] receives the number of peaks.
+// tmp F floats, mask 64*nblk words, cnt nblk and off nblk+1 int64 with
+// nblk = ceil(F/4096).  off[nblk] receives the number of peaks.
 int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, float* tmp,
                         unsigned long long* mask, int64_t* cnt, int64_t* off, void* stream) {
   TM_REQUIRE(scores && shape && tmp && mask && cnt && off, "tm_peaks_max_filter: null pointer");
@@ -381,7 +497,7 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
   // gridDim.z (<= 65535); coordinates stay far inside int32
   TM_REQUIRE(n0 > 0 && n1 > 0 && n2 > 0 && n0 < 65536 && n1 < (1 << 20) && n2 < (1 << 20),
              "tm_peaks_max_filter: shape must be positive with z < 65536 and y, x < 2^20");
-  const int64_t F = n0 * n1 * n2, nblk = tm_ceil_div(F, PK_THREADS);
+  const int64_t F = n0 * n1 * n2, nblk = tm_ceil_div(F, PK_MF_CHUNK);
   hipStream_t s = (hipStream_t)stream;
   const int hx = PK_MF_TX + size - 1, hy = PK_MF_TY + size - 1;
   const size_t shm = sizeof(float) * (size_t)hy * (hx + PK_MF_TX);
@@ -403,7 +519,7 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
 int tm_peaks_write(const unsigned long long* mask, const int64_t* off, const int64_t* shape, int64_t* coords,
                    void* stream) {
   TM_REQUIRE(mask && off && shape && coords, "tm_peaks_write: null pointer");
-  const int64_t F = shape[0] * shape[1] * shape[2], nblk = tm_ceil_div(F, PK_THREADS);
+  const int64_t F = shape[0] * shape[1] * shape[2], nblk = tm_ceil_div(F, PK_MF_CHUNK);
   hipStream_t s = (hipStream_t)stream;
   TM_TIMED("peaks_write", F / 8.0, s,
            hipLaunchKernelGGL(k_mf_write, dim3((unsigned)nblk), dim3(PK_THREADS), 0, s, mask, off, coords,
@@ -448,7 +564,7 @@ int tm_topk_f32(const float* v, int64_t n, int64_t k, int64_t* out, void* work, 
     TM_TIMED("topk_hist", 4.0 * n, s,
              hipLaunchKernelGGL(k_topk_hist, dim3(hb), dim3(PK_THREADS), 0, s, v, n, st, hist + 256 * p, shift));
     TM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_topk_pick, dim3(1), dim3(64), 0, s, st, hist + 256 * p, shift);
+    hipLaunchKernelGGL(k_topk_pick, dim3(1), dim3(256), 0, s, st, hist + 256 * p, shift);
     TM_CHECK_LAUNCH();
   }
   hipLaunchKernelGGL(k_topk_count, dim3((unsigned)nb), dim3(PK_THREADS), 0, s, v, n, st, cgt, ceq);
@@ -469,16 +585,35 @@ int tm_topk_f32(const float* v, int64_t n, int64_t k, int64_t* out, void* work, 
   return TM_OK;
 }
 
+// Workspace (bytes) of tm_peak_filter for n rows.
+int64_t tm_peak_filter_workspace_bytes(int64_t n) { return (n > 0 ? n : 1) * 4 * (1 + PK_CF_MAX); }
+
 // Greedy minimum-distance filter over n int64 (z, y, x) rows taken in the
-// given order: writes the kept row indices to keep[0..*nkeep) and stops once
-// max_keep rows are kept (max_keep <= 4096).  nkeep is a device int64.
+// given order: writes the kept row indices to keep[0..*nkeep), at most
+// max_keep of them.  Up to 16384 rows are resolved in parallel (any
+// max_keep); beyond that a one-block sequential pass keeps at most 4096.
+// nkeep is a device int64.
 int tm_peak_filter(const int64_t* coords, int64_t n, int64_t min_distance, int64_t max_keep, int64_t* keep,
-                   int64_t* nkeep, void* stream) {
+                   int64_t* nkeep, void* work, int64_t work_bytes, void* stream) {
   TM_REQUIRE(coords && keep && nkeep, "tm_peak_filter: null pointer");
-  TM_REQUIRE(n >= 0, "tm_peak_filter: negative count");
+  TM_REQUIRE(n >= 1, "tm_peak_filter: needs at least one row");
   TM_REQUIRE(min_distance >= 0 && min_distance < (1 << 20), "tm_peak_filter: min_distance out of range");
-  TM_REQUIRE(max_keep >= 1 && max_keep <= PK_KEEP_MAX, "tm_peak_filter: max_keep must be in [1, %d]", PK_KEEP_MAX);
+  TM_REQUIRE(max_keep >= 1, "tm_peak_filter: max_keep must be positive");
   hipStream_t s = (hipStream_t)stream;
+  if (n <= PK_PAR_MAX) {
+    TM_REQUIRE(work && work_bytes >= tm_peak_filter_workspace_bytes(n), "tm_peak_filter: workspace too small");
+    int* ncf = (int*)work;
+    int* cf = ncf + n;
+    hipLaunchKernelGGL(k_pf_conflicts, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, s, coords, n,
+                       min_distance, ncf, cf);
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pf_resolve, dim3(1), dim3(1024), 0, s, coords, n, min_distance, ncf, cf, max_keep, keep,
+                       nkeep);
+    TM_CHECK_LAUNCH();
+    return TM_OK;
+  }
+  TM_REQUIRE(max_keep <= PK_KEEP_MAX, "tm_peak_filter: beyond %d rows at most %d can be kept", PK_PAR_MAX,
+             PK_KEEP_MAX);
   hipLaunchKernelGGL(k_peak_filter, dim3(1), dim3(PK_THREADS), 0, s, coords, n, min_distance, max_keep, keep, nkeep);
   TM_CHECK_LAUNCH();
   return TM_OK;

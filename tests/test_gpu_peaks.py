@@ -80,19 +80,28 @@ def test_topk_unravels_like_numpy(be):
         np.testing.assert_array_equal(g.cpu().numpy(), w)
 
 
-@pytest.mark.parametrize("n", [1, 2, 50, 5000, 12288])
+@pytest.mark.parametrize("n", [1, 2, 50, 5000, 12288, 16384, 20000])
 @pytest.mark.parametrize("md", [1, 5, 12])
-def test_greedy_distance_filter(be, n, md):
+@pytest.mark.parametrize("layout", ["spread", "clustered"])
+def test_greedy_distance_filter(be, n, md, layout):
+    """Parallel rounds (n <= 16384) and the sequential pass (larger n); the
+    clustered layout gives candidates with more than 8 earlier neighbours."""
     import torch
 
-    rng = np.random.default_rng(n * 31 + md)
-    c = rng.integers(-1, 120, size=(n, 3)).astype(np.int64)
+    rng = np.random.default_rng(n * 31 + md + (layout == "clustered"))
+    if layout == "spread":
+        c = rng.integers(-1, 120, size=(n, 3)).astype(np.int64)
+    else:
+        centres = rng.integers(0, 200, size=(max(1, n // 200), 3))
+        c = (centres[rng.integers(0, centres.shape[0], n)] + rng.integers(-4, 5, size=(n, 3))).astype(np.int64)
     want = op.find_candidate_indices(c, md)
     cd = torch.as_tensor(c).cuda()
-    for max_keep in (1, 100, 4096):
-        print(f"distance filter n {n} md {md} max_keep {max_keep}", flush=True)
+    for max_keep in (1, 100, 4096, None):
+        if max_keep is None and n > 16384:
+            continue
+        print(f"distance filter n {n} md {md} {layout} max_keep {max_keep}", flush=True)
         got = be.greedy_distance_filter(cd, md, max_keep).cpu().numpy()
-        np.testing.assert_array_equal(got, want[:max_keep])
+        np.testing.assert_array_equal(got, want if max_keep is None else want[:max_keep])
 
 
 def _compare_state(got, want):
