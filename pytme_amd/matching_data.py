@@ -306,6 +306,35 @@ class MatchingData:
     def target_filter(self, value):
         self._target_filter = value
 
+    def computation_schedule(self, matching_method: str = "FLCSphericalMask", max_cores: int = 1,
+                             use_gpu: bool = False, pad_fourier: bool = False, pad_target_edges: bool = False,
+                             analyzer_method: str = None, available_memory: int = None, max_splits: int = 256):
+        """Target splits and (outer, inner) jobs that fit ``available_memory``
+        (tme/matching_data.py:559-634); on the hip backend the budget is the
+        device's free HBM and the estimate includes the fused scan's batch
+        workspace.  GPU schedules split only into outer jobs."""
+        from .backends import backend as be
+        from .matching_utils import compute_parallelization_schedule
+
+        if available_memory is None:
+            available_memory = be.get_available_memory() * be.device_count()
+        shape1 = self._output_target_shape
+        shape2 = tuple(0 for _ in self._output_template_shape)
+        if pad_fourier:
+            shape2 = self._output_template_shape
+        padding = tuple(0 for _ in shape1)
+        if pad_target_edges:
+            padding = tuple(int(x) for x in self._output_template_shape)
+        dt = getattr(be, "datatype_bytes", None)
+        nbytes = (4, 8, 4) if dt is None else (dt(be._float_dtype), dt(be._complex_dtype), dt(be._int_dtype))
+        return compute_parallelization_schedule(
+            shape1=shape1, shape2=shape2, shape1_padding=padding, max_cores=max_cores, max_ram=available_memory,
+            matching_method=matching_method, analyzer_method=analyzer_method,
+            backend=getattr(be, "_backend_name", None), float_nbytes=nbytes[0], complex_nbytes=nbytes[1],
+            integer_nbytes=nbytes[2], split_only_outer=use_gpu,
+            split_axes=self._target_dim if len(self._target_dim) else None, max_splits=max_splits,
+        )
+
     def _split_rotations_on_jobs(self, n_jobs: int) -> List[np.ndarray]:
         """Contiguous rotation chunks; the last chunk takes the remainder."""
         per = self.rotations.shape[0] // n_jobs

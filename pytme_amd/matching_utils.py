@@ -1,6 +1,11 @@
 """Utilities of the exhaustive scan (subset of ``tme/matching_utils.py``)."""
 
+import os
+import pickle
+from concurrent.futures import ThreadPoolExecutor
 from itertools import product
+from shutil import move
+from tempfile import mkstemp
 from typing import Callable, Dict, Optional, Tuple
 
 import numpy as np
@@ -16,6 +21,10 @@ __all__ = [
     "apply_convolution_mode",
     "split_shape",
     "elliptical_mask",
+    "write_pickle",
+    "load_pickle",
+    "compute_extraction_box",
+    "compute_parallelization_schedule",
 ]
 
 
@@ -164,3 +173,148 @@ def elliptical_mask(
     grid = grid - center.reshape((-1,) + (1,) * shape.size)
     dist = np.linalg.norm(grid / radius.reshape((-1,) + (1,) * shape.size), axis=0)
     return (dist <= 1).astype(int)
+
+
+def write_pickle(data: object, filename: str) -> None:
+    """Write ``data`` (a tuple or list is written item by item) as a stream
+    of pickles (tme/matching_utils.py:178-209): the result bundle of the
+    scan, ``(scores, offset, rotations, rotation_mapping, metadata)``.  A
+    ``numpy.memmap`` item is moved next to the output and written as the
+    tuple ("np.memmap", shape, dtype, path), which invalidates the input."""
+    if type(data) not in (list, tuple):
+        data = (data,)
+    dirname = os.path.dirname(filename)
+    futures = []
+    with open(filename, "wb") as ofile, ThreadPoolExecutor() as executor:
+        for item in data:
+            if isinstance(item, np.memmap):
+                _, new_name = mkstemp(suffix=".mm", dir=dirname or None)
+                futures.append(executor.submit(move, item.filename, new_name))
+                item = ("np.memmap", item.shape, item.dtype, new_name)
+            pickle.dump(item, ofile)
+        for future in futures:
+            future.result()
+
+
+def load_pickle(filename: str) -> object:
+    """Read a file written by :func:`write_pickle` (tme/matching_utils.py:
+    212-252): the single item, or the list of items; memmap entries are
+    reopened.  Unpickling runs code from the file: load only files this
+    library (or the reference) wrote."""
+    items = []
+    with open(filename, "rb") as ifile:
+        while True:
+            try:
+                item = pickle.load(ifile)
+            except EOFError:
+                break
+            if isinstance(item, tuple) and len(item) == 4 and isinstance(item[0], str) and item[0] == "np.memmap":
+                _, shape, dtype, path = item
+                item = np.memmap(path, shape=shape, dtype=dtype)
+            items.append(item)
+    return items[0] if len(items) == 1 else items
+
+
+def compute_extraction_box(centers, extraction_shape: Tuple[int], original_shape: Tuple[int]):
+    """Boxes of ``extraction_shape`` around integer ``centers`` (n, d) inside
+    an array of ``original_shape`` (tme/matching_utils.py:1231-1281): the
+    clamped observation box in the array, the matching box in the output and
+    whether the box fits without clamping.  Host (numpy) arrays."""
+    centers = np.asarray(centers)
+    target_shape = np.asarray(original_shape)
+    extraction_shape = np.asarray(extraction_shape)
+    left_pad = (extraction_shape / 2).astype(int)
+    right_pad = (left_pad + np.mod(extraction_shape, 2)).astype(int)
+    obs_beg = centers - left_pad
+    obs_end = centers + right_pad
+    obs_beg_clamp = np.maximum(obs_beg, 0)
+    obs_end_clamp = np.minimum(obs_end, target_shape)
+    clamp_change = np.sum((obs_beg != obs_beg_clamp) + (obs_end != obs_end_clamp), axis=1)
+    cand_beg = left_pad - (centers - obs_beg_clamp)
+    cand_end = left_pad + (obs_end_clamp - centers)
+    stops = cand_end - extraction_shape
+    keep = np.sum((cand_beg == 0) * (stops == 0), axis=1) == centers.shape[1]
+    keep = keep * (clamp_change == 0)
+    return obs_beg_clamp, obs_end_clamp, cand_beg, cand_end, keep
+
+
+def compute_parallelization_schedule(
+    shape1,
+    shape2,
+    max_cores: int,
+    max_ram: int,
+    matching_method: str,
+    split_axes: Tuple[int] = None,
+    backend: str = None,
+    split_only_outer: bool = False,
+    shape1_padding=None,
+    analyzer_method: str = None,
+    max_splits: int = 256,
+    float_nbytes: int = 4,
+    complex_nbytes: int = 8,
+    integer_nbytes: int = 4,
+):
+    """Target splits and (outer, inner) job counts under a memory budget
+    (tme/matching_utils.py:255-409).
+
+    Split counts grow one axis at a time (the largest axis first, or cycling
+    through ``split_axes``) up to ``max_splits`` parts; for each split
+    pattern and each factorisation of ``max_cores`` into outer x inner jobs
+    (outer only with ``split_only_outer``, as for GPU backends) the peak
+    memory of ``outer`` concurrent splits is estimated with
+    :func:`pytme_amd.memory.estimate_memory_usage` (each split padded by
+    ``shape1_padding``).  Of the patterns under ``max_ram`` the one with the
+    fewest splits, then the fewest sequential rounds, wins.  Returns
+    ``(splits, (outer, inner))`` or ``(None, None)``.
+    """
+    from .memory import estimate_memory_usage
+
+    shape1 = tuple(int(x) for x in shape1)
+    shape2 = tuple(int(x) for x in shape2)
+    padding = np.zeros(len(shape1), dtype=int) if shape1_padding is None else np.asarray(shape1_padding, int)
+    assignments = []
+    for i in range(1, int(max_cores ** 0.5) + 1):
+        if max_cores % i == 0:
+            assignments += [(i, max_cores // i), (max_cores // i, i)]
+    if split_only_outer:
+        assignments = [(1, max_cores)]
+
+    if split_axes is None:
+        split_axes = tuple(range(len(shape1)))
+        axis_pos = int(np.argmax(shape1))
+    else:
+        split_axes = tuple(int(x) for x in split_axes)
+        axis_pos = 0
+    axis = split_axes[axis_pos] if axis_pos < len(split_axes) else int(np.argmax(shape1))
+
+    factor = [1] * len(shape1)
+    candidates, n_splits = [], 0
+    while n_splits <= max_splits:
+        splits = {k: factor[k] for k in range(len(shape1))}
+        widths = [tuple(x.stop - x.start for x in part) for part in split_shape(shape1, splits)]
+        n_splits = int(np.prod(list(splits.values())))
+        for inner, outer in assignments:
+            if outer > n_splits:
+                continue
+            usage = [
+                estimate_memory_usage(
+                    shape1=tuple(int(a) + int(b) for a, b in zip(w, padding)), shape2=shape2,
+                    matching_method=matching_method, ncores=inner, analyzer_method=analyzer_method,
+                    backend=backend, float_nbytes=float_nbytes, complex_nbytes=complex_nbytes,
+                    integer_nbytes=integer_nbytes,
+                )
+                for w in widths
+            ]
+            peak = max(sum(usage[i:i + outer]) for i in range(0, len(usage), outer))
+            if peak < max_ram:
+                candidates.append((*factor, outer, inner, n_splits, n_splits // outer))
+        factor[axis] += 1
+        axis_pos = (axis_pos + 1) % len(split_axes)
+        axis = split_axes[axis_pos]
+
+    if not candidates:
+        return None, None
+    table = np.asarray(candidates)
+    best = table[np.lexsort((table[:, -1], table[:, -2]))][0]
+    d = len(shape1)
+    return {k: int(best[k]) for k in range(d)}, (int(best[d]), int(best[d + 1]))

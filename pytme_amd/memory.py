@@ -70,12 +70,51 @@ class MaxScoreOverRotationsMemoryUsage(MatchingMemoryUsage):
     base = (2, 0)
 
 
+class PeakCallerMaximumFilterMemoryUsage(MatchingMemoryUsage):
+    """Peak calling on the max-score volume (tme/memory.py:197-212): the
+    volume and the hip backend's filtered copy; the peak bitmask (F/8
+    bytes) is folded into the second array."""
+
+    base = (2, 0)
+
+
+class HipBackendMemoryUsage(MatchingMemoryUsage):
+    """Device memory of the hip backend beyond the score's arrays: the fused
+    scan's batch workspace (``tm_scan_workspace_bytes``) for the FLC plan
+    with 16 rotations per batch, rows padded to 16 bins, and a template of
+    edge min(N0, 64) for the pruned intermediates (the memory classes see
+    only the transform shapes).  Plays the role of the reference's
+    ``CupyBackendMemoryUsage`` (tme/memory.py:216-232) for an HBM budget."""
+
+    batch = 16
+    template_edge = 64
+
+    def __init__(self, fast_shape, ft_shape, float_nbytes=4, complex_nbytes=8, integer_nbytes=4):
+        super().__init__(fast_shape, ft_shape, float_nbytes, complex_nbytes, integer_nbytes)
+        self.fast_shape = tuple(int(x) for x in fast_shape)
+
+    def base_usage(self) -> int:
+        n0, n1, n2 = (self.fast_shape + (1, 1, 1))[:3]
+        n2s = -(-(n2 // 2 + 1) // 16) * 16
+        t0 = min(n0, self.template_edge)
+        B, cb = self.batch, self.complex_nbytes
+        spectra = 3 * B * n0 * n1 * n2s * cb
+        pruned = 2 * B * t0 * (t0 + n1) * n2s * cb
+        padded_target = 2 * n0 * n1 * n2s * cb
+        return int(spectra + pruned + padded_target)
+
+    def per_fork(self) -> int:
+        return 0
+
+
 MATCHING_MEMORY_REGISTRY: Dict[str, type] = {
     "CC": CCMemoryUsage,
     "LCC": LCCMemoryUsage,
     "FLC": FLCMemoryUsage,
     "FLCSphericalMask": FLCSphericalMaskMemoryUsage,
     "MaxScoreOverRotations": MaxScoreOverRotationsMemoryUsage,
+    "PeakCallerMaximumFilter": PeakCallerMaximumFilterMemoryUsage,
+    "hip": HipBackendMemoryUsage,
 }
 
 
@@ -85,12 +124,15 @@ def estimate_memory_usage(
     matching_method: str,
     ncores: int = 1,
     analyzer_method: str = None,
+    backend: str = None,
     float_nbytes: int = 4,
     complex_nbytes: int = 8,
     integer_nbytes: int = 4,
     **kwargs,
 ) -> int:
-    """Bytes for matching ``shape2`` against ``shape1`` with ``ncores`` loops."""
+    """Bytes for matching ``shape2`` against ``shape1`` with ``ncores``
+    loops: the score's arrays, the analyzer's and the backend's
+    (tme/memory.py:252-338)."""
     if matching_method not in MATCHING_MEMORY_REGISTRY:
         raise ValueError(f"Supported options are {','.join(MATCHING_MEMORY_REGISTRY)}")
     from .backends.hip_backend import _next_fast_len
@@ -101,6 +143,8 @@ def estimate_memory_usage(
     args = (fast, ft, float_nbytes, complex_nbytes, integer_nbytes)
     usage = MATCHING_MEMORY_REGISTRY[matching_method](*args)
     total = usage.base_usage() + ncores * usage.per_fork()
-    if analyzer_method in MATCHING_MEMORY_REGISTRY:
-        total += MATCHING_MEMORY_REGISTRY[analyzer_method](*args).base_usage()
+    for extra in (analyzer_method, backend):
+        if extra in MATCHING_MEMORY_REGISTRY:
+            usage = MATCHING_MEMORY_REGISTRY[extra](*args)
+            total += usage.base_usage() + ncores * usage.per_fork()
     return int(total)

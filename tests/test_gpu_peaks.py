@@ -204,5 +204,29 @@ def test_peaks_of_the_scan_result(be):
     want = ref.result(ref(ref.init_state(), scores))
     _compare_state(res, want)
     assert res[0].shape[0] == 1000
-    # the rotation of each peak is looked up from the rotation array
-    assert np.all(rot_array[tuple(res[0].T)] >= 0)
+    # postprocess.py:543-566: rotation of each peak from the rotation array
+    # and mapping, as zyz Euler angles, into Orientations; STAR and the
+    # result bundle round-trip
+    import os
+    import tempfile
+
+    from pytme_amd import Orientations
+    from pytme_amd.matching_utils import load_pickle, write_pickle
+    from pytme_amd.rotations import euler_from_rotationmatrix
+
+    idx = rot_array[tuple(res[0].T)]
+    assert np.all(idx >= 0)
+    angles = np.vstack([euler_from_rotationmatrix(np.asarray(mapping[int(i)])) for i in idx])
+    orientations = Orientations(translations=res[0], rotations=angles, scores=res[2], details=res[3])
+    with tempfile.TemporaryDirectory() as d:
+        star = os.path.join(d, "peaks.star")
+        orientations.to_file(star)
+        back = Orientations.from_file(star)
+        np.testing.assert_array_equal(back.translations, res[0].astype(np.float32))
+        np.testing.assert_allclose(back.rotations, angles, atol=1e-4)
+        bundle = os.path.join(d, "result.pickle")
+        write_pickle((scores, np.zeros(3, int), rot_array, mapping), bundle)
+        s2, _, r2, m2 = load_pickle(bundle)
+        np.testing.assert_array_equal(s2, scores)
+        np.testing.assert_array_equal(r2, rot_array)
+        assert set(m2) == set(mapping)
