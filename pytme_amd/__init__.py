@@ -10,4 +10,5 @@ gfx950 HIP kernels from ``libpytme_hip.so``.
 __version__ = "0.1.0"
 
 from .backends import backend  # noqa: F401
+from .density import Density  # noqa: F401
 from .orientations import Orientations  # noqa: F401

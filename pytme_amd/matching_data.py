@@ -12,6 +12,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
+from .density import Density
 from .backends import backend as be
 
 __all__ = ["MatchingData"]
@@ -107,7 +108,14 @@ class MatchingData:
         take_left = np.minimum(start, left_pad)
         take_right = np.minimum(np.subtract(arr.shape, stop), right_pad).astype(int)
         box = tuple(slice(int(a), int(b)) for a, b in zip(start - take_left, stop + take_right))
-        sub = np.asarray(be.to_numpy_array(arr[box]) if not isinstance(arr, np.ndarray) else arr[box])
+        if isinstance(arr, Density):
+            # a memory-mapped map is re-read for just this box (row reads)
+            if isinstance(arr.data, np.memmap):
+                sub = Density.from_file(arr.data.filename, subset=box).data
+            else:
+                sub = np.asarray(arr.data[box])
+        else:
+            sub = np.asarray(be.to_numpy_array(arr[box]) if not isinstance(arr, np.ndarray) else arr[box])
         pad = tuple((int(l), int(r)) for l, r in zip(left_pad - take_left, right_pad - take_right))
         if any(l or r for l, r in pad):
             sub = np.pad(sub, pad, mode="reflect")
@@ -250,7 +258,7 @@ class MatchingData:
 
     @property
     def target(self):
-        return self._target
+        return self._target.data if isinstance(self._target, Density) else self._target
 
     @target.setter
     def target(self, arr):
@@ -271,7 +279,7 @@ class MatchingData:
 
     @template.setter
     def template(self, arr):
-        self._template = arr
+        self._template = arr.data if isinstance(arr, Density) else arr
         if getattr(self, "_template_mask", None) is None:
             self._template_mask = np.ones(arr.shape, dtype=np.float32)
 
@@ -286,6 +294,8 @@ class MatchingData:
 
     @staticmethod
     def _set_mask(mask, shape):
+        if isinstance(mask, Density):
+            mask = mask.data
         if mask is not None and np.broadcast_shapes(tuple(mask.shape), tuple(shape)) != tuple(shape):
             raise ValueError("Mask and data shape need to be broadcastable.")
         return mask
