@@ -214,6 +214,16 @@ class PeakCaller(AbstractAnalyzer):
         rotations = t.cat((rotations.to(be._float_dtype), be.to_backend_array(peak_rotations).to(be._float_dtype)))
         scores = t.cat((scores.to(be._float_dtype), be.to_backend_array(peak_scores).to(be._float_dtype)))
         details = t.cat((details.to(be._float_dtype), be.to_backend_array(peak_details).to(be._float_dtype)))
+        if self.min_distance > 0 and translations.shape[0] > 1:
+            # placeholder rows (position -1, score -1) are identical: each one
+            # after the first lies at distance 0 from it and shares its fate
+            # or is dropped, so one of them decides the filter's result
+            n0 = translations.shape[0]
+            filler = (positions[:n0] == -1).all(dim=1) & (scores[:n0] == -1)
+            drop = filler & (t.cumsum(filler.to(t.int64), 0) > 1)
+            if bool(drop.any()):
+                keep = t.cat((~drop, t.ones(positions.shape[0] - n0, dtype=t.bool, device=positions.device)))
+                positions, rotations, scores, details = positions[keep], rotations[keep], scores[keep], details[keep]
         distance_order = filter_points_indices(positions, self.min_distance, batch_dims=None, scores=scores,
                                                max_keep=self.num_peaks)
         top = self._top_peaks(positions[distance_order], scores[distance_order], self.num_peaks)

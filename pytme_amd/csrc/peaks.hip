@@ -365,21 +365,39 @@ __device__ __forceinline__ bool pk_near(const int64_t* __restrict__ a, const int
 }
 
 // earlier candidates within the distance of each candidate (first PK_CF_MAX
-// listed, all counted)
+// listed, all counted); earlier rows stream through LDS in tiles of 256
 __global__ __launch_bounds__(256) void k_pf_conflicts(const int64_t* __restrict__ xyz, int64_t n, int64_t md,
                                                       int* __restrict__ ncf, int* __restrict__ cf) {
+  __shared__ int tz[256], ty[256], tx[256];
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  const bool valid = i < n;
   const int64_t r2 = (md + 1) * (md + 1);
-  const int64_t c[3] = {xyz[3 * i + 0], xyz[3 * i + 1], xyz[3 * i + 2]};
+  const int cz = valid ? (int)xyz[3 * i + 0] : 0, cy = valid ? (int)xyz[3 * i + 1] : 0,
+            cx = valid ? (int)xyz[3 * i + 2] : 0;
   int k = 0;
-  for (int64_t j = 0; j < i; ++j) {
-    if (pk_near(c, xyz + 3 * j, md, r2)) {
-      if (k < PK_CF_MAX) cf[i * PK_CF_MAX + k] = (int)j;
-      ++k;
+  const int64_t last = (int64_t)blockIdx.x * 256;  // the block's first row: earlier tiles are whole
+  for (int64_t j0 = 0; j0 <= last; j0 += 256) {
+    const int64_t j = j0 + threadIdx.x;
+    if (j < n) {
+      tz[threadIdx.x] = (int)xyz[3 * j + 0];
+      ty[threadIdx.x] = (int)xyz[3 * j + 1];
+      tx[threadIdx.x] = (int)xyz[3 * j + 2];
     }
+    __syncthreads();
+    if (valid) {
+      const int64_t jn = i - j0 < 256 ? i - j0 : 256;  // rows j0 .. i-1 of this tile
+      for (int64_t q = 0; q < jn; ++q) {
+        const int64_t dz = llabs((int64_t)cz - tz[q]), dy = llabs((int64_t)cy - ty[q]),
+                      dx = llabs((int64_t)cx - tx[q]);
+        if (dz <= md && dy <= md && dx <= md && dz * dz + dy * dy + dx * dx < r2) {
+          if (k < PK_CF_MAX) cf[i * PK_CF_MAX + k] = (int)(j0 + q);
+          ++k;
+        }
+      }
+    }
+    __syncthreads();
   }
-  ncf[i] = k;
+  if (valid) ncf[i] = k;
 }
 
 // rounds of decisions in LDS (0 open, 1 kept, 2 dropped), then the kept rows
