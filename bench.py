@@ -26,13 +26,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the default workload (scaled by
 # a 1 GiB copy); its traffic_bytes_per_launch line feeds roofline.traffic
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48_b16.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48_b64.md")
 
 
 def measured_traffic(kernel, workload, batch, path):
     """HBM bytes per launch of `kernel` from the committed counter summary,
     or None when the run is not the profiled configuration."""
-    if workload != "flc_512x512x200_t48" or batch != 16 or path != "fused":
+    if workload != "flc_512x512x200_t48" or batch != 64 or path != "fused":
         return None
     try:
         with open(TRAFFIC_PROFILE) as fh:
@@ -133,8 +133,18 @@ def build_scan(workload, target, template, mask, filters, rotations, path, batch
     )
     proxy = SharedAnalyzerProxy(MaxScoreOverRotations, cb_args)
     rot_dev = chunks[0]
+    if batch is None:
+        # the backend's own choice: largest of 64, 32, 16 ... whose workspace
+        # fits a quarter of the free HBM
+        from pytme_amd.backends.hip_fused import choose_batch
+
+        fast = tuple(int(x) for x in setup["fast_shape"])
+        mode = 0 if workload["score"] == "FLC" else 1
+        batch = choose_batch(fast, template.shape, mode, filters[0] is not None, be.get_available_memory(),
+                             None, int(rot_dev.shape[0]))
+        be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch)
     torch.cuda.synchronize()
-    return scoring, setup, proxy, rot_dev
+    return scoring, setup, proxy, rot_dev, batch
 
 
 def time_steps(scoring, setup, proxy, rot_dev, batch, steps, warmup, timing=False):
@@ -296,7 +306,7 @@ def main():
 
     spec = WORKLOADS[args.workload]
     target, template, mask, rotations, filters, spec = make_workload(args.workload, copies=args.copies)
-    batch = args.batch or (16 if args.path == "fused" else 4)
+    batch = args.batch or (None if args.path == "fused" else 4)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -319,7 +329,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    scoring, setup, proxy, rot_dev = build_scan(spec, target, template, mask, filters, rotations, args.path, batch)
+    scoring, setup, proxy, rot_dev, batch = build_scan(spec, target, template, mask, filters, rotations, args.path,
+                                                      batch)
     elapsed, report = time_steps(scoring, setup, proxy, rot_dev, batch, args.steps, args.warmup, timing=True)
     n_rot = args.steps * batch
     value = n_rot / elapsed
