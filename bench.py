@@ -133,18 +133,20 @@ def build_scan(workload, target, template, mask, filters, rotations, path, batch
     )
     proxy = SharedAnalyzerProxy(MaxScoreOverRotations, cb_args)
     rot_dev = chunks[0]
+    tuned = None
     if batch is None:
-        # the backend's own choice: largest of 64, 32, 16 ... whose workspace
-        # fits a quarter of the free HBM
-        from pytme_amd.backends.hip_fused import choose_batch
+        # the backend's autotuner: times the memory-fitting batch sizes on
+        # scratch state during a first call (outside the timed region)
+        from pytme_amd.backends import hip_fused
 
-        fast = tuple(int(x) for x in setup["fast_shape"])
-        mode = 0 if workload["score"] == "FLC" else 1
-        batch = choose_batch(fast, template.shape, mode, filters[0] is not None, be.get_available_memory(),
-                             None, int(rot_dev.shape[0]))
+        be.change_backend("hip", device="cuda", fused=True, rotation_batch="auto")
+        scoring(rotations=rot_dev[: min(256, rot_dev.shape[0])],
+                callback=SharedAnalyzerProxy(MaxScoreOverRotations, cb_args), **setup)
+        tuned = next(iter(hip_fused.TUNED.values()))
+        batch = tuned["batch"]
         be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch)
     torch.cuda.synchronize()
-    return scoring, setup, proxy, rot_dev, batch
+    return scoring, setup, proxy, rot_dev, batch, tuned
 
 
 def time_steps(scoring, setup, proxy, rot_dev, batch, steps, warmup, timing=False):
@@ -329,8 +331,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    scoring, setup, proxy, rot_dev, batch = build_scan(spec, target, template, mask, filters, rotations, args.path,
-                                                      batch)
+    scoring, setup, proxy, rot_dev, batch, tuned = build_scan(spec, target, template, mask, filters, rotations,
+                                                              args.path, batch)
     elapsed, report = time_steps(scoring, setup, proxy, rot_dev, batch, args.steps, args.warmup, timing=True)
     n_rot = args.steps * batch
     value = n_rot / elapsed
@@ -397,6 +399,7 @@ def main():
             "template": spec["T"],
             "fast_shape": list(fast),
             "rotations_per_step": batch,
+            "autotune_ms_per_rotation": None if tuned is None else tuned["ms_per_rotation"],
             "path": args.path,
             "filters": "wedge x CTF template filter, continuous-wedge target filter" if spec["filters"] else None,
         },

@@ -89,8 +89,11 @@ class HipBackend(MatchingBackend):
         Let the registered scores use the batched fused FLC/CC scan when the
         analyzer is :class:`MaxScoreOverRotations` (default True).  False
         forces the plain per-rotation loop.
-    rotation_batch : int, optional
-        Rotations per fused launch; chosen automatically by default.
+    rotation_batch : int or "auto", optional
+        Rotations per fused launch.  By default the largest of 64, 32, 16 ...
+        whose workspace fits a quarter of the free HBM; ``"auto"`` also
+        times that size and its halves on the first call of a configuration
+        and keeps the fastest (``hip_fused.autotune_batch``).
     scan_graph : bool, optional
         Replay each fused batch's kernel sequence as a hipGraph (default True).
     """

@@ -23,9 +23,13 @@ import numpy as np
 
 from .. import _lib
 
-__all__ = ["run", "choose_batch"]
+__all__ = ["run", "choose_batch", "autotune_batch", "TUNED"]
 
 _PLAN = {"key": None, "handle": None, "keep": None}
+
+#: measured batch choices: (device, mode, fast_shape, template_shape, filter,
+#: normalize) -> {"batch": B, "ms_per_rotation": {B: ms}}
+TUNED = {}
 
 
 def _free_plan():
@@ -55,6 +59,40 @@ def choose_batch(fast_shape, template_shape, mode: int, with_filter: bool, free_
     while b > 1 and lib.tm_scan_workspace_bytes(mode, fs, ts, b, int(with_filter)) > free_bytes // 4:
         b //= 2
     return int(max(1, min(b, n_rotations)))
+
+
+def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, candidates):
+    """Time each candidate batch size on scratch max/argmax state (the
+    caller's state is not touched) with the call's own rotations: one
+    warm-up batch and two timed batches per candidate, HIP events on the
+    launch stream; keep the fastest per rotation.  ``create(B)`` returns a
+    plan handle for batch B.  Results are cached in :data:`TUNED`."""
+    t = backend._torch
+    if key in TUNED:
+        return TUNED[key]["batch"]
+    scores = t.empty(state_shape, dtype=t.float32, device=rots.device)
+    rot = t.empty(state_shape, dtype=t.int32, device=rots.device)
+    n = int(rots.shape[0])
+    timings = {}
+    for B in candidates:
+        if B > n:
+            continue
+        handle = create(B)
+        try:
+            start, stop = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            for rep in range(3):
+                if rep == 1:
+                    start.record()
+                _lib.call("tm_scan_batch", handle, rots[0].data_ptr(), idx_dev[0].data_ptr(), B, scores.data_ptr(),
+                          rot.data_ptr(), stream)
+            stop.record()
+            stop.synchronize()
+            timings[B] = start.elapsed_time(stop) / (2 * B)
+        finally:
+            _lib.call("tm_scan_plan_destroy", handle)
+    best = min(timings, key=timings.get) if timings else min(candidates)
+    TUNED[key] = {"batch": int(best), "ms_per_rotation": {int(b): round(v, 5) for b, v in timings.items()}}
+    return int(best)
 
 
 def _proxy_state(callback):
@@ -133,31 +171,42 @@ def run(backend, score: str, **kw):
     n = int(rots.shape[0])
 
     free = backend.get_available_memory()
-    B = choose_batch(fast_shape, tshape, mode, filt is not None, free, backend.rotation_batch, n)
+    auto = backend.rotation_batch == "auto"
+    B = choose_batch(fast_shape, tshape, mode, filt is not None, free, None if auto else backend.rotation_batch, n)
 
     keep = (templ, mask, filt, ft_target, ft2, inv_den)
+    ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
+    stream = backend._stream()
+
+    def create(batch):
+        handle = ctypes.c_void_p()
+        _lib.call(
+            "tm_scan_plan_create", mode, _lib.i64_array(fast_shape), _lib.i32_array(tshape), ptr(templ), ptr(mask),
+            ptr(filt), ptr(ft_target), ptr(ft2), ptr(inv_den), normalize, mask_sum, batch, ctypes.byref(handle),
+        )
+        return handle.value
+
+    idx = analyzer.rotation_indices(state, rots)
+    idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
+    if auto:
+        tune_key = (t.cuda.current_device(), mode, fast_shape, tshape, filt is not None, normalize)
+        if tune_key not in TUNED:
+            _free_plan()  # the candidates' workspaces need the memory
+        candidates = [b for b in (B, B // 2, B // 4) if b >= 1]
+        B = autotune_batch(backend, tune_key, create, stream, rots, idx_dev, fast_shape, candidates)
     key = (
         t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,
         tuple(0 if x is None else x.data_ptr() for x in keep), bool(getattr(backend, "scan_graph", True)),
     )
-    ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
-    stream = backend._stream()
     if _PLAN["key"] != key:
         _free_plan()
-        handle = ctypes.c_void_p()
-        _lib.call(
-            "tm_scan_plan_create", mode, _lib.i64_array(fast_shape), _lib.i32_array(tshape), ptr(templ), ptr(mask),
-            ptr(filt), ptr(ft_target), ptr(ft2), ptr(inv_den), normalize, mask_sum, B, ctypes.byref(handle),
-        )
-        _PLAN.update(key=key, handle=handle.value, keep=keep)
+        _PLAN.update(key=key, handle=create(B), keep=keep)
         _lib.call("tm_scan_plan_use_graph", _PLAN["handle"], int(getattr(backend, "scan_graph", True)))
     else:
         # the plan holds a row-padded copy of the target spectra: refresh it,
         # the caller's arrays may hold new values at the same addresses
         _lib.call("tm_scan_plan_set_target", _PLAN["handle"], ptr(ft_target), ptr(ft2), stream)
 
-    idx = analyzer.rotation_indices(state, rots)
-    idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
     for start in range(0, n, B):
         nb = min(B, n - start)
         _lib.call(

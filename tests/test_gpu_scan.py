@@ -311,6 +311,30 @@ def test_fused_partial_batches_and_plan_reuse():
     assert len(mapping) == 20
 
 
+def test_autotuned_batch_matches_fixed_batches():
+    """rotation_batch="auto" times the fitting batch sizes on scratch state
+    and keeps the fastest; the scan result is bit-identical to fixed batch
+    sizes 5 and 24 (per-rotation arithmetic does not depend on the batch) and
+    to the FP64 oracle within the parity bound."""
+    from pytme_amd.backends import hip_fused
+
+    target, template, mask, rots = _workload(seed=71, n_rot=48)
+    print("autotuned vs fixed batches, 48 rotations", flush=True)
+    hip_fused.TUNED.clear()
+    s_auto, _, r_auto, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch="auto")
+    tuned = next(iter(hip_fused.TUNED.values()))
+    print(f"  tuned {tuned}", flush=True)
+    assert tuned["batch"] in tuned["ms_per_rotation"] and len(tuned["ms_per_rotation"]) >= 2
+    for b in (5, 24):
+        s_b, _, r_b, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=b)
+        np.testing.assert_array_equal(s_b, s_auto)
+        np.testing.assert_array_equal(r_b, r_auto)
+    _, _, raws, oc = orc.scan("FLC", target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True)
+    n_diff, err = assert_scan_parity(s_auto, r_auto, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
 def test_graph_replays_match_eager_batches():
     """The fused batch replayed as a hipGraph: every replay (batch 3 onwards;
     batch 1 runs eagerly, batch 2 is captured) equals the eager batch, with
