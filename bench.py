@@ -259,7 +259,8 @@ def bench_peaks(scores, num_peaks, min_distance, reps, with_cpu):
     kernels = {k: {"us_per_launch": round(v[0] * 1e3 / max(v[1], 1), 1), "launches": v[1],
                    "GB/s": round(v[2] / max(v[0], 1e-9) / 1e6, 1)}
                for k, v in sorted(report.items(), key=lambda kv: -kv[1][0])}
-    name = "peaks_max_filter_z"
+    # one fused pass for edges up to 8 ("peaks_max_filter"), else the z pass
+    name = "peaks_max_filter" if "peaks_max_filter" in report else "peaks_max_filter_z"
     roofline = None
     if name in report:
         ms, count, by = report[name]

@@ -25,6 +25,7 @@
 #define PK_THREADS 256                 // threads per compaction block (4 waves)
 #define PK_MF_TX 64                    // max-filter plane tile: 64 x 16
 #define PK_MF_TY 16
+#define PK_MF_TZ 8                     // z planes per block of the xy pass
 #define PK_MAX_SIZE 64                 // largest filter edge
 #define PK_SORT_LDS 8192               // bitonic sort in LDS up to this many keys
 #define PK_SORT_MAX 65536              // larger sorts run on a global buffer
@@ -89,7 +90,7 @@ __device__ __forceinline__ int pk_block_rank(bool flag, int* wsum, int* total) {
 
 // x and y maxima over a 64 x 16 plane tile with a (d-1) halo; outside the
 // volume the input is the constant 0
-__global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, float* __restrict__ out, int n1,
+__global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, float* __restrict__ out, int n0, int n1,
                                                int n2, int d) {
   extern __shared__ float sm[];
   const int lo = d / 2;
@@ -97,7 +98,9 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
   float* in = sm;             // hy x hx
   float* mx = sm + hy * hx;   // hy x 64
   const int x0 = blockIdx.x * PK_MF_TX, y0 = blockIdx.y * PK_MF_TY;
-  const int64_t z = blockIdx.z;
+  const int zb = blockIdx.z * PK_MF_TZ, ze = min(zb + PK_MF_TZ, n0);
+  for (int64_t z = zb; z < ze; ++z) {
+  __syncthreads();  // the previous plane's tiles are consumed
   const float* plane = s + z * (int64_t)n1 * n2;
   for (int i = threadIdx.x; i < hy * hx; i += blockDim.x) {
     const int r = i / hx, c = i - r * hx;
@@ -122,6 +125,7 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
     float m = col[0];
     for (int k = 1; k < d; ++k) m = fmaxf(m, col[k * PK_MF_TX]);
     oplane[(int64_t)y * n2 + x] = m;
+  }
   }
 }
 
@@ -163,6 +167,61 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s
   if (lane == 0) wsum[w] = total;
   __syncthreads();
   if (threadIdx.x == 0) cnt[blockIdx.x] = (int64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// z maxima and the peak test for edges up to 8: one thread per (y, x)
+// column walks a chunk of 64 z planes (plus the halo planes) holding the
+// last D xy maxima in registers, so every xy maximum is read about once and
+// consecutive threads read consecutive voxels.  A wave's 64 columns are 64
+// consecutive voxels of a plane; its ballot is OR-ed into the linear-order
+// words (zeroed beforehand), split across a word boundary when needed.
+#define PK_ZC 64
+#define PK_FUSED_MAX 8
+
+template <int D>
+__global__ __launch_bounds__(256) void k_mf_zcol(const float* __restrict__ s, const float* __restrict__ mxy,
+                                                 unsigned long long* __restrict__ mask, int n0, int64_t plane) {
+  constexpr int LO = D / 2, HI = D - 1 - LO;
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool col = q < plane;
+  const int z0 = blockIdx.y * PK_ZC, zend = min(z0 + PK_ZC, n0);
+  float w[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) w[k] = 0.f;
+  // window slots hold planes zi - D + 1 .. zi; planes outside the volume are 0
+  for (int zi = z0 - LO; zi <= zend - 1 + HI; ++zi) {
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k) w[k] = w[k + 1];
+    w[D - 1] = (col && zi >= 0 && zi < n0) ? mxy[(int64_t)zi * plane + q] : 0.f;
+    const int zo = zi - HI;
+    if (zo < z0) continue;  // uniform
+    float m = w[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) m = fmaxf(m, w[k]);
+    const bool peak = col && s[(int64_t)zo * plane + q] == m;
+    const unsigned long long b = __ballot(peak);
+    if (lane == 0 && b) {
+      const int64_t L0 = (int64_t)zo * plane + (q - lane);
+      const int64_t wd = L0 >> 6;
+      const int sh = (int)(L0 & 63);
+      atomicOr(&mask[wd], b << sh);
+      if (sh && (b >> (64 - sh))) atomicOr(&mask[wd + 1], b >> (64 - sh));
+    }
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// peak count of each 4096-voxel block of the bit words (64 words per block)
+__global__ __launch_bounds__(64) void k_mask_count(const unsigned long long* __restrict__ mask,
+                                                   int64_t* __restrict__ cnt) {
+  int c = __popcll(mask[(int64_t)blockIdx.x * 64 + threadIdx.x]);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
 
 // C-ordered coordinates of the set bits
@@ -502,7 +561,9 @@ __global__ __launch_bounds__(PK_THREADS) void k_peak_filter(const int64_t* __res
 extern "C" {
 
 // Peaks of a 3-D float32 volume under a box maximum filter of edge `size`
-// (constant-zero borders), first half: filters, marks and counts.  Buffers:
+// (constant-zero borders), first half: filters, marks and counts (separable:
+// x and y maxima in one pass, then z maxima and the peak test; for edges up
+// to 8 the z pass walks columns with the window in registers).  Buffers:
 // tmp F floats, mask 64*nblk words, cnt nblk and off nblk+1 int64 with
 // nblk = ceil(F/4096).  off[nblk] receives the number of peaks.
 int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, float* tmp,
@@ -519,10 +580,32 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
   hipStream_t s = (hipStream_t)stream;
   const int hx = PK_MF_TX + size - 1, hy = PK_MF_TY + size - 1;
   const size_t shm = sizeof(float) * (size_t)hy * (hx + PK_MF_TX);
-  dim3 g1((unsigned)tm_ceil_div(n2, PK_MF_TX), (unsigned)tm_ceil_div(n1, PK_MF_TY), (unsigned)n0);
+  dim3 g1((unsigned)tm_ceil_div(n2, PK_MF_TX), (unsigned)tm_ceil_div(n1, PK_MF_TY), (unsigned)tm_ceil_div(n0, PK_MF_TZ));
   TM_TIMED("peaks_max_filter_xy", 8.0 * F, s,
-           hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n1, (int)n2, size));
+           hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n0, (int)n1, (int)n2, size));
   TM_CHECK_LAUNCH();
+  if (size <= PK_FUSED_MAX) {
+    const int64_t words = 64 * nblk, plane = n1 * n2;
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>(tm_ceil_div(words, 256), 4096)), dim3(256), 0, s,
+                       mask, words);
+    TM_CHECK_LAUNCH();
+    dim3 gz((unsigned)tm_ceil_div(plane, 256), (unsigned)tm_ceil_div(n0, PK_ZC));
+#define PK_ZCOL(D_)                                                                                       \
+  case D_:                                                                                                \
+    TM_TIMED("peaks_max_filter_z", 8.0 * F + F / 8.0, s,                                                  \
+             hipLaunchKernelGGL(k_mf_zcol<D_>, gz, dim3(256), 0, s, scores, tmp, mask, (int)n0, plane)); \
+    break;
+    switch (size) {
+      PK_ZCOL(1) PK_ZCOL(2) PK_ZCOL(3) PK_ZCOL(4) PK_ZCOL(5) PK_ZCOL(6) PK_ZCOL(7) PK_ZCOL(8)
+    }
+#undef PK_ZCOL
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_mask_count, dim3((unsigned)nblk), dim3(64), 0, s, mask, cnt);
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pk_scan, dim3(1), dim3(1024), 0, s, cnt, off, nblk);
+    TM_CHECK_LAUNCH();
+    return TM_OK;
+  }
   TM_TIMED("peaks_max_filter_z", 8.0 * F + F / 8.0, s,
            hipLaunchKernelGGL(k_mf_z, dim3((unsigned)nblk), dim3(PK_THREADS), 0, s, scores, tmp, mask, cnt, (int)n0,
                               (int)n1, (int)n2, size));
