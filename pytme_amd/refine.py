@@ -289,10 +289,12 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
         sc = torch.cat(scores)
         evals += n
         with torch.no_grad():
+            # selects instead of masked assignment: no host synchronisation,
+            # so the next step's launches queue behind this step's kernels
             better = sc > best_s
             best_s = torch.where(better, sc, best_s)
-            best_t[better] = t.detach()[better]
-            best_a[better] = a.detach()[better]
+            best_t = torch.where(better[:, None], t.detach(), best_t)
+            best_a = torch.where(better[:, None], a.detach(), best_a)
         if step == steps:
             break
         opt.step()
