@@ -24,6 +24,10 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <string.h>
+
+#include <algorithm>
+
 #include "tm_common.h"
 
 #define TM_RF_THREADS 256
@@ -33,6 +37,7 @@ struct tm_flc_local_plan {
   int64_t nvox;
   int max_poses;
   int nblk;  // blocks per pose
+  float rmask;  // largest distance of a nonzero mask voxel from the centre T // 2
   float2* gm;
   double* partial;  // [max_poses][nblk][12]
 };
@@ -111,7 +116,7 @@ template <bool BWD>
 __global__ void __launch_bounds__(TM_RF_THREADS)
 k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const float2* __restrict__ gm, int T0, int T1,
             int T2, const float* __restrict__ poses, const int* __restrict__ corners,
-            const double* __restrict__ weights, int64_t nvox, int nblk, double* __restrict__ partial) {
+            const double* __restrict__ weights, float rskip, int nblk, double* __restrict__ partial) {
   const int p = blockIdx.y, blk = blockIdx.x;
   const float* P = poses + 12 * p;  // R^T row-major (9), s (3)
   const float r00 = P[0], r01 = P[1], r02 = P[2], r10 = P[3], r11 = P[4], r12 = P[5], r20 = P[6], r21 = P[7],
@@ -124,8 +129,23 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
 #pragma unroll
     for (int k = 0; k < 6; ++k) w[k] = (float)weights[6 * p + k];
   }
-  const int64_t per = (nvox + nblk - 1) / nblk;
-  const int64_t o0 = per * blk, o1 = min(o0 + per, nvox);
+  // voxels are walked in 4x4x4 cubes, one cube per wave instruction: the
+  // rotated samples of a cube fall into a compact region of the template
+  // brick, so the 8 trilinear gathers of a wave touch few cache lines (a
+  // row of 64 voxels would sample a line across the brick).  Every term of
+  // the moments and of their gradient carries the interpolated mask, so a
+  // cube whose samples are all farther than rskip = rmask + sqrt(3) from the
+  // template centre adds exact zeros and is skipped: |q - c| >= |o - c| -
+  // |shift| for q = R^T (o - c) + c + shift.
+  const int ttx = (T2 + 3) >> 2, tty = (T1 + 3) >> 2, ttz = (T0 + 3) >> 2;
+  const int ncube = ttx * tty * ttz;
+  const int per = (ncube + nblk - 1) / nblk;
+  const int c0 = per * blk, c1 = min(c0 + per, ncube);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+  const float shift = sqrtf(P[9] * P[9] + P[10] * P[10] + P[11] * P[11]);
+  const float reach = rskip + shift + 2.6f;  // + the half diagonal of a cube (1.5 sqrt 3) and rounding
+  const float reach2 = reach * reach;
   double acc[BWD ? 12 : 6];
   float ga[12];
 #pragma unroll
@@ -133,10 +153,23 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
 #pragma unroll
   for (int k = 0; k < 12; ++k) ga[k] = 0.f;
   int nacc = 0;
-  for (int64_t o = o0 + threadIdx.x; o < o1; o += TM_RF_THREADS) {
-    const int ix = (int)(o % T2);
-    const int64_t zy = o / T2;
-    const int iy = (int)(zy % T1), iz = (int)(zy / T1);
+  int cube = c0 + wv;
+  int ux = cube % ttx, uy = (cube / ttx) % tty, uz = cube / (ttx * tty);
+  for (; cube < c1; cube += 4) {
+    if (cube != c0 + wv) {
+      ux += 4;
+      while (ux >= ttx) {
+        ux -= ttx;
+        if (++uy >= tty) {
+          uy = 0;
+          ++uz;
+        }
+      }
+    }
+    const float ez = 4.f * uz + 1.5f - cz, ey = 4.f * uy + 1.5f - cy, ex = 4.f * ux + 1.5f - cx;
+    if (ez * ez + ey * ey + ex * ex > reach2) continue;  // uniform over the wave
+    const int ix = 4 * ux + lx, iy = 4 * uy + ly, iz = 4 * uz + lz;
+    if (ix >= T2 || iy >= T1 || iz >= T0) continue;
     const float dz = (float)iz - cz, dy = (float)iy - cy, dx = (float)ix - cx;
     const float qz = r00 * dz + r01 * dy + r02 * dx + bz;
     const float qy = r10 * dz + r11 * dy + r12 * dx + by;
@@ -147,12 +180,21 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
     const float f = inside ? target[((int64_t)pz * N1 + py) * N2 + px] : 0.f;
     if (!BWD) {
       const float gmv = sm.g * sm.m, mf = sm.m * f;
-      acc[0] += (double)sm.m;
-      acc[1] += (double)gmv;
-      acc[2] += (double)(gmv * sm.g);
-      acc[3] += (double)(gmv * f);
-      acc[4] += (double)mf;
-      acc[5] += (double)(mf * f);
+      ga[0] += sm.m;
+      ga[1] += gmv;
+      ga[2] += gmv * sm.g;
+      ga[3] += gmv * f;
+      ga[4] += mf;
+      ga[5] += mf * f;
+      // flush the float partial sums into FP64 every 32 voxels
+      if (++nacc == 32) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          acc[k] += (double)ga[k];
+          ga[k] = 0.f;
+        }
+        nacc = 0;
+      }
     } else {
       const float ag = w[1] * sm.m + 2.f * w[2] * sm.g * sm.m + w[3] * sm.m * f;
       const float am = w[0] + w[1] * sm.g + w[2] * sm.g * sm.g + w[3] * sm.g * f + w[4] * f + w[5] * f * f;
@@ -183,10 +225,8 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
       }
     }
   }
-  if (BWD) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] += (double)ga[k];
-  }
+  for (int k = 0; k < (BWD ? 12 : 6); ++k) acc[k] += (double)ga[k];
   block_store<BWD ? 12 : 6>(acc, partial + ((int64_t)p * nblk + blk) * 12);
 }
 
@@ -198,6 +238,18 @@ __global__ void k_flc_local_reduce(const double* __restrict__ partial, int npose
   double v = 0.0;
   for (int b = 0; b < nblk; ++b) v += partial[((int64_t)p * nblk + b) * 12 + k];
   out[i] = v;
+}
+
+// largest distance of a nonzero mask voxel from the centre (T // 2 per axis),
+// as order-preserving bits of a non-negative float
+__global__ void k_mask_radius(const float* __restrict__ mask, int T0, int T1, int T2, unsigned* __restrict__ out) {
+  const int64_t n = (int64_t)T0 * T1 * T2;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (mask[i] == 0.f) continue;
+    const int x = (int)(i % T2), y = (int)((i / T2) % T1), z = (int)(i / ((int64_t)T1 * T2));
+    const float dz = (float)(z - T0 / 2), dy = (float)(y - T1 / 2), dx = (float)(x - T2 / 2);
+    atomicMax(out, __float_as_uint(sqrtf(dz * dz + dy * dy + dx * dx)));
+  }
 }
 
 __global__ void k_interleave(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
@@ -218,12 +270,12 @@ int local_launch(tm_flc_local_plan* p, bool bwd, const float* target, const int6
     TM_TIMED("flc_local_bwd", bytes, s,
              hipLaunchKernelGGL(k_flc_local<true>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
                                 (int)tshape[1], (int)tshape[2], p->gm, p->T[0], p->T[1], p->T[2], poses, corners,
-                                weights, p->nvox, p->nblk, p->partial));
+                                weights, p->rmask + 1.7321f, p->nblk, p->partial));
   else
     TM_TIMED("flc_local_fwd", bytes, s,
              hipLaunchKernelGGL(k_flc_local<false>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
                                 (int)tshape[1], (int)tshape[2], p->gm, p->T[0], p->T[1], p->T[2], poses, corners,
-                                (const double*)nullptr, p->nvox, p->nblk, p->partial));
+                                (const double*)nullptr, p->rmask + 1.7321f, p->nblk, p->partial));
   TM_CHECK_LAUNCH();
   const int nv = bwd ? 12 : 6;
   hipLaunchKernelGGL(k_flc_local_reduce, dim3((unsigned)tm_ceil_div((int64_t)nposes * nv, 256)), dim3(256), 0, s,
@@ -263,6 +315,16 @@ int tm_flc_local_plan_create(const float* template_std, const float* mask, const
   hipLaunchKernelGGL(k_interleave, dim3((unsigned)tm_ceil_div(p->nvox, 256)), dim3(256), 0, 0, template_std, mask,
                      p->nvox, p->gm);
   e = hipGetLastError();
+  // the mask radius (the partial buffer's first word serves as the counter)
+  unsigned rbits = 0u;
+  if (e == hipSuccess) e = hipMemcpy(p->partial, &rbits, sizeof(unsigned), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mask_radius, dim3((unsigned)std::min<int64_t>(tm_ceil_div(p->nvox, 256), 1024)), dim3(256), 0,
+                       0, mask, p->T[0], p->T[1], p->T[2], (unsigned*)p->partial);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&rbits, p->partial, sizeof(unsigned), hipMemcpyDeviceToHost);
+  memcpy(&p->rmask, &rbits, sizeof(float));
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) {
     tm_set_error("preparing the template brick: %s", hipGetErrorString(e));
