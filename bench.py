@@ -217,7 +217,7 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps):
     kern = {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 3), "launches": v[1]} for k, v in report.items()
             if k.startswith("flc_local")}
     return {
-        "metric": "refined peaks/s (local FLC, Adam with the backward kernel)",
+        "metric": "refined peaks/s (local FLC, Adam; moments and their Jacobian in one kernel pass per step)",
         "value": round(n_peaks / elapsed, 1),
         "unit": "refined peaks/s",
         "peaks": n_peaks,

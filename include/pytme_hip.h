@@ -155,6 +155,12 @@ int tm_flc_local_fwd(tm_flc_local_plan* plan, const float* target, const int64_t
                      const int* corners, int n_poses, double* moments, void* stream);
 int tm_flc_local_bwd(tm_flc_local_plan* plan, const float* target, const int64_t* target_shape, const float* poses,
                      const int* corners, int n_poses, const double* weights, double* grad, void* stream);
+/* No counterpart in the reference: the moments and their Jacobian in one
+ * pass, out float64 [n][78] = six moments, then per moment the 12
+ * derivatives d(moment)/d(R^T) (9, row-major) and d(moment)/ds (3); the
+ * gradient of any function of the moments is then a 6-term contraction. */
+int tm_flc_local_fwd_jac(tm_flc_local_plan* plan, const float* target, const int64_t* target_shape,
+                         const float* poses, const int* corners, int n_poses, double* out, void* stream);
 
 /* ---- peak extraction on the max-score volume -------------------------------
  * tm_peaks_max_filter + tm_peaks_write replace max_filter_coordinates

@@ -78,6 +78,7 @@ SIGNATURES = {
     "tm_flc_local_plan_create": (c_int, [c_void_p, c_void_p, c_int_p, c_int, POINTER(c_void_p)]),
     "tm_flc_local_plan_destroy": (c_int, [c_void_p]),
     "tm_flc_local_fwd": (c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "tm_flc_local_fwd_jac": (c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "tm_flc_local_bwd": (
         c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
     ),

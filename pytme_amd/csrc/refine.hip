@@ -31,6 +31,7 @@
 #include "tm_common.h"
 
 #define TM_RF_THREADS 256
+#define TM_RF_NJ 78  // moments (6) + Jacobian (6 x 12) per pose
 
 struct tm_flc_local_plan {
   int T[3];
@@ -112,11 +113,17 @@ __device__ __forceinline__ void block_store(double* acc, double* out) {
   }
 }
 
-template <bool BWD>
+// MODE 0: the six moments; 1: dScore/d(R^T, shift) for given weights
+// dScore/d(moment); 2: the moments and their Jacobian d(moment)/d(R^T,
+// shift) (6 x 12) in one pass, so that the gradient of any function of the
+// moments needs no second pass over the samples.
+template <int MODE>
 __global__ void __launch_bounds__(TM_RF_THREADS)
 k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const float2* __restrict__ gm, int T0, int T1,
             int T2, const float* __restrict__ poses, const int* __restrict__ corners,
             const double* __restrict__ weights, float rskip, int nblk, double* __restrict__ partial) {
+  constexpr bool BWD = MODE == 1, JAC = MODE == 2;
+  constexpr int NV = MODE == 0 ? 6 : (MODE == 1 ? 12 : TM_RF_NJ);
   const int p = blockIdx.y, blk = blockIdx.x;
   const float* P = poses + 12 * p;  // R^T row-major (9), s (3)
   const float r00 = P[0], r01 = P[1], r02 = P[2], r10 = P[3], r11 = P[4], r12 = P[5], r20 = P[6], r21 = P[7],
@@ -146,12 +153,15 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
   const float shift = sqrtf(P[9] * P[9] + P[10] * P[10] + P[11] * P[11]);
   const float reach = rskip + shift + 2.6f;  // + the half diagonal of a cube (1.5 sqrt 3) and rounding
   const float reach2 = reach * reach;
-  double acc[BWD ? 12 : 6];
-  float ga[12];
+  // MODE 0/1: FP32 partial sums flushed into FP64 every 32 samples; MODE 2
+  // (78 sums) keeps FP32 per thread (a thread sees at most 64 samples per
+  // pose: nblk = ceil(T^3 / (256 * 64))) and reduces in FP64
+  double acc[JAC ? 1 : NV];
+  float ga[NV];
 #pragma unroll
-  for (int k = 0; k < (BWD ? 12 : 6); ++k) acc[k] = 0.0;
+  for (int k = 0; k < (JAC ? 1 : NV); ++k) acc[k] = 0.0;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) ga[k] = 0.f;
+  for (int k = 0; k < NV; ++k) ga[k] = 0.f;
   int nacc = 0;
   int cube = c0 + wv;
   int ux = cube % ttx, uy = (cube / ttx) % tty, uz = cube / (ttx * tty);
@@ -174,11 +184,35 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
     const float qz = r00 * dz + r01 * dy + r02 * dx + bz;
     const float qy = r10 * dz + r11 * dy + r12 * dx + by;
     const float qx = r20 * dz + r21 * dy + r22 * dx + bx;
-    const Sample sm = trilinear<BWD>(gm, T0, T1, T2, qz, qy, qx);
+    const Sample sm = trilinear<MODE != 0>(gm, T0, T1, T2, qz, qy, qx);
     const int pz = vz + iz, py = vy + iy, px = vx + ix;
     const bool inside = pz >= 0 && py >= 0 && px >= 0 && pz < N0 && py < N1 && px < N2;
     const float f = inside ? target[((int64_t)pz * N1 + py) * N2 + px] : 0.f;
-    if (!BWD) {
+    if (JAC) {
+      const float g = sm.g, m = sm.m, gmv = g * m, mf = m * f;
+      ga[0] += m;
+      ga[1] += gmv;
+      ga[2] += gmv * g;
+      ga[3] += gmv * f;
+      ga[4] += mf;
+      ga[5] += mf * f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        // d(moment)/dq_a of n, sum gm, sum g^2 m, sum g m f, sum m f, sum m f^2
+        const float dm = sm.dm[a], dg = sm.dg[a];
+        const float G1 = g * dm + m * dg;
+        const float G[6] = {dm, G1, g * (g * dm + 2.f * m * dg), f * G1, f * dm, f * f * dm};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          // q = R^T d + c + shift: d q_a / d (R^T)_ab = d_b, d q / d shift = I
+          float* J = ga + 6 + 12 * k;
+          J[3 * a + 0] += G[k] * dz;
+          J[3 * a + 1] += G[k] * dy;
+          J[3 * a + 2] += G[k] * dx;
+          J[9 + a] += G[k];
+        }
+      }
+    } else if (!BWD) {
       const float gmv = sm.g * sm.m, mf = sm.m * f;
       ga[0] += sm.m;
       ga[1] += gmv;
@@ -225,9 +259,16 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
       }
     }
   }
+  if (JAC) {
+    double out[NV];
 #pragma unroll
-  for (int k = 0; k < (BWD ? 12 : 6); ++k) acc[k] += (double)ga[k];
-  block_store<BWD ? 12 : 6>(acc, partial + ((int64_t)p * nblk + blk) * 12);
+    for (int k = 0; k < NV; ++k) out[k] = (double)ga[k];
+    block_store<NV>(out, partial + ((int64_t)p * nblk + blk) * TM_RF_NJ);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += (double)ga[k];
+    block_store<NV>(acc, partial + ((int64_t)p * nblk + blk) * TM_RF_NJ);
+  }
 }
 
 __global__ void k_flc_local_reduce(const double* __restrict__ partial, int nposes, int nblk, int nv,
@@ -236,7 +277,7 @@ __global__ void k_flc_local_reduce(const double* __restrict__ partial, int npose
   if (i >= nposes * nv) return;
   const int p = i / nv, k = i - p * nv;
   double v = 0.0;
-  for (int b = 0; b < nblk; ++b) v += partial[((int64_t)p * nblk + b) * 12 + k];
+  for (int b = 0; b < nblk; ++b) v += partial[((int64_t)p * nblk + b) * TM_RF_NJ + k];
   out[i] = v;
 }
 
@@ -258,26 +299,34 @@ __global__ void k_interleave(const float* __restrict__ a, const float* __restric
   if (i < n) out[i] = make_float2(a[i], b[i]);
 }
 
-int local_launch(tm_flc_local_plan* p, bool bwd, const float* target, const int64_t* tshape, const float* poses,
+int local_launch(tm_flc_local_plan* p, int mode, const float* target, const int64_t* tshape, const float* poses,
                  const int* corners, int nposes, const double* weights, double* out, hipStream_t s) {
+  const bool bwd = mode == 1;
   TM_REQUIRE(p && target && tshape && poses && corners && out, "null argument to the local FLC kernels");
   TM_REQUIRE(nposes >= 1 && nposes <= p->max_poses, "%d poses outside [1, %d]", nposes, p->max_poses);
   TM_REQUIRE(!bwd || weights, "the gradient needs the score weights");
   for (int i = 0; i < 3; ++i) TM_REQUIRE(tshape[i] >= 1 && tshape[i] < (1 << 30), "target axis %d out of range", i);
   const dim3 grid((unsigned)p->nblk, (unsigned)nposes);
   const double bytes = (double)nposes * p->nvox * 4.0;  // target window (template brick is L2-resident)
-  if (bwd)
+  const float rskip = p->rmask + 1.7321f;  // + sqrt(3): the trilinear cell's reach
+  const int T0 = p->T[0], T1 = p->T[1], T2 = p->T[2];
+  if (mode == 1)
     TM_TIMED("flc_local_bwd", bytes, s,
-             hipLaunchKernelGGL(k_flc_local<true>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
-                                (int)tshape[1], (int)tshape[2], p->gm, p->T[0], p->T[1], p->T[2], poses, corners,
-                                weights, p->rmask + 1.7321f, p->nblk, p->partial));
+             hipLaunchKernelGGL(k_flc_local<1>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
+                                (int)tshape[1], (int)tshape[2], p->gm, T0, T1, T2, poses, corners, weights, rskip,
+                                p->nblk, p->partial));
+  else if (mode == 2)
+    TM_TIMED("flc_local_fwd_jac", bytes, s,
+             hipLaunchKernelGGL(k_flc_local<2>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
+                                (int)tshape[1], (int)tshape[2], p->gm, T0, T1, T2, poses, corners,
+                                (const double*)nullptr, rskip, p->nblk, p->partial));
   else
     TM_TIMED("flc_local_fwd", bytes, s,
-             hipLaunchKernelGGL(k_flc_local<false>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
-                                (int)tshape[1], (int)tshape[2], p->gm, p->T[0], p->T[1], p->T[2], poses, corners,
-                                (const double*)nullptr, p->rmask + 1.7321f, p->nblk, p->partial));
+             hipLaunchKernelGGL(k_flc_local<0>, grid, dim3(TM_RF_THREADS), 0, s, target, (int)tshape[0],
+                                (int)tshape[1], (int)tshape[2], p->gm, T0, T1, T2, poses, corners,
+                                (const double*)nullptr, rskip, p->nblk, p->partial));
   TM_CHECK_LAUNCH();
-  const int nv = bwd ? 12 : 6;
+  const int nv = mode == 1 ? 12 : (mode == 2 ? TM_RF_NJ : 6);
   hipLaunchKernelGGL(k_flc_local_reduce, dim3((unsigned)tm_ceil_div((int64_t)nposes * nv, 256)), dim3(256), 0, s,
                      p->partial, nposes, p->nblk, nv, out);
   TM_CHECK_LAUNCH();
@@ -300,11 +349,13 @@ int tm_flc_local_plan_create(const float* template_std, const float* mask, const
   for (int i = 0; i < 3; ++i) p->T[i] = tshape[i];
   p->nvox = (int64_t)tshape[0] * tshape[1] * tshape[2];
   p->max_poses = max_poses;
-  // ~64 voxels per thread, at least one block per pose
+  // at most 64 voxels per thread (the FP32 per-thread sums of the Jacobian
+  // pass rely on it), at least one block per pose
   int64_t nb = tm_ceil_div(p->nvox, (int64_t)TM_RF_THREADS * 64);
-  p->nblk = (int)(nb < 1 ? 1 : (nb > 64 ? 64 : nb));
+  p->nblk = (int)(nb < 1 ? 1 : nb);
   hipError_t e = hipMalloc((void**)&p->gm, sizeof(float2) * p->nvox);
-  if (e == hipSuccess) e = hipMalloc((void**)&p->partial, sizeof(double) * 12 * (size_t)max_poses * p->nblk);
+  if (e == hipSuccess)
+    e = hipMalloc((void**)&p->partial, sizeof(double) * TM_RF_NJ * (size_t)max_poses * p->nblk);
   if (e != hipSuccess) {
     tm_set_error("hipMalloc failed: %s", hipGetErrorString(e));
     if (p->gm) hipFree(p->gm);
@@ -350,15 +401,22 @@ int tm_flc_local_plan_destroy(tm_flc_local_plan* p) {
 // corners int32 [n][3] = v; out float64 [n][6].
 int tm_flc_local_fwd(tm_flc_local_plan* p, const float* target, const int64_t* target_shape, const float* poses,
                      const int* corners, int n_poses, double* moments, void* stream) {
-  return local_launch(p, false, target, target_shape, poses, corners, n_poses, nullptr, moments,
-                      (hipStream_t)stream);
+  return local_launch(p, 0, target, target_shape, poses, corners, n_poses, nullptr, moments, (hipStream_t)stream);
+}
+
+// Moments and their Jacobian in one pass: out float64 [n][78] = the six
+// moments, then for each moment k the 12 derivatives d(moment_k)/d(R^T)
+// row-major (9) and d(moment_k)/ds (3).
+int tm_flc_local_fwd_jac(tm_flc_local_plan* p, const float* target, const int64_t* target_shape, const float* poses,
+                         const int* corners, int n_poses, double* out, void* stream) {
+  return local_launch(p, 2, target, target_shape, poses, corners, n_poses, nullptr, out, (hipStream_t)stream);
 }
 
 // Gradient: weights float64 [n][6] = dS/d(moment); out float64 [n][12] =
 // dS/d(R^T) row-major (9), dS/ds (3).
 int tm_flc_local_bwd(tm_flc_local_plan* p, const float* target, const int64_t* target_shape, const float* poses,
                      const int* corners, int n_poses, const double* weights, double* grad, void* stream) {
-  return local_launch(p, true, target, target_shape, poses, corners, n_poses, weights, grad, (hipStream_t)stream);
+  return local_launch(p, 1, target, target_shape, poses, corners, n_poses, weights, grad, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -128,6 +128,20 @@ class LocalFLC:
         self.evaluations += poses.shape[0]
         return out
 
+    def moments_jacobian(self, Rt, s, v):
+        """(n, 6) float64 moments and (n, 6, 12) float64 Jacobian
+        d(moment)/d(R^T row-major (9), s (3)) from one pass over the samples
+        (``tm_flc_local_fwd_jac``)."""
+        torch = _torch()
+        poses, corners = self._pack(Rt, s, v)
+        n = poses.shape[0]
+        out = torch.empty((n, 78), dtype=torch.float64, device=self.device)
+        _lib.call("tm_flc_local_fwd_jac", self._plan, self.target.data_ptr(), self._tshape, poses.data_ptr(),
+                  corners.data_ptr(), n, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        self.evaluations += n
+        self.gradients += n
+        return out[:, :6], out[:, 6:].reshape(n, 6, 12)
+
     def moment_grad(self, Rt, s, v, weights):
         """(n, 12) float64: dScore/d(R^T) (9, row-major) and dScore/ds (3)
         for weights = dScore/d(moment) (n, 6)."""
@@ -152,16 +166,23 @@ def _make_function():
 
         @staticmethod
         def forward(ctx, Rt, s, v, local):
-            ctx.save_for_backward(Rt, s)
-            ctx.v, ctx.local = v, local
+            # a gradient will be asked for: moments and their Jacobian in one
+            # pass (the backward is then a 6-term contraction); otherwise the
+            # moments alone
+            if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+                moments, jac = local.moments_jacobian(Rt.detach(), s.detach(), v)
+                ctx.save_for_backward(jac)
+                ctx.shapes = (Rt.dtype, s.dtype)
+                return moments
             return local.moments(Rt.detach(), s.detach(), v)
 
         @staticmethod
         def backward(ctx, grad_moments):
-            Rt, s = ctx.saved_tensors
-            g = ctx.local.moment_grad(Rt.detach(), s.detach(), ctx.v, grad_moments)
-            n = Rt.shape[0]
-            return g[:, :9].reshape(n, 3, 3).to(Rt.dtype), g[:, 9:].to(s.dtype), None, None
+            (jac,) = ctx.saved_tensors
+            g = torch.einsum("nk,nkj->nj", grad_moments.to(jac.dtype), jac)
+            n = g.shape[0]
+            rt_dtype, s_dtype = ctx.shapes
+            return g[:, :9].reshape(n, 3, 3).to(rt_dtype), g[:, 9:].to(s_dtype), None, None
 
     return LocalMoments
 
@@ -282,7 +303,8 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
         scores = []
         for lo in range(0, n, bsz):
             hi = min(lo + bsz, n)
-            sc = score_from_moments(pose_moments(local, t[lo:hi], a[lo:hi], subvoxel))
+            with torch.set_grad_enabled(step < steps):  # the last evaluation needs no Jacobian
+                sc = score_from_moments(pose_moments(local, t[lo:hi], a[lo:hi], subvoxel))
             scores.append(sc.detach())
             if step < steps:
                 (-sc.sum()).backward()

@@ -120,6 +120,35 @@ def test_backward_matches_cpu_fp64_autograd(subvoxel):
         assert np.abs(ga[i] - ra).max() < 2e-3 * scale, (i, ga[i], ra)
 
 
+def test_one_pass_jacobian_matches_the_backward_kernel():
+    """The moments and Jacobian of tm_flc_local_fwd_jac: the moments equal
+    tm_flc_local_fwd's and w . J equals tm_flc_local_bwd's gradient for
+    random weights w (two kernels with different accumulation)."""
+    import torch
+
+    from pytme_amd.refine import LocalFLC
+
+    rng = np.random.default_rng(31)
+    target = gaussian_filter(rng.normal(size=(34, 32, 30)), 0.7).astype(np.float32)
+    template, mask = _template(32, 13)
+    t, a = _poses(rng, 24, target.shape, 13)
+    local = LocalFLC(target, template, mask, max_poses=32)
+    print("one-pass Jacobian vs backward kernel, 24 poses", flush=True)
+    Rt = _gpu_R(a).cuda()
+    v = np.trunc(t).astype(np.int32)
+    s = torch.as_tensor(t - v, dtype=torch.float64).cuda()
+    mom, jac = local.moments_jacobian(Rt, s, v)
+    ref = local.moments(Rt, s, v)
+    scale = torch.maximum(ref.abs(), torch.ones_like(ref))
+    assert float(((mom - ref).abs() / scale).max()) < 1e-5
+    w = torch.as_tensor(rng.normal(size=(24, 6)), dtype=torch.float64).cuda()
+    g_bwd = local.moment_grad(Rt, s, v, w)
+    g_jac = torch.einsum("nk,nkj->nj", w, jac)
+    rel = (g_jac - g_bwd).abs().max() / g_bwd.abs().max()
+    print(f"  max relative gradient difference {float(rel):.2e}", flush=True)
+    assert float(rel) < 1e-4
+
+
 def test_backward_matches_central_differences():
     """h = 1e-2 voxel / 1e-3 rad on FP32 GPU scores; the trilinear interpolant
     is piecewise linear, so the differences average slopes over +-h."""
