@@ -72,7 +72,8 @@ class LocalFLC:
     ``_post_init`` does.
     """
 
-    def __init__(self, target, template, template_mask=None, max_poses: int = 4096, device="cuda"):
+    def __init__(self, target, template, template_mask=None, max_poses: int = 4096, device="cuda",
+                 one_pass_gradient: bool = True):
         torch = _torch()
         template = np.asarray(template, dtype=np.float32)
         if template.ndim != 3:
@@ -98,6 +99,10 @@ class LocalFLC:
         self._tshape = _lib.i64_array(self.target.shape)
         self.evaluations = 0  # poses scored (forward kernel)
         self.gradients = 0  # poses differentiated (backward kernel)
+        #: gradients from the one-pass moments-and-Jacobian kernel (default)
+        #: or from the backward kernel, which recomputes the interpolation
+        #: weights of every sample
+        self.one_pass_gradient = bool(one_pass_gradient)
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -169,17 +174,25 @@ def _make_function():
             # a gradient will be asked for: moments and their Jacobian in one
             # pass (the backward is then a 6-term contraction); otherwise the
             # moments alone
-            if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+            ctx.shapes = (Rt.dtype, s.dtype)
+            ctx.jac = False
+            if (ctx.needs_input_grad[0] or ctx.needs_input_grad[1]) and local.one_pass_gradient:
                 moments, jac = local.moments_jacobian(Rt.detach(), s.detach(), v)
                 ctx.save_for_backward(jac)
-                ctx.shapes = (Rt.dtype, s.dtype)
+                ctx.jac = True
                 return moments
+            ctx.save_for_backward(Rt, s)
+            ctx.v, ctx.local = v, local
             return local.moments(Rt.detach(), s.detach(), v)
 
         @staticmethod
         def backward(ctx, grad_moments):
-            (jac,) = ctx.saved_tensors
-            g = torch.einsum("nk,nkj->nj", grad_moments.to(jac.dtype), jac)
+            if ctx.jac:
+                (jac,) = ctx.saved_tensors
+                g = torch.einsum("nk,nkj->nj", grad_moments.to(jac.dtype), jac)
+            else:
+                Rt, s = ctx.saved_tensors
+                g = ctx.local.moment_grad(Rt.detach(), s.detach(), ctx.v, grad_moments)
             n = g.shape[0]
             rt_dtype, s_dtype = ctx.shapes
             return g[:, :9].reshape(n, 3, 3).to(rt_dtype), g[:, 9:].to(s_dtype), None, None

@@ -100,7 +100,10 @@ def _gpu_score_and_grad(local, t, a_rad, subvoxel="rigid"):
 
 
 @pytest.mark.parametrize("subvoxel", ["rigid", "reference"])
-def test_backward_matches_cpu_fp64_autograd(subvoxel):
+@pytest.mark.parametrize("one_pass", [True, False])
+def test_backward_matches_cpu_fp64_autograd(subvoxel, one_pass):
+    """Both gradient paths: the one-pass moments-and-Jacobian kernel and the
+    backward kernel that recomputes the interpolation weights."""
     from pytme_amd.refine import LocalFLC
 
     rng = np.random.default_rng(11)
@@ -109,8 +112,8 @@ def test_backward_matches_cpu_fp64_autograd(subvoxel):
     ts = orf.prepare_template(template, mask)
     t = rng.uniform(2, 13, size=(6, 3))
     a = np.deg2rad(rng.uniform(-60, 60, size=(6, 3)))
-    local = LocalFLC(target, template, mask, max_poses=8)
-    print("local backward vs FP64 autograd, 6 poses, 11^3 template", flush=True)
+    local = LocalFLC(target, template, mask, max_poses=8, one_pass_gradient=one_pass)
+    print(f"local gradient (one_pass={one_pass}) vs FP64 autograd, 6 poses, 11^3 template", flush=True)
     sc, gt, ga = _gpu_score_and_grad(local, t, a, subvoxel)
     for i in range(t.shape[0]):
         ref, rt, ra = _autograd_reference(target, ts, mask, t[i], a[i], subvoxel)
@@ -149,7 +152,8 @@ def test_one_pass_jacobian_matches_the_backward_kernel():
     assert float(rel) < 1e-4
 
 
-def test_backward_matches_central_differences():
+@pytest.mark.parametrize("one_pass", [True, False])
+def test_backward_matches_central_differences(one_pass):
     """h = 1e-2 voxel / 1e-3 rad on FP32 GPU scores; the trilinear interpolant
     is piecewise linear, so the differences average slopes over +-h."""
     from pytme_amd.refine import LocalFLC
@@ -159,7 +163,7 @@ def test_backward_matches_central_differences():
     template, mask = _template(14, 13)
     t = np.array([[6.3, 7.6, 5.45], [8.2, 4.7, 9.35]])
     a = np.deg2rad(np.array([[15.0, 30.0, -20.0], [-40.0, 75.0, 110.0]]))
-    local = LocalFLC(target, template, mask, max_poses=64)
+    local = LocalFLC(target, template, mask, max_poses=64, one_pass_gradient=one_pass)
     print("local backward vs central differences", flush=True)
     _, gt, ga = _gpu_score_and_grad(local, t, a)
 
