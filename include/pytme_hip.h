@@ -161,6 +161,21 @@ int tm_flc_local_bwd(tm_flc_local_plan* plan, const float* target, const int64_t
  * gradient of any function of the moments is then a 6-term contraction. */
 int tm_flc_local_fwd_jac(tm_flc_local_plan* plan, const float* target, const int64_t* target_shape,
                          const float* poses, const int* corners, int n_poses, double* out, void* stream);
+/* No counterpart in the reference (its refinement is a derivative-free
+ * scipy loop per peak, scripts/postprocess.py:588-613): the batched Adam
+ * step of the gradient refinement on the device.  tm_refine_prepare turns
+ * translations t and zyz angles (float64 [n][3], radians) into poses and
+ * corners (v = trunc(t), shift -R^T (t - v) or t - v); tm_refine_adam takes
+ * the moments-and-Jacobian output, tracks the best score and pose, and
+ * applies one Adam step (torch's update, ascending the score) to t and the
+ * angles, with adam_m / adam_v float64 [n][6] state and t kept within
+ * bound of t0 when bound >= 0. */
+int tm_refine_prepare(const double* t, const double* angles, int n, int rigid, float* poses, int* corners,
+                      void* stream);
+int tm_refine_adam(const double* moments_jac, double* t, double* angles, const double* t0, double* adam_m,
+                   double* adam_v, double* best_score, double* best_t, double* best_angles, int n, int rigid,
+                   int update, double lr_t, double lr_a, double beta1, double beta2, double eps, int adam_step,
+                   double bound, double eps32, void* stream);
 
 /* ---- peak extraction on the max-score volume -------------------------------
  * tm_peaks_max_filter + tm_peaks_write replace max_filter_coordinates

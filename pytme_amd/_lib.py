@@ -79,6 +79,14 @@ SIGNATURES = {
     "tm_flc_local_plan_destroy": (c_int, [c_void_p]),
     "tm_flc_local_fwd": (c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "tm_flc_local_fwd_jac": (c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "tm_refine_prepare": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "tm_refine_adam": (
+        c_int,
+        [
+            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+            c_int, c_double, c_double, c_double, c_double, c_double, c_int, c_double, c_double, c_void_p,
+        ],
+    ),
     "tm_flc_local_bwd": (
         c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
     ),

@@ -333,6 +333,179 @@ int local_launch(tm_flc_local_plan* p, int mode, const float* target, const int6
   return TM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// fused refinement step (one thread per pose)
+
+// R = Rz(c) Ry(b) Rz(a) for zyz angles (a, b, c) (scipy "zyz" extrinsic) and
+// dR/da, dR/db, dR/dc
+__device__ void zyz_and_derivs(double a, double b, double c, double R[9], double D[3][9]) {
+  const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cc = cos(c), sc = sin(c);
+  R[0] = cc * cb * ca - sc * sa;
+  R[1] = -cc * cb * sa - sc * ca;
+  R[2] = cc * sb;
+  R[3] = sc * cb * ca + cc * sa;
+  R[4] = -sc * cb * sa + cc * ca;
+  R[5] = sc * sb;
+  R[6] = -sb * ca;
+  R[7] = sb * sa;
+  R[8] = cb;
+  // d/da
+  D[0][0] = -cc * cb * sa - sc * ca;
+  D[0][1] = -cc * cb * ca + sc * sa;
+  D[0][2] = 0.0;
+  D[0][3] = -sc * cb * sa + cc * ca;
+  D[0][4] = -sc * cb * ca - cc * sa;
+  D[0][5] = 0.0;
+  D[0][6] = sb * sa;
+  D[0][7] = sb * ca;
+  D[0][8] = 0.0;
+  // d/db
+  D[1][0] = -cc * sb * ca;
+  D[1][1] = cc * sb * sa;
+  D[1][2] = cc * cb;
+  D[1][3] = -sc * sb * ca;
+  D[1][4] = sc * sb * sa;
+  D[1][5] = sc * cb;
+  D[1][6] = -cb * ca;
+  D[1][7] = cb * sa;
+  D[1][8] = -sb;
+  // d/dc
+  D[2][0] = -sc * cb * ca - cc * sa;
+  D[2][1] = sc * cb * sa - cc * ca;
+  D[2][2] = -sc * sb;
+  D[2][3] = cc * cb * ca - sc * sa;
+  D[2][4] = -cc * cb * sa - sc * ca;
+  D[2][5] = cc * sb;
+  D[2][6] = 0.0;
+  D[2][7] = 0.0;
+  D[2][8] = 0.0;
+}
+
+// poses float32 [n][12] (R^T row-major, shift) and corners int32 [n][3] from
+// translations t and zyz angles a (float64 [n][3], radians): v = trunc(t),
+// s = t - v; the shift is -R^T s (rigid) or s (the reference's sampling)
+__global__ void k_refine_prepare(const double* __restrict__ t, const double* __restrict__ ang, int n, int rigid,
+                                 float* __restrict__ poses, int* __restrict__ corners) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double R[9], D[3][9];
+  zyz_and_derivs(ang[3 * i], ang[3 * i + 1], ang[3 * i + 2], R, D);
+  double sv[3];
+  for (int k = 0; k < 3; ++k) {
+    const double v = trunc(t[3 * i + k]);
+    corners[3 * i + k] = (int)v;
+    sv[k] = t[3 * i + k] - v;
+  }
+  float* P = poses + 12 * i;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) P[3 * r + c] = (float)R[3 * c + r];  // R^T
+  for (int r = 0; r < 3; ++r) {
+    double sh = 0.0;  // (R^T s)_r in float64, then rounded like the rest of the pose
+    for (int c = 0; c < 3; ++c) sh += R[3 * c + r] * sv[c];
+    P[9 + r] = rigid ? (float)(-sh) : (float)sv[r];
+  }
+}
+
+struct RefineAdam {
+  double lr_t, lr_a, beta1, beta2, eps, bc1, bc2sqrt, bound, eps32;
+  int rigid, update;
+};
+
+// score and its gradient from the moments and their Jacobian (mj [n][78]),
+// best-pose tracking, then one Adam step (torch's update) ascending the
+// score in t and a; t stays within bound of t0 when bound >= 0
+__global__ void k_refine_adam(const double* __restrict__ mj, double* __restrict__ t, double* __restrict__ ang,
+                              const double* __restrict__ t0, double* __restrict__ mom1, double* __restrict__ mom2,
+                              double* __restrict__ best_s, double* __restrict__ best_t, double* __restrict__ best_a,
+                              int n, RefineAdam h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* M = mj + 78 * i;
+  const double nn = M[0], A = M[1], Bm = M[2], C = M[3], Dd = M[4], E = M[5];
+  const double mu = A / nn;
+  const double var_t = fmax(Bm / nn - mu * mu, 0.0), var_f = fmax(E / nn - (Dd / nn) * (Dd / nn), 0.0);
+  const double sd_t = sqrt(var_t), sd_f = sqrt(var_f);
+  const double P = C - mu * Dd, Q = sd_t * sd_f * nn;
+  const bool flat = sd_f < h.eps32;
+  const double score = flat ? 0.0 : P / Q;
+  if (score > best_s[i]) {
+    best_s[i] = score;
+    for (int k = 0; k < 3; ++k) {
+      best_t[3 * i + k] = t[3 * i + k];
+      best_a[3 * i + k] = ang[3 * i + k];
+    }
+  }
+  if (!h.update) return;
+  // w = dScore / d(n, A, B, C, D, E)
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  if (!flat) {
+    const double dPdn = Dd * A / (nn * nn), dPdA = -Dd / nn, dPdC = 1.0, dPdD = -mu;
+    const double dsdt_dB = 1.0 / (2.0 * sd_t * nn), dsdt_dA = -(A / (nn * nn)) / sd_t,
+                 dsdt_dn = (-Bm / (nn * nn) + 2.0 * A * A / (nn * nn * nn)) / (2.0 * sd_t);
+    const double dsdf_dE = 1.0 / (2.0 * sd_f * nn), dsdf_dD = -(Dd / (nn * nn)) / sd_f,
+                 dsdf_dn = (-E / (nn * nn) + 2.0 * Dd * Dd / (nn * nn * nn)) / (2.0 * sd_f);
+    const double dQdn = nn * (sd_f * dsdt_dn + sd_t * dsdf_dn) + sd_t * sd_f;
+    const double dQdA = nn * sd_f * dsdt_dA, dQdB = nn * sd_f * dsdt_dB;
+    const double dQdD = nn * sd_t * dsdf_dD, dQdE = nn * sd_t * dsdf_dE;
+    const double S = score;
+    w[0] = (dPdn - S * dQdn) / Q;
+    w[1] = (dPdA - S * dQdA) / Q;
+    w[2] = (-S * dQdB) / Q;
+    w[3] = dPdC / Q;
+    w[4] = (dPdD - S * dQdD) / Q;
+    w[5] = (-S * dQdE) / Q;
+  }
+  double g[12];
+  for (int j = 0; j < 12; ++j) {
+    double v = 0.0;
+    for (int k = 0; k < 6; ++k) v += w[k] * M[6 + 12 * k + j];
+    g[j] = v;
+  }
+  // back to (t, a): shift = -R^T s (rigid) or s; s = t - trunc(t); R^T from a
+  double R[9], Dr[3][9];
+  zyz_and_derivs(ang[3 * i], ang[3 * i + 1], ang[3 * i + 2], R, Dr);
+  double Rt[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Rt[3 * r + c] = R[3 * c + r];
+  double sv[3];
+  for (int k = 0; k < 3; ++k) sv[k] = t[3 * i + k] - trunc(t[3 * i + k]);
+  double gRt[9], gs[3];
+  for (int k = 0; k < 9; ++k) gRt[k] = g[k];
+  if (h.rigid) {
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      for (int r = 0; r < 3; ++r) v -= g[9 + r] * Rt[3 * r + c];
+      gs[c] = v;
+    }
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) gRt[3 * r + c] -= g[9 + r] * sv[c];
+  } else {
+    for (int c = 0; c < 3; ++c) gs[c] = g[9 + c];
+  }
+  double ga[3];
+  for (int k = 0; k < 3; ++k) {
+    double v = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) v += gRt[3 * r + c] * Dr[k][3 * c + r];  // dS/dR[c][r] = dS/dRt[r][c]
+    ga[k] = v;
+  }
+  // Adam on the loss -score: m <- lerp(m, grad, 1 - beta1), v <- beta2 v +
+  // (1 - beta2) grad^2, p <- p - lr / bc1 * m / (sqrt(v) / sqrt(bc2) + eps)
+  for (int k = 0; k < 6; ++k) {
+    const bool is_t = k < 3;
+    const double grad = -(is_t ? gs[k] : ga[k - 3]);
+    double* m1 = mom1 + 6 * i + k;
+    double* m2 = mom2 + 6 * i + k;
+    *m1 = *m1 + (1.0 - h.beta1) * (grad - *m1);
+    *m2 = h.beta2 * *m2 + (1.0 - h.beta2) * grad * grad;
+    const double denom = sqrt(*m2) / h.bc2sqrt + h.eps;
+    const double step = (is_t ? h.lr_t : h.lr_a) / h.bc1;
+    double* p = is_t ? t + 3 * i + k : ang + 3 * i + (k - 3);
+    *p -= step * *m1 / denom;
+    if (is_t && h.bound >= 0.0) *p = fmin(fmax(*p, t0[3 * i + k] - h.bound), t0[3 * i + k] + h.bound);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -417,6 +590,42 @@ int tm_flc_local_fwd_jac(tm_flc_local_plan* p, const float* target, const int64_
 int tm_flc_local_bwd(tm_flc_local_plan* p, const float* target, const int64_t* target_shape, const float* poses,
                      const int* corners, int n_poses, const double* weights, double* grad, void* stream) {
   return local_launch(p, 1, target, target_shape, poses, corners, n_poses, weights, grad, (hipStream_t)stream);
+}
+
+// Fused refinement step helpers (no counterpart in the reference): pose
+// preparation and the score / gradient / Adam / best-pose update.
+int tm_refine_prepare(const double* t, const double* angles, int n, int rigid, float* poses, int* corners,
+                      void* stream) {
+  TM_REQUIRE(t && angles && poses && corners && n >= 1, "bad arguments to tm_refine_prepare");
+  hipLaunchKernelGGL(k_refine_prepare, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, t,
+                     angles, n, rigid, poses, corners);
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+int tm_refine_adam(const double* moments_jac, double* t, double* angles, const double* t0, double* adam_m,
+                   double* adam_v, double* best_score, double* best_t, double* best_angles, int n, int rigid,
+                   int update, double lr_t, double lr_a, double beta1, double beta2, double eps, int adam_step,
+                   double bound, double eps32, void* stream) {
+  TM_REQUIRE(moments_jac && t && angles && t0 && adam_m && adam_v && best_score && best_t && best_angles && n >= 1,
+             "bad arguments to tm_refine_adam");
+  TM_REQUIRE(adam_step >= 1, "Adam steps count from 1");
+  RefineAdam h;
+  h.lr_t = lr_t;
+  h.lr_a = lr_a;
+  h.beta1 = beta1;
+  h.beta2 = beta2;
+  h.eps = eps;
+  h.bc1 = 1.0 - pow(beta1, (double)adam_step);
+  h.bc2sqrt = sqrt(1.0 - pow(beta2, (double)adam_step));
+  h.bound = bound;
+  h.eps32 = eps32;
+  h.rigid = rigid;
+  h.update = update;
+  hipLaunchKernelGGL(k_refine_adam, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     moments_jac, t, angles, t0, adam_m, adam_v, best_score, best_t, best_angles, n, h);
+  TM_CHECK_LAUNCH();
+  return TM_OK;
 }
 
 }  // extern "C"

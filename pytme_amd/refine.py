@@ -285,17 +285,60 @@ def score_poses(local: LocalFLC, translations, angles_deg, subvoxel: str = "rigi
         return torch.cat(out)
 
 
+def _refine_fused(local, t0, a0, steps, lr_t, lr_a, bound, bsz, subvoxel):
+    """The Adam loop of :func:`refine_peaks` on the device: per batch and
+    step one pose-preparation kernel, the moments-and-Jacobian kernel and one
+    kernel for the score, its analytic gradient in (t, angles), best-pose
+    tracking and the Adam update (torch's formulas), so no torch work runs
+    between the kernels."""
+    torch = _torch()
+    dev = local.device
+    n = t0.shape[0]
+    if subvoxel not in ("rigid", "reference"):
+        raise ValueError("subvoxel is 'rigid' or 'reference'")
+    rigid = 1 if subvoxel == "rigid" else 0
+    t, a = t0.clone().contiguous(), a0.clone().contiguous()
+    t0 = t0.contiguous()
+    m1 = torch.zeros((n, 6), dtype=torch.float64, device=dev)
+    m2 = torch.zeros_like(m1)
+    best_s = torch.full((n,), -math.inf, dtype=torch.float64, device=dev)
+    best_t, best_a = t0.clone(), a0.clone().contiguous()
+    poses = torch.empty((bsz, 12), dtype=torch.float32, device=dev)
+    corners = torch.empty((bsz, 3), dtype=torch.int32, device=dev)
+    mj = torch.empty((bsz, 78), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    ptr = lambda x, lo: x.data_ptr() + lo * x.stride(0) * x.element_size()  # noqa: E731
+    for step in range(steps + 1):
+        update = int(step < steps)
+        for lo in range(0, n, bsz):
+            k = min(bsz, n - lo)
+            _lib.call("tm_refine_prepare", ptr(t, lo), ptr(a, lo), k, rigid, poses.data_ptr(), corners.data_ptr(),
+                      stream)
+            _lib.call("tm_flc_local_fwd_jac", local._plan, local.target.data_ptr(), local._tshape, poses.data_ptr(),
+                      corners.data_ptr(), k, mj.data_ptr(), stream)
+            _lib.call("tm_refine_adam", mj.data_ptr(), ptr(t, lo), ptr(a, lo), ptr(t0, lo), ptr(m1, lo), ptr(m2, lo),
+                      ptr(best_s, lo), ptr(best_t, lo), ptr(best_a, lo), k, rigid, update, float(lr_t), float(lr_a),
+                      0.9, 0.999, 1e-8, step + 1, -1.0 if bound is None else float(bound), EPS32, stream)
+        local.evaluations += n
+        local.gradients += n * update
+    return best_t, best_a, best_s
+
+
 def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_translation: float = 0.1,
                  lr_angle_deg: float = 1.0, bounds_translation: Optional[float] = None,
-                 batch: Optional[int] = None, subvoxel: str = "rigid") -> Tuple[np.ndarray, np.ndarray, np.ndarray, dict]:
+                 batch: Optional[int] = None, subvoxel: str = "rigid",
+                 fused: bool = True) -> Tuple[np.ndarray, np.ndarray, np.ndarray, dict]:
     """Gradient ascent (Adam) on the local FLC of every pose at once.
 
     ``translations`` (n,3) are the template-corner positions the reference's
     caller passes (peak - T/2, scripts/postprocess.py:598-606) and
     ``angles_deg`` (n,3) zyz Euler angles.  Each step evaluates the moments
-    and their gradient for all poses (one forward and one backward kernel per
-    batch of poses) and keeps the best pose seen per peak.  Returns the best
-    translations, angles (degrees), scores and evaluation counts.
+    and their gradient for all poses and keeps the best pose seen per peak.
+    ``fused`` (default, with the one-pass gradient) runs the whole step on
+    the device (``tm_refine_prepare``, ``tm_flc_local_fwd_jac``,
+    ``tm_refine_adam``); otherwise the step goes through the autograd
+    function and ``torch.optim.Adam``.  Returns the best translations,
+    angles (degrees), scores and evaluation counts.
     """
     torch = _torch()
     dev = local.device
@@ -303,6 +346,11 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
     a0 = torch.deg2rad(torch.as_tensor(np.asarray(angles_deg, dtype=np.float64), device=dev).reshape(-1, 3))
     n = t0.shape[0]
     bsz = min(batch or local.max_poses, local.max_poses)
+    if fused and local.one_pass_gradient:
+        bt, ba, bs = _refine_fused(local, t0, a0, steps, lr_translation, math.radians(lr_angle_deg),
+                                   bounds_translation, bsz, subvoxel)
+        info = {"score_evaluations_per_peak": steps + 1, "gradient_evaluations_per_peak": steps}
+        return bt.cpu().numpy(), np.rad2deg(ba.cpu().numpy()), bs.cpu().numpy(), info
     best_t, best_a = t0.clone(), a0.clone()
     best_s = torch.full((n,), -math.inf, dtype=torch.float64, device=dev)
     t = t0.clone().requires_grad_(True)

@@ -239,6 +239,37 @@ def test_gradient_refinement_vs_derivative_free():
         assert 5 * info["score_evaluations_per_peak"] <= evals[i], (info, evals)
 
 
+@pytest.mark.parametrize("subvoxel", ["rigid", "reference"])
+def test_fused_step_matches_the_autograd_step(subvoxel):
+    """The device Adam step (analytic score gradient and chain rule to the
+    translations and zyz angles) follows the autograd + torch.optim.Adam
+    trajectory: 20 steps from perturbed starts, with translation bounds."""
+    from pytme_amd.refine import LocalFLC, refine_peaks
+    from pytme_amd.synthetic import blob_template, threshold_mask
+
+    template = blob_template(16, n_blobs=10, seed=41)
+    mask = threshold_mask(template, 0.1, 2)
+    rng = np.random.default_rng(42)
+    truth = [(np.array([5.0, 6.0, 7.0]) + rng.uniform(0, 1, 3), rng.uniform(-90, 90, 3)),
+             (np.array([26.0, 8.0, 25.0]) + rng.uniform(0, 1, 3), rng.uniform(-90, 90, 3))]
+    target = _embedded_target(43, (48, 48, 48), template, truth, noise=0.2)
+    t_true = np.array([p[0] for p in truth])
+    a_true = np.array([p[1] for p in truth])
+    t0 = np.repeat(t_true, 4, axis=0) + rng.normal(0, 1.0, size=(8, 3))
+    a0 = np.repeat(a_true, 4, axis=0) + rng.normal(0, 3.0, size=(8, 3))
+    local = LocalFLC(target, template, mask, max_poses=8)
+    print(f"fused vs autograd refinement step ({subvoxel}), 8 poses, 20 steps", flush=True)
+    kw = dict(steps=20, lr_translation=0.1, lr_angle_deg=1.0, bounds_translation=2.0, subvoxel=subvoxel)
+    t1, a1, s1, i1 = refine_peaks(local, t0, a0, fused=True, **kw)
+    t2, a2, s2, i2 = refine_peaks(local, t0, a0, fused=False, **kw)
+    print(f"  max score difference {np.abs(s1 - s2).max():.2e}, pose {np.abs(t1 - t2).max():.2e} voxel", flush=True)
+    assert i1 == i2
+    np.testing.assert_allclose(s1, s2, atol=1e-6)
+    np.testing.assert_allclose(t1, t2, atol=1e-3)
+    np.testing.assert_allclose(a1, a2, atol=1e-2)
+    assert np.all(np.abs(t1 - t0) <= 2.0 + 1e-9)
+
+
 def test_refinement_on_the_benchmark_tomogram():
     """128 embedded copies of the 512x512x200 / 48^3 workload from perturbed
     starts: every refined score is at least its start score and the mean
