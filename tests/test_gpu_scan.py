@@ -162,6 +162,28 @@ def test_fused_matches_plain_full_size():
     assert agree > 0.9999, agree
 
 
+def test_c1_full_size_vs_fp64_oracle():
+    """The benchmark configuration itself against the FP64 oracle: the
+    512x512x200 tomogram, 48^3 template, wedge x CTF template filter and
+    continuous-wedge target filter, 3 rotations through the fused path;
+    scores within 1e-4 and identical rotation indices except at ties."""
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.synthetic import make_workload
+
+    target, template, mask, rotations, filters, spec = make_workload("flc_512x512x200_t48", n_rotations=3)
+    print("C1 full size, 3 rotations, fused vs FP64 oracle", flush=True)
+    s, off, r, _ = _run("FLC", target, template, mask, rotations, True, threshold=-1.0,
+                        template_filter=filters[0], target_filter=filters[1])
+    conv, fast, ft, shift = MatchingData._fourier_padding(target.shape, template.shape)
+    tfilt = filters[0](shape=template.shape, return_real_fourier=True)["data"]
+    gfilt = filters[1](shape=fast, return_real_fourier=True)["data"]
+    print("  oracle ...", flush=True)
+    _, _, raws, oc = orc.scan("FLC", target, template, rotations, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True, template_filter=tfilt, target_filter=gfilt)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
 @pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
 def test_kept_plan_sees_new_target_values(score):
     """The fused plan is kept across calls with the same arrays; it holds a
