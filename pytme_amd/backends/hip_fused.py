@@ -28,7 +28,7 @@ __all__ = ["run", "choose_batch", "autotune_batch", "TUNED"]
 _PLAN = {"key": None, "handle": None, "keep": None}
 
 #: measured batch choices: (device, mode, fast_shape, template_shape, filter,
-#: normalize) -> {"batch": B, "ms_per_rotation": {B: ms}}
+#: normalize, candidates) -> {"batch": B, "ms_per_rotation": {B: ms}}
 TUNED = {}
 
 
@@ -189,10 +189,10 @@ def run(backend, score: str, **kw):
     idx = analyzer.rotation_indices(state, rots)
     idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
     if auto:
-        tune_key = (t.cuda.current_device(), mode, fast_shape, tshape, filt is not None, normalize)
+        candidates = tuple(b for b in (B, B // 2, B // 4) if b >= 1)
+        tune_key = (t.cuda.current_device(), mode, fast_shape, tshape, filt is not None, normalize, candidates)
         if tune_key not in TUNED:
             _free_plan()  # the candidates' workspaces need the memory
-        candidates = [b for b in (B, B // 2, B // 4) if b >= 1]
         B = autotune_batch(backend, tune_key, create, stream, rots, idx_dev, fast_shape, candidates)
     key = (
         t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,

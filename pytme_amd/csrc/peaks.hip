@@ -176,7 +176,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s
 // consecutive voxels of a plane; its ballot is OR-ed into the linear-order
 // words (zeroed beforehand), split across a word boundary when needed.
 #define PK_ZC 64
-#define PK_FUSED_MAX 8
+#define PK_ZCOL_MAX 8
 
 template <int D>
 __global__ __launch_bounds__(256) void k_mf_zcol(const float* __restrict__ s, const float* __restrict__ mxy,
@@ -584,7 +584,7 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
   TM_TIMED("peaks_max_filter_xy", 8.0 * F, s,
            hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n0, (int)n1, (int)n2, size));
   TM_CHECK_LAUNCH();
-  if (size <= PK_FUSED_MAX) {
+  if (size <= PK_ZCOL_MAX) {
     const int64_t words = 64 * nblk, plane = n1 * n2;
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>(tm_ceil_div(words, 256), 4096)), dim3(256), 0, s,
                        mask, words);
