@@ -79,6 +79,44 @@ def _cpu_worker(rotations):
     return time.perf_counter() - t0
 
 
+_CPU_LOCAL = None
+
+
+def _cpu_local_worker(args):
+    """Oracle local-FLC evaluations at perturbed poses for `seconds`."""
+    from oracle import refine as orf
+
+    seed, seconds = args
+    target, ts, mask, x0 = _CPU_LOCAL
+    rng = np.random.default_rng(seed)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        x = x0[n % len(x0)] + rng.normal(0, [1.0, 1.0, 1.0, 3.0, 3.0, 3.0])
+        orf.flc_local_score(target, ts, mask, x)
+        n += 1
+    return n, time.perf_counter() - t0
+
+
+def cpu_refinement_rate(target, template, mask, starts, seconds=8.0):
+    """Host rate of the reference's score evaluation: the oracle's local FLC
+    of the 48^3 template (float64 numpy/scipy restatement of
+    tme/matching_optimization.py:488-548) at perturbed start poses, one
+    process per host core for `seconds`.  Runs before the GPU is touched;
+    returns (evaluations/s over all processes, processes)."""
+    import multiprocessing as mp
+
+    from oracle import refine as orf
+
+    global _CPU_LOCAL
+    cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    _CPU_LOCAL = (target, orf.prepare_template(template, mask), mask, starts)
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        out = pool.map(_cpu_local_worker, [(k, seconds) for k in range(cores)])
+    _CPU_LOCAL = None
+    return sum(n / t for n, t in out), cores
+
+
 def cpu_baseline(target, template, mask, tfilt, gfilt, rotations, score, n_rot):
     """The oracle's float32 dataflow (scipy.fft, one thread per process) over
     a bounded rotation sample, one process per host core, forked after the
@@ -181,7 +219,7 @@ def time_steps(scoring, setup, proxy, rot_dev, batch, steps, warmup, timing=Fals
     return t1 - t0, report
 
 
-def bench_refinement(target, template, mask, spec, n_peaks, steps):
+def bench_refinement(target, template, mask, spec, n_peaks, steps, cpu_rate=None):
     """Batched Adam refinement (gradient kernel) of n_peaks candidate peaks:
     the embedded copies of the tomogram, each start pose perturbed by
     N(0, 4 deg) per zyz angle and N(0, 1.5 voxel) per axis.  Returns the
@@ -216,6 +254,25 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps):
     _lib.timing_enable(False)
     kern = {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 3), "launches": v[1]} for k, v in report.items()
             if k.startswith("flc_local")}
+    cpu = None
+    if cpu_rate is not None:
+        # evaluations per peak of the reference's optimiser (basinhopping with
+        # COBYLA steps, niter 3, translation bounds +-5, scripts/postprocess.py:
+        # 588-613) on 4 start poses, through the GPU-backed score object
+        from pytme_amd.matching_optimization import FLC, optimize_match
+
+        evals = []
+        for i in range(4):
+            obj = FLC(target, template, mask)
+            optimize_match(obj, optimization_method="basinhopping", maxiter=3,
+                           bounds_translation=tuple((x - 5, x + 5) for x in t0[i]), x0=[*t0[i], *a0[i]])
+            evals.append(obj.evaluations)
+        rate, cores = cpu_rate
+        per_peak = float(np.mean(evals))
+        cpu = {"value": round(rate / per_peak, 4), "unit": "refined peaks/s", "cores": cores, "kind": "port",
+               "sample": f"oracle local-FLC evaluations (float64 numpy/scipy, 48^3) on {cores} processes for 8 s: "
+                         f"{rate:.1f} evaluations/s; divided by {per_peak:.0f} evaluations per peak, the mean of the "
+                         "reference's derivative-free optimiser (basinhopping niter 3, COBYLA) on 4 start poses"}
     return {
         "metric": "refined peaks/s (local FLC, Adam; moments and their Jacobian in one kernel pass per step)",
         "value": round(n_peaks / elapsed, 1),
@@ -230,6 +287,7 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps):
         "score_evaluations_per_peak": info["score_evaluations_per_peak"],
         "kernels": kern,
         "start_poses": "embedded copies, N(0,4 deg) per zyz angle, N(0,1.5 voxel) per axis",
+        "cpu_baseline": cpu,
     }
 
 
@@ -311,7 +369,7 @@ def main():
     target, template, mask, rotations, filters, spec = make_workload(args.workload, copies=args.copies)
     batch = args.batch or (None if args.path == "fused" else 4)
 
-    cpu = None
+    cpu = cpu_refine = None
     if not args.no_cpu_baseline:
         tfilt = gfilt = None
         if filters[0] is not None:
@@ -322,6 +380,14 @@ def main():
             gfilt = filters[1](shape=fast, return_real_fourier=True)["data"]
         cpu_rots = rotations[: args.cpu_rotations or min(16, len(os.sched_getaffinity(0)))]
         cpu = cpu_baseline(target, template, mask, tfilt, gfilt, cpu_rots, spec["score"], len(cpu_rots))
+        if not args.no_refinement and args.workload.startswith("flc_512"):
+            from scipy.spatial.transform import Rotation
+
+            T = int(spec["T"])
+            starts = np.concatenate([np.asarray(spec["centres"][:16], dtype=np.float64) - T // 2,
+                                     Rotation.from_matrix(np.asarray(spec["copy_rotations"][:16], dtype=np.float64)
+                                                          ).as_euler("zyz", degrees=True)], axis=1)
+            cpu_refine = cpu_refinement_rate(target, template, mask, starts)
 
     import torch
 
@@ -414,7 +480,8 @@ def main():
         out["peak_calling"] = bench_peaks(proxy._state[0], args.num_peaks, args.min_distance, 5,
                                           not args.no_cpu_baseline)
     if not args.no_refinement and args.workload.startswith("flc_512"):
-        out["refinement"] = bench_refinement(target, template, mask, spec, args.refine_peaks, args.refine_steps)
+        out["refinement"] = bench_refinement(target, template, mask, spec, args.refine_peaks, args.refine_steps,
+                                             cpu_refine)
     out.update(extra)
     print(json.dumps(out), flush=True)
 
