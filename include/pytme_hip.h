@@ -169,13 +169,23 @@ int tm_flc_local_fwd_jac(tm_flc_local_plan* plan, const float* target, const int
  * the moments-and-Jacobian output, tracks the best score and pose, and
  * applies one Adam step (torch's update, ascending the score) to t and the
  * angles, with adam_m / adam_v float64 [n][6] state and t kept within
- * bound of t0 when bound >= 0. */
+ * bound of t0 when bound >= 0.  The two-pass form of the same step:
+ * tm_refine_weights takes tm_flc_local_fwd's moments (float64 [n][6]),
+ * tracks the best score and pose and, when update, writes the weights
+ * dScore/d(moment) (float64 [n][6]) for tm_flc_local_bwd, whose gradient
+ * (float64 [n][12]) tm_refine_adam_grad turns into the Adam step. */
 int tm_refine_prepare(const double* t, const double* angles, int n, int rigid, float* poses, int* corners,
                       void* stream);
 int tm_refine_adam(const double* moments_jac, double* t, double* angles, const double* t0, double* adam_m,
                    double* adam_v, double* best_score, double* best_t, double* best_angles, int n, int rigid,
                    int update, double lr_t, double lr_a, double beta1, double beta2, double eps, int adam_step,
                    double bound, double eps32, void* stream);
+int tm_refine_weights(const double* moments, const double* t, const double* angles, double* best_score,
+                      double* best_t, double* best_angles, double* weights, int n, int update, double eps32,
+                      void* stream);
+int tm_refine_adam_grad(const double* grad, double* t, double* angles, const double* t0, double* adam_m,
+                        double* adam_v, int n, int rigid, double lr_t, double lr_a, double beta1, double beta2,
+                        double eps, int adam_step, double bound, void* stream);
 
 /* ---- peak extraction on the max-score volume -------------------------------
  * tm_peaks_max_filter + tm_peaks_write replace max_filter_coordinates

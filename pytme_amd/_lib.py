@@ -90,6 +90,16 @@ SIGNATURES = {
     "tm_flc_local_bwd": (
         c_int, [c_void_p, c_void_p, c_int64_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
     ),
+    "tm_refine_weights": (
+        c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_void_p]
+    ),
+    "tm_refine_adam_grad": (
+        c_int,
+        [
+            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_double,
+            c_double, c_double, c_int, c_double, c_void_p,
+        ],
+    ),
     "tm_scan_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     # peak extraction
     "tm_peaks_max_filter": (c_int, [c_void_p, c_int64_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

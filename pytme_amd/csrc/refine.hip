@@ -14,8 +14,11 @@
 // w_k = dS/d(moment k) and returns dS/d(R^T) (3x3) and dS/ds (3),
 // recomputing the interpolation weights instead of storing them.
 //
-// Layout: template and mask interleaved as float2 [T0][T1][T2] (x fastest,
-// L2-resident: 2 MB for 64^3); the target is the caller's float32
+// Layout: template and mask interleaved with their +x neighbour as float4
+// [T0][T1][T2] (g(x), m(x), g(x1), m(x1)), x1 = min(x + 1, T2 - 1), x
+// fastest: one 16-byte gather fetches both x corners of a trilinear sample,
+// so a sample takes 4 gathers instead of 8 (1.8 MB for 48^3, L2-resident);
+// the target is the caller's float32
 // [N0][N1][N2] volume in HBM.  Grid: (blocks per pose, poses); each block
 // owns a contiguous range of template voxels, every thread accumulates its
 // voxels (moments in FP64), the block reduces in LDS and writes one partial
@@ -39,7 +42,7 @@ struct tm_flc_local_plan {
   int max_poses;
   int nblk;  // blocks per pose
   float rmask;  // largest distance of a nonzero mask voxel from the centre T // 2
-  float2* gm;
+  float4* gm;
   double* partial;  // [max_poses][nblk][12]
 };
 
@@ -51,7 +54,7 @@ struct Sample {
 };
 
 template <bool GRAD>
-__device__ __forceinline__ Sample trilinear(const float2* __restrict__ gm, int T0, int T1, int T2, float qz, float qy,
+__device__ __forceinline__ Sample trilinear(const float4* __restrict__ gm, int T0, int T1, int T2, float qz, float qy,
                                             float qx) {
   Sample r;
   const bool valid = qz >= 0.f && qy >= 0.f && qx >= 0.f && qz <= (float)(T0 - 1) && qy <= (float)(T1 - 1) &&
@@ -63,11 +66,15 @@ __device__ __forceinline__ Sample trilinear(const float2* __restrict__ gm, int T
   z0 = valid ? z0 : 0;
   y0 = valid ? y0 : 0;
   x0 = valid ? x0 : 0;
-  const int z1 = min(z0 + 1, T0 - 1), y1 = min(y0 + 1, T1 - 1), x1 = min(x0 + 1, T2 - 1);
+  const int z1 = min(z0 + 1, T0 - 1), y1 = min(y0 + 1, T1 - 1);
   const int r00 = (z0 * T1 + y0) * T2, r01 = (z0 * T1 + y1) * T2, r10 = (z1 * T1 + y0) * T2,
             r11 = (z1 * T1 + y1) * T2;
-  const float2 v000 = gm[r00 + x0], v001 = gm[r00 + x1], v010 = gm[r01 + x0], v011 = gm[r01 + x1];
-  const float2 v100 = gm[r10 + x0], v101 = gm[r10 + x1], v110 = gm[r11 + x0], v111 = gm[r11 + x1];
+  // each gather holds the x0 and x1 = min(x0 + 1, T2 - 1) corners
+  const float4 a00 = gm[r00 + x0], a01 = gm[r01 + x0], a10 = gm[r10 + x0], a11 = gm[r11 + x0];
+  const float2 v000 = make_float2(a00.x, a00.y), v001 = make_float2(a00.z, a00.w);
+  const float2 v010 = make_float2(a01.x, a01.y), v011 = make_float2(a01.z, a01.w);
+  const float2 v100 = make_float2(a10.x, a10.y), v101 = make_float2(a10.z, a10.w);
+  const float2 v110 = make_float2(a11.x, a11.y), v111 = make_float2(a11.z, a11.w);
   auto lerp = [](float a, float b, float t) { return a + t * (b - a); };
   // along x
   const float g00 = lerp(v000.x, v001.x, wx), g01 = lerp(v010.x, v011.x, wx);
@@ -119,7 +126,7 @@ __device__ __forceinline__ void block_store(double* acc, double* out) {
 // moments needs no second pass over the samples.
 template <int MODE>
 __global__ void __launch_bounds__(TM_RF_THREADS)
-k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const float2* __restrict__ gm, int T0, int T1,
+k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const float4* __restrict__ gm, int T0, int T1,
             int T2, const float* __restrict__ poses, const int* __restrict__ corners,
             const double* __restrict__ weights, float rskip, int nblk, double* __restrict__ partial) {
   constexpr bool BWD = MODE == 1, JAC = MODE == 2;
@@ -138,7 +145,7 @@ k_flc_local(const float* __restrict__ target, int N0, int N1, int N2, const floa
   }
   // voxels are walked in 4x4x4 cubes, one cube per wave instruction: the
   // rotated samples of a cube fall into a compact region of the template
-  // brick, so the 8 trilinear gathers of a wave touch few cache lines (a
+  // brick, so the 4 trilinear gathers of a wave touch few cache lines (a
   // row of 64 voxels would sample a line across the brick).  Every term of
   // the moments and of their gradient carries the interpolated mask, so a
   // cube whose samples are all farther than rskip = rmask + sqrt(3) from the
@@ -293,10 +300,12 @@ __global__ void k_mask_radius(const float* __restrict__ mask, int T0, int T1, in
   }
 }
 
-__global__ void k_interleave(const float* __restrict__ a, const float* __restrict__ b, int64_t n,
-                             float2* __restrict__ out) {
+__global__ void k_interleave(const float* __restrict__ a, const float* __restrict__ b, int64_t n, int T2,
+                             float4* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = make_float2(a[i], b[i]);
+  if (i >= n) return;
+  const int64_t i1 = (int)(i % T2) == T2 - 1 ? i : i + 1;
+  out[i] = make_float4(a[i], b[i], a[i1], b[i1]);
 }
 
 int local_launch(tm_flc_local_plan* p, int mode, const float* target, const int64_t* tshape, const float* poses,
@@ -411,33 +420,18 @@ struct RefineAdam {
   int rigid, update;
 };
 
-// score and its gradient from the moments and their Jacobian (mj [n][78]),
-// best-pose tracking, then one Adam step (torch's update) ascending the
-// score in t and a; t stays within bound of t0 when bound >= 0
-__global__ void k_refine_adam(const double* __restrict__ mj, double* __restrict__ t, double* __restrict__ ang,
-                              const double* __restrict__ t0, double* __restrict__ mom1, double* __restrict__ mom2,
-                              double* __restrict__ best_s, double* __restrict__ best_t, double* __restrict__ best_a,
-                              int n, RefineAdam h) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* M = mj + 78 * i;
+// score of pose i from its six moments M; with want_w also the weights
+// w = dScore / d(n, A, B, C, D, E)
+__device__ double score_and_weights(const double* M, double eps32, bool want_w, double* w) {
   const double nn = M[0], A = M[1], Bm = M[2], C = M[3], Dd = M[4], E = M[5];
   const double mu = A / nn;
   const double var_t = fmax(Bm / nn - mu * mu, 0.0), var_f = fmax(E / nn - (Dd / nn) * (Dd / nn), 0.0);
   const double sd_t = sqrt(var_t), sd_f = sqrt(var_f);
   const double P = C - mu * Dd, Q = sd_t * sd_f * nn;
-  const bool flat = sd_f < h.eps32;
+  const bool flat = sd_f < eps32;
   const double score = flat ? 0.0 : P / Q;
-  if (score > best_s[i]) {
-    best_s[i] = score;
-    for (int k = 0; k < 3; ++k) {
-      best_t[3 * i + k] = t[3 * i + k];
-      best_a[3 * i + k] = ang[3 * i + k];
-    }
-  }
-  if (!h.update) return;
-  // w = dScore / d(n, A, B, C, D, E)
-  double w[6] = {0, 0, 0, 0, 0, 0};
+  if (!want_w) return score;
+  for (int k = 0; k < 6; ++k) w[k] = 0.0;
   if (!flat) {
     const double dPdn = Dd * A / (nn * nn), dPdA = -Dd / nn, dPdC = 1.0, dPdD = -mu;
     const double dsdt_dB = 1.0 / (2.0 * sd_t * nn), dsdt_dA = -(A / (nn * nn)) / sd_t,
@@ -455,13 +449,27 @@ __global__ void k_refine_adam(const double* __restrict__ mj, double* __restrict_
     w[4] = (dPdD - S * dQdD) / Q;
     w[5] = (-S * dQdE) / Q;
   }
-  double g[12];
-  for (int j = 0; j < 12; ++j) {
-    double v = 0.0;
-    for (int k = 0; k < 6; ++k) v += w[k] * M[6 + 12 * k + j];
-    g[j] = v;
+  return score;
+}
+
+__device__ void track_best(int i, double score, const double* t, const double* ang, double* best_s, double* best_t,
+                           double* best_a) {
+  if (score > best_s[i]) {
+    best_s[i] = score;
+    for (int k = 0; k < 3; ++k) {
+      best_t[3 * i + k] = t[3 * i + k];
+      best_a[3 * i + k] = ang[3 * i + k];
+    }
   }
-  // back to (t, a): shift = -R^T s (rigid) or s; s = t - trunc(t); R^T from a
+}
+
+// g = dScore/d(R^T) row-major (9), dScore/d(shift) (3) of pose i, back to
+// (t, a) (shift = -R^T s (rigid) or s; s = t - trunc(t); R^T from a), then
+// one Adam step on the loss -score (torch's update): m <- lerp(m, grad,
+// 1 - beta1), v <- beta2 v + (1 - beta2) grad^2, p <- p - lr / bc1 * m /
+// (sqrt(v) / sqrt(bc2) + eps); t stays within bound of t0 when bound >= 0
+__device__ void chain_and_adam(int i, const double* g, double* t, double* ang, const double* t0, double* mom1,
+                               double* mom2, const RefineAdam& h) {
   double R[9], Dr[3][9];
   zyz_and_derivs(ang[3 * i], ang[3 * i + 1], ang[3 * i + 2], R, Dr);
   double Rt[9];
@@ -489,8 +497,6 @@ __global__ void k_refine_adam(const double* __restrict__ mj, double* __restrict_
       for (int c = 0; c < 3; ++c) v += gRt[3 * r + c] * Dr[k][3 * c + r];  // dS/dR[c][r] = dS/dRt[r][c]
     ga[k] = v;
   }
-  // Adam on the loss -score: m <- lerp(m, grad, 1 - beta1), v <- beta2 v +
-  // (1 - beta2) grad^2, p <- p - lr / bc1 * m / (sqrt(v) / sqrt(bc2) + eps)
   for (int k = 0; k < 6; ++k) {
     const bool is_t = k < 3;
     const double grad = -(is_t ? gs[k] : ga[k - 3]);
@@ -504,6 +510,54 @@ __global__ void k_refine_adam(const double* __restrict__ mj, double* __restrict_
     *p -= step * *m1 / denom;
     if (is_t && h.bound >= 0.0) *p = fmin(fmax(*p, t0[3 * i + k] - h.bound), t0[3 * i + k] + h.bound);
   }
+}
+
+// one-pass form: score, its gradient from the moments and their Jacobian
+// (mj [n][78]), best-pose tracking and the Adam step
+__global__ void k_refine_adam(const double* __restrict__ mj, double* __restrict__ t, double* __restrict__ ang,
+                              const double* __restrict__ t0, double* __restrict__ mom1, double* __restrict__ mom2,
+                              double* __restrict__ best_s, double* __restrict__ best_t, double* __restrict__ best_a,
+                              int n, RefineAdam h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* M = mj + 78 * i;
+  double w[6];
+  const double score = score_and_weights(M, h.eps32, h.update, w);
+  track_best(i, score, t, ang, best_s, best_t, best_a);
+  if (!h.update) return;
+  double g[12];
+  for (int j = 0; j < 12; ++j) {
+    double v = 0.0;
+    for (int k = 0; k < 6; ++k) v += w[k] * M[6 + 12 * k + j];
+    g[j] = v;
+  }
+  chain_and_adam(i, g, t, ang, t0, mom1, mom2, h);
+}
+
+// two-pass form, first half: score from the moments (m [n][6]), best-pose
+// tracking and, when update, the weights dScore/d(moment) for the gradient
+// kernel (w [n][6])
+__global__ void k_refine_weights(const double* __restrict__ m, const double* __restrict__ t,
+                                 const double* __restrict__ ang, double* __restrict__ best_s,
+                                 double* __restrict__ best_t, double* __restrict__ best_a, double* __restrict__ w,
+                                 int n, int update, double eps32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double wi[6];
+  const double score = score_and_weights(m + 6 * i, eps32, update != 0, wi);
+  track_best(i, score, t, ang, best_s, best_t, best_a);
+  if (update)
+    for (int k = 0; k < 6; ++k) w[6 * i + k] = wi[k];
+}
+
+// two-pass form, second half: the Adam step from the gradient kernel's
+// dScore/d(R^T, shift) (g [n][12])
+__global__ void k_refine_adam_grad(const double* __restrict__ g, double* __restrict__ t, double* __restrict__ ang,
+                                   const double* __restrict__ t0, double* __restrict__ mom1,
+                                   double* __restrict__ mom2, int n, RefineAdam h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  chain_and_adam(i, g + 12 * i, t, ang, t0, mom1, mom2, h);
 }
 
 }  // namespace
@@ -526,7 +580,7 @@ int tm_flc_local_plan_create(const float* template_std, const float* mask, const
   // pass rely on it), at least one block per pose
   int64_t nb = tm_ceil_div(p->nvox, (int64_t)TM_RF_THREADS * 64);
   p->nblk = (int)(nb < 1 ? 1 : nb);
-  hipError_t e = hipMalloc((void**)&p->gm, sizeof(float2) * p->nvox);
+  hipError_t e = hipMalloc((void**)&p->gm, sizeof(float4) * p->nvox);
   if (e == hipSuccess)
     e = hipMalloc((void**)&p->partial, sizeof(double) * TM_RF_NJ * (size_t)max_poses * p->nblk);
   if (e != hipSuccess) {
@@ -537,7 +591,7 @@ int tm_flc_local_plan_create(const float* template_std, const float* mask, const
     return e == hipErrorOutOfMemory ? TM_ERR_OOM : TM_ERR_HIP;
   }
   hipLaunchKernelGGL(k_interleave, dim3((unsigned)tm_ceil_div(p->nvox, 256)), dim3(256), 0, 0, template_std, mask,
-                     p->nvox, p->gm);
+                     p->nvox, p->T[2], p->gm);
   e = hipGetLastError();
   // the mask radius (the partial buffer's first word serves as the counter)
   unsigned rbits = 0u;
@@ -593,7 +647,10 @@ int tm_flc_local_bwd(tm_flc_local_plan* p, const float* target, const int64_t* t
 }
 
 // Fused refinement step helpers (no counterpart in the reference): pose
-// preparation and the score / gradient / Adam / best-pose update.
+// preparation and the score / gradient / Adam / best-pose update, in the
+// one-pass form (tm_refine_adam after tm_flc_local_fwd_jac) and the
+// two-pass form (tm_refine_weights after tm_flc_local_fwd, then
+// tm_flc_local_bwd with those weights and tm_refine_adam_grad).
 int tm_refine_prepare(const double* t, const double* angles, int n, int rigid, float* poses, int* corners,
                       void* stream) {
   TM_REQUIRE(t && angles && poses && corners && n >= 1, "bad arguments to tm_refine_prepare");
@@ -603,13 +660,8 @@ int tm_refine_prepare(const double* t, const double* angles, int n, int rigid, f
   return TM_OK;
 }
 
-int tm_refine_adam(const double* moments_jac, double* t, double* angles, const double* t0, double* adam_m,
-                   double* adam_v, double* best_score, double* best_t, double* best_angles, int n, int rigid,
-                   int update, double lr_t, double lr_a, double beta1, double beta2, double eps, int adam_step,
-                   double bound, double eps32, void* stream) {
-  TM_REQUIRE(moments_jac && t && angles && t0 && adam_m && adam_v && best_score && best_t && best_angles && n >= 1,
-             "bad arguments to tm_refine_adam");
-  TM_REQUIRE(adam_step >= 1, "Adam steps count from 1");
+static RefineAdam adam_params(int rigid, int update, double lr_t, double lr_a, double beta1, double beta2,
+                              double eps, int adam_step, double bound, double eps32) {
   RefineAdam h;
   h.lr_t = lr_t;
   h.lr_a = lr_a;
@@ -622,8 +674,42 @@ int tm_refine_adam(const double* moments_jac, double* t, double* angles, const d
   h.eps32 = eps32;
   h.rigid = rigid;
   h.update = update;
+  return h;
+}
+
+int tm_refine_adam(const double* moments_jac, double* t, double* angles, const double* t0, double* adam_m,
+                   double* adam_v, double* best_score, double* best_t, double* best_angles, int n, int rigid,
+                   int update, double lr_t, double lr_a, double beta1, double beta2, double eps, int adam_step,
+                   double bound, double eps32, void* stream) {
+  TM_REQUIRE(moments_jac && t && angles && t0 && adam_m && adam_v && best_score && best_t && best_angles && n >= 1,
+             "bad arguments to tm_refine_adam");
+  TM_REQUIRE(adam_step >= 1, "Adam steps count from 1");
+  const RefineAdam h = adam_params(rigid, update, lr_t, lr_a, beta1, beta2, eps, adam_step, bound, eps32);
   hipLaunchKernelGGL(k_refine_adam, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
                      moments_jac, t, angles, t0, adam_m, adam_v, best_score, best_t, best_angles, n, h);
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+int tm_refine_weights(const double* moments, const double* t, const double* angles, double* best_score,
+                      double* best_t, double* best_angles, double* weights, int n, int update, double eps32,
+                      void* stream) {
+  TM_REQUIRE(moments && t && angles && best_score && best_t && best_angles && n >= 1 && (weights || !update),
+             "bad arguments to tm_refine_weights");
+  hipLaunchKernelGGL(k_refine_weights, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     moments, t, angles, best_score, best_t, best_angles, weights, n, update, eps32);
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+int tm_refine_adam_grad(const double* grad, double* t, double* angles, const double* t0, double* adam_m,
+                        double* adam_v, int n, int rigid, double lr_t, double lr_a, double beta1, double beta2,
+                        double eps, int adam_step, double bound, void* stream) {
+  TM_REQUIRE(grad && t && angles && t0 && adam_m && adam_v && n >= 1, "bad arguments to tm_refine_adam_grad");
+  TM_REQUIRE(adam_step >= 1, "Adam steps count from 1");
+  const RefineAdam h = adam_params(rigid, 1, lr_t, lr_a, beta1, beta2, eps, adam_step, bound, 0.0);
+  hipLaunchKernelGGL(k_refine_adam_grad, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     grad, t, angles, t0, adam_m, adam_v, n, h);
   TM_CHECK_LAUNCH();
   return TM_OK;
 }

@@ -73,7 +73,7 @@ class LocalFLC:
     """
 
     def __init__(self, target, template, template_mask=None, max_poses: int = 4096, device="cuda",
-                 one_pass_gradient: bool = True):
+                 one_pass_gradient: bool = False):
         torch = _torch()
         template = np.asarray(template, dtype=np.float32)
         if template.ndim != 3:
@@ -99,9 +99,11 @@ class LocalFLC:
         self._tshape = _lib.i64_array(self.target.shape)
         self.evaluations = 0  # poses scored (forward kernel)
         self.gradients = 0  # poses differentiated (backward kernel)
-        #: gradients from the one-pass moments-and-Jacobian kernel (default)
-        #: or from the backward kernel, which recomputes the interpolation
-        #: weights of every sample
+        #: gradients from the backward kernel after the moments kernel
+        #: (default: it recomputes the interpolation weights of every sample;
+        #: both kernels run at 4 waves/SIMD or more, 1.84 + 1.79 ms for 4096
+        #: poses of a 48^3 template) or from the one-pass moments-and-Jacobian
+        #: kernel (78 sums per thread, 2 waves/SIMD, 4.02 ms)
         self.one_pass_gradient = bool(one_pass_gradient)
 
     def __del__(self):
@@ -286,11 +288,14 @@ def score_poses(local: LocalFLC, translations, angles_deg, subvoxel: str = "rigi
 
 
 def _refine_fused(local, t0, a0, steps, lr_t, lr_a, bound, bsz, subvoxel):
-    """The Adam loop of :func:`refine_peaks` on the device: per batch and
-    step one pose-preparation kernel, the moments-and-Jacobian kernel and one
-    kernel for the score, its analytic gradient in (t, angles), best-pose
-    tracking and the Adam update (torch's formulas), so no torch work runs
-    between the kernels."""
+    """The Adam loop of :func:`refine_peaks` on the device, so no torch work
+    runs between the kernels.  Per batch and step: the pose-preparation
+    kernel, then either (two-pass, the default) the moments kernel, one
+    kernel for the score, best-pose tracking and the weights
+    dScore/d(moment), the gradient kernel with those weights and one kernel
+    for the chain rule to (t, angles) and the Adam update, or (one-pass) the
+    moments-and-Jacobian kernel and one kernel for the score, its gradient,
+    best-pose tracking and the Adam update (torch's formulas)."""
     torch = _torch()
     dev = local.device
     n = t0.shape[0]
@@ -305,20 +310,37 @@ def _refine_fused(local, t0, a0, steps, lr_t, lr_a, bound, bsz, subvoxel):
     best_t, best_a = t0.clone(), a0.clone().contiguous()
     poses = torch.empty((bsz, 12), dtype=torch.float32, device=dev)
     corners = torch.empty((bsz, 3), dtype=torch.int32, device=dev)
-    mj = torch.empty((bsz, 78), dtype=torch.float64, device=dev)
+    one_pass = local.one_pass_gradient
+    mj = torch.empty((bsz, 78 if one_pass else 6), dtype=torch.float64, device=dev)
+    w = torch.empty((bsz, 6), dtype=torch.float64, device=dev)
+    g = torch.empty((bsz, 12), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
     ptr = lambda x, lo: x.data_ptr() + lo * x.stride(0) * x.element_size()  # noqa: E731
+    bnd = -1.0 if bound is None else float(bound)
     for step in range(steps + 1):
         update = int(step < steps)
+        lt, la = float(lr_t), float(lr_a)
         for lo in range(0, n, bsz):
             k = min(bsz, n - lo)
             _lib.call("tm_refine_prepare", ptr(t, lo), ptr(a, lo), k, rigid, poses.data_ptr(), corners.data_ptr(),
                       stream)
-            _lib.call("tm_flc_local_fwd_jac", local._plan, local.target.data_ptr(), local._tshape, poses.data_ptr(),
+            if one_pass:
+                _lib.call("tm_flc_local_fwd_jac", local._plan, local.target.data_ptr(), local._tshape,
+                          poses.data_ptr(), corners.data_ptr(), k, mj.data_ptr(), stream)
+                _lib.call("tm_refine_adam", mj.data_ptr(), ptr(t, lo), ptr(a, lo), ptr(t0, lo), ptr(m1, lo),
+                          ptr(m2, lo), ptr(best_s, lo), ptr(best_t, lo), ptr(best_a, lo), k, rigid, update,
+                          lt, la, 0.9, 0.999, 1e-8, step + 1, bnd, EPS32, stream)
+                continue
+            _lib.call("tm_flc_local_fwd", local._plan, local.target.data_ptr(), local._tshape, poses.data_ptr(),
                       corners.data_ptr(), k, mj.data_ptr(), stream)
-            _lib.call("tm_refine_adam", mj.data_ptr(), ptr(t, lo), ptr(a, lo), ptr(t0, lo), ptr(m1, lo), ptr(m2, lo),
-                      ptr(best_s, lo), ptr(best_t, lo), ptr(best_a, lo), k, rigid, update, float(lr_t), float(lr_a),
-                      0.9, 0.999, 1e-8, step + 1, -1.0 if bound is None else float(bound), EPS32, stream)
+            _lib.call("tm_refine_weights", mj.data_ptr(), ptr(t, lo), ptr(a, lo), ptr(best_s, lo), ptr(best_t, lo),
+                      ptr(best_a, lo), w.data_ptr(), k, update, EPS32, stream)
+            if not update:
+                continue
+            _lib.call("tm_flc_local_bwd", local._plan, local.target.data_ptr(), local._tshape, poses.data_ptr(),
+                      corners.data_ptr(), k, w.data_ptr(), g.data_ptr(), stream)
+            _lib.call("tm_refine_adam_grad", g.data_ptr(), ptr(t, lo), ptr(a, lo), ptr(t0, lo), ptr(m1, lo),
+                      ptr(m2, lo), k, rigid, lt, la, 0.9, 0.999, 1e-8, step + 1, bnd, stream)
         local.evaluations += n
         local.gradients += n * update
     return best_t, best_a, best_s
@@ -334,11 +356,13 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
     caller passes (peak - T/2, scripts/postprocess.py:598-606) and
     ``angles_deg`` (n,3) zyz Euler angles.  Each step evaluates the moments
     and their gradient for all poses and keeps the best pose seen per peak.
-    ``fused`` (default, with the one-pass gradient) runs the whole step on
-    the device (``tm_refine_prepare``, ``tm_flc_local_fwd_jac``,
-    ``tm_refine_adam``); otherwise the step goes through the autograd
-    function and ``torch.optim.Adam``.  Returns the best translations,
-    angles (degrees), scores and evaluation counts.
+    ``fused`` (default) runs the whole step on the device
+    (``tm_refine_prepare``, then ``tm_flc_local_fwd``, ``tm_refine_weights``,
+    ``tm_flc_local_bwd`` and ``tm_refine_adam_grad``, or with the one-pass
+    gradient ``tm_flc_local_fwd_jac`` and ``tm_refine_adam``); otherwise the
+    step goes through the autograd function and ``torch.optim.Adam``.
+    Returns the best translations, angles (degrees), scores and evaluation
+    counts.
     """
     torch = _torch()
     dev = local.device
@@ -346,7 +370,7 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
     a0 = torch.deg2rad(torch.as_tensor(np.asarray(angles_deg, dtype=np.float64), device=dev).reshape(-1, 3))
     n = t0.shape[0]
     bsz = min(batch or local.max_poses, local.max_poses)
-    if fused and local.one_pass_gradient:
+    if fused:
         bt, ba, bs = _refine_fused(local, t0, a0, steps, lr_translation, math.radians(lr_angle_deg),
                                    bounds_translation, bsz, subvoxel)
         info = {"score_evaluations_per_peak": steps + 1, "gradient_evaluations_per_peak": steps}
