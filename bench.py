@@ -354,10 +354,11 @@ def bench_multi_template(target, spec, n_rot, steps):
 
 def bench_peaks(scores, num_peaks, min_distance, reps, with_cpu):
     """Peak calling on the scan's max-score volume: PeakCallerMaximumFilter
-    (box maximum filter + peak compaction, top-2k preselection, greedy
-    distance filter) then result(); ms per tomogram with the kernels' HIP
-    event times, and the oracle on the same volume (one host core) as the
-    CPU baseline and parity check."""
+    (box maximum filter + peak compaction, then one device call for the
+    boundary / score mask, top-2k preselection, merge with the state, sort
+    and greedy distance filter) then result(); ms per tomogram with the
+    kernels' HIP event times, and the oracle on the same volume (one host
+    core) as the CPU baseline and parity check."""
     import torch
 
     from pytme_amd import _lib

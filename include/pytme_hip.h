@@ -215,6 +215,22 @@ int tm_topk_f32(const float* values, int64_t n, int64_t k, int64_t* out, void* w
 int64_t tm_peak_filter_workspace_bytes(int64_t n);
 int tm_peak_filter(const int64_t* coords, int64_t n, int64_t min_distance, int64_t max_keep, int64_t* keep,
                    int64_t* n_keep, void* work, int64_t work_bytes, void* stream);
+/* tm_peak_caller_fold replaces the rest of PeakCaller.__call__ after
+ * call_peaks (tme/analyzer/peaks.py:327-384 with _get_peak_mask :241-290 and
+ * _update :531-583) for one score volume: boundary and score-window mask,
+ * the top-2 num_peaks preselection when there are more than num_peaks
+ * candidates, merge with the state (placeholder rows reduced to one),
+ * sort by score, distance filter, the first num_peaks rows.  Counts stay on
+ * the device (counts[0] valid candidates: 0 means the state is unchanged,
+ * counts[1] new rows, counts[2] output rows), so one call needs no host
+ * synchronisation.  State rows + new rows <= 8192. */
+int64_t tm_peak_caller_fold_workspace_bytes(int64_t n_cand, int64_t n_state, int64_t num_peaks);
+int tm_peak_caller_fold(const int64_t* cand, int64_t n_cand, const float* scores, const int64_t* shape,
+                        const int64_t* st_pos, const float* st_rot, const float* st_score, const float* st_det,
+                        int64_t n_state, const float* rotation, int64_t num_peaks, int64_t min_distance,
+                        int64_t min_boundary_distance, int has_min, float min_score, int has_max, float max_score,
+                        int64_t* out_pos, float* out_rot, float* out_score, float* out_det, int64_t* counts,
+                        void* work, int64_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,11 @@ and selection work runs on the GPU through the backend:
   compaction (``tm_peaks_max_filter``, ``tm_peaks_write``);
 * ``topk_indices`` -- radix select + sort (``tm_topk_f32``);
 * ``greedy_distance_filter`` -- the distance filter (``tm_peak_filter``),
-  with the reference's integer rule ``trunc(sqrt(d2)) <= min_distance``.
+  with the reference's integer rule ``trunc(sqrt(d2)) <= min_distance``;
+* ``fold_peaks`` -- the rest of one ``__call__`` (mask, preselection,
+  merge, sort, distance filter) in one device call
+  (``tm_peak_caller_fold``) when the state and the candidates fit it; the
+  step below is the same computation in backend primitives.
 
 Deterministic ties: where the reference's result depends on numpy's
 unstable ``argpartition`` / ``argsort`` order (equal scores), the lower index
@@ -161,6 +165,15 @@ class PeakCaller(AbstractAnalyzer):
         if peak_positions is None:
             return state
         peak_positions = be.to_backend_array(peak_positions).to(t.int64)
+        if peak_details is None and len(peak_positions):
+            # the whole step in one device call when the backend has it
+            fold = getattr(be, "fold_peaks", None)
+            new_state = None if fold is None else fold(
+                state, peak_positions, scores, rotation_matrix, self.num_peaks, self.min_distance,
+                self.min_boundary_distance, self.min_score, self.max_score,
+            )
+            if new_state is not None:
+                return new_state
         valid = self._get_peak_mask(peaks=peak_positions, scores=scores)
         if valid is None:
             return state

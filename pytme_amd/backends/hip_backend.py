@@ -861,6 +861,68 @@ class HipBackend(MatchingBackend):
                   ws.data_ptr(), nbytes, self._stream())
         return keep[: int(nkeep.item())]
 
+    #: state rows + new rows the device caller step takes
+    fold_max_rows = 8192
+
+    def fold_peaks(self, state, candidates, score_space, rotation_matrix, num_peaks: int, min_distance: int,
+                   min_boundary_distance: int, min_score=None, max_score=None):
+        """One step of the peak caller on the device (``tm_peak_caller_fold``):
+        the candidate positions (n, 3) of ``score_space`` folded into the
+        caller ``state`` exactly as ``PeakCaller.__call__`` does with its
+        mask, preselection, merge, sort and distance filter
+        (tme/analyzer/peaks.py:241-290, 327-384, 531-583).  Returns the new
+        state, or ``None`` when the step is outside what the kernel takes
+        (the caller then runs the step itself)."""
+        t = self._torch
+        if not self._gpu:
+            return None
+        s = self.to_backend_array(score_space)
+        c = self.to_backend_array(candidates)
+        if s.ndim != 3 or c.ndim != 2 or c.shape[1] != 3 or s.dtype != t.float32 or not s.is_contiguous():
+            return None
+        pos, rot, sc, det = (self.to_backend_array(x) for x in state)
+        n_cand, n_state = int(c.shape[0]), int(pos.shape[0])
+        if n_cand == 0:
+            return None
+        if (pos.ndim != 2 or pos.shape[1] != 3 or tuple(rot.shape[1:]) != (3, 3) or rot.shape[0] != n_state
+                or sc.shape != (n_state,) or det.shape != (n_state,)):
+            return None
+        kk = min(n_cand, 2 * num_peaks) if n_cand > num_peaks else n_cand
+        if n_state + kk > self.fold_max_rows:
+            return None
+        dev = s.device
+        c = c.to(t.int64).contiguous()
+        pos = pos.to(device=dev, dtype=t.int64).contiguous()
+        rot = rot.to(device=dev, dtype=t.float32).contiguous()
+        sc = sc.to(device=dev, dtype=t.float32).contiguous()
+        det = det.to(device=dev, dtype=t.float32).contiguous()
+        R = self.to_backend_array(rotation_matrix).to(device=dev, dtype=t.float32).contiguous()
+        if R.numel() != 9:
+            return None
+        nbytes = int(_lib.lib().tm_peak_caller_fold_workspace_bytes(n_cand, n_state, int(num_peaks)))
+        key = (t.cuda.current_device(), "peak_fold")
+        ws = _WORKSPACES.get(key)
+        if ws is None or ws.numel() < nbytes:
+            ws = t.empty(nbytes, dtype=t.uint8, device=dev)
+            _WORKSPACES[key] = ws
+        out_pos = t.empty((num_peaks, 3), dtype=t.int64, device=dev)
+        out_rot = t.empty((num_peaks, 3, 3), dtype=t.float32, device=dev)
+        out_sc = t.empty(num_peaks, dtype=t.float32, device=dev)
+        out_det = t.empty(num_peaks, dtype=t.float32, device=dev)
+        counts = t.empty(3, dtype=t.int64, device=dev)
+        ptr = lambda x: x.data_ptr() if x.numel() else None  # noqa: E731
+        _lib.call(
+            "tm_peak_caller_fold", c.data_ptr(), n_cand, s.data_ptr(), _lib.i64_array(s.shape), ptr(pos), ptr(rot),
+            ptr(sc), ptr(det), n_state, R.data_ptr(), int(num_peaks), int(min_distance), int(min_boundary_distance),
+            int(min_score is not None), float(0.0 if min_score is None else min_score), int(max_score is not None),
+            float(0.0 if max_score is None else max_score), out_pos.data_ptr(), out_rot.data_ptr(),
+            out_sc.data_ptr(), out_det.data_ptr(), counts.data_ptr(), ws.data_ptr(), nbytes, self._stream(),
+        )
+        n_valid, _, n_out = (int(x) for x in counts.tolist())
+        if n_valid == 0:
+            return tuple(state)
+        return out_pos[:n_out], out_rot[:n_out], out_sc[:n_out], out_det[:n_out]
+
     # ------------------------------------------------------------- fused scan
     def fused_scoring(self, **kwargs):
         """Batched fused scan hook used by the registered scores.

@@ -228,28 +228,35 @@ __global__ __launch_bounds__(64) void k_mask_count(const unsigned long long* __r
 __global__ __launch_bounds__(PK_THREADS) void k_mf_write(const unsigned long long* __restrict__ mask,
                                                          const int64_t* __restrict__ off, int64_t* __restrict__ xyz,
                                                          int n1, int n2, int64_t F) {
-  __shared__ int wsum[4];
+  // a block owns 64 words (4096 voxels), wave w the 16 words from 16 w on;
+  // every wave reads all 64 counts for its prefix, so no barrier is needed
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t plane = (int64_t)n1 * n2;
-  int64_t base = off[blockIdx.x];
-  for (int it = 0; it < PK_MF_ITEMS; ++it) {
-    const int64_t L0 = (int64_t)blockIdx.x * PK_MF_CHUNK + it * PK_THREADS;
-    const unsigned long long b = L0 < F ? mask[L0 / 64 + w] : 0ull;
-    if (lane == 0) wsum[w] = __popcll(b);
-    __syncthreads();
-    int wb = 0;
-    for (int i = 0; i < w; ++i) wb += wsum[i];
-    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    const int64_t L = L0 + threadIdx.x;
+  const int64_t nw = (F + 63) / 64;
+  const int64_t wd0 = (int64_t)blockIdx.x * 64;
+  const int c = wd0 + lane < nw ? __popcll(mask[wd0 + lane]) : 0;
+  // inclusive prefix of the 64 counts over the lanes
+  int pre = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(pre, o, 64);
+    if (lane >= o) pre += v;
+  }
+  const int64_t base = off[blockIdx.x];
+  for (int j = 0; j < 16; ++j) {
+    const int q = 16 * w + j;  // word of the block
+    const int64_t wd = wd0 + q;
+    if (wd >= nw) break;  // uniform over the wave
+    const unsigned long long b = mask[wd];
+    const int before = __shfl(pre, q, 64) - __shfl(c, q, 64);  // set bits in the block's words before q
+    const int64_t L = wd * 64 + lane;
     if (((b >> lane) & 1ull) && L < F) {
-      const int64_t o = base + wb + __popcll(b & ((1ull << lane) - 1ull));
+      const int64_t o = base + before + __popcll(b & ((1ull << lane) - 1ull));
       const int64_t z = L / plane, r = L - z * plane, y = r / n2, x = r - y * n2;
       xyz[3 * o + 0] = z;
       xyz[3 * o + 1] = y;
       xyz[3 * o + 2] = x;
     }
-    base += tot;
-    __syncthreads();
   }
 }
 
@@ -379,6 +386,61 @@ __global__ __launch_bounds__(PK_THREADS) void k_topk_write(const float* __restri
   }
 }
 
+// compare-exchange of element e with element e ^ j (j < 64, so both lie in
+// the wave's 64-element segment), ascending where (e & k) == 0
+__device__ __forceinline__ unsigned long long pk_cas(unsigned long long v, int e, int j, int k) {
+  const unsigned long long o = __shfl_xor(v, j, 64);
+  const unsigned long long mn = v < o ? v : o, mx = v < o ? o : v;
+  return (((e & j) == 0) == ((e & k) == 0)) ? mn : mx;
+}
+
+// one bitonic stage (k, j) over the block through a[] (LDS or global)
+__device__ __forceinline__ void pk_stage(unsigned long long* a, int np2, int k, int j) {
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    const int p = i ^ j;
+    if (p > i) {
+      const unsigned long long x = a[i], y = a[p];
+      const bool up = (i & k) == 0;
+      if ((x > y) == up) {
+        a[i] = y;
+        a[p] = x;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ascending bitonic sort of np2 64-bit keys (np2 a power of two) by the
+// whole block (a multiple of 64 threads), in place: the stages with a
+// stride below 64 run in registers within each wave's 64-element segment
+// (shuffles, no barrier), the others through a[]
+__device__ __forceinline__ void pk_bitonic(unsigned long long* a, int np2) {
+  if (np2 < 64) {
+    for (int k = 2; k <= np2; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) pk_stage(a, np2, k, j);
+    return;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int seg = w; seg < np2 / 64; seg += nw) {
+    const int e = seg * 64 + lane;
+    unsigned long long v = a[e];
+    for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) v = pk_cas(v, e, j, k);
+    a[e] = v;
+  }
+  __syncthreads();
+  for (int k = 128; k <= np2; k <<= 1) {
+    for (int j = k >> 1; j >= 64; j >>= 1) pk_stage(a, np2, k, j);
+    for (int seg = w; seg < np2 / 64; seg += nw) {
+      const int e = seg * 64 + lane;
+      unsigned long long v = a[e];
+      for (int j = 32; j > 0; j >>= 1) v = pk_cas(v, e, j, k);
+      a[e] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // ascending bitonic sort of np2 64-bit keys (np2 a power of two) by one
 // block; keys beyond n are the sentinel ~0
 template <bool LDS>
@@ -391,23 +453,21 @@ __global__ __launch_bounds__(1024) void k_topk_sort(unsigned long long* __restri
     else if (i >= n) a[i] = ~0ull;
   }
   __syncthreads();
-  for (int k = 2; k <= np2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < np2; i += blockDim.x) {
-        const int p = i ^ j;
-        if (p > i) {
-          const unsigned long long x = a[i], y = a[p];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) {
-            a[i] = y;
-            a[p] = x;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  pk_bitonic(a, np2);
   for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = (int64_t)(a[i] & 0xffffffffull);
+}
+
+// top-k of few values (n <= PK_SORT_LDS) in one block: every value's
+// composite (~key, index) sorted in LDS, the first k indices written; the
+// same order as the selection path (key descending, ties by lower index)
+__global__ __launch_bounds__(1024) void k_topk_small(const float* __restrict__ v, int n, int np2, int k,
+                                                     int64_t* __restrict__ out) {
+  extern __shared__ unsigned long long sk[];
+  for (int i = threadIdx.x; i < np2; i += blockDim.x)
+    sk[i] = i < n ? ((unsigned long long)(~pk_key(v[i])) << 32) | (unsigned long long)(unsigned)i : ~0ull;
+  __syncthreads();
+  pk_bitonic(sk, np2);
+  for (int i = threadIdx.x; i < k; i += blockDim.x) out[i] = (int64_t)(sk[i] & 0xffffffffull);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,49 +483,53 @@ __device__ __forceinline__ bool pk_near(const int64_t* __restrict__ a, const int
   return dz <= md && dy <= md && dx <= md && dz * dz + dy * dy + dx * dx < r2;
 }
 
-// earlier candidates within the distance of each candidate (first PK_CF_MAX
-// listed, all counted); earlier rows stream through LDS in tiles of 256
-__global__ __launch_bounds__(256) void k_pf_conflicts(const int64_t* __restrict__ xyz, int64_t n, int64_t md,
-                                                      int* __restrict__ ncf, int* __restrict__ cf) {
-  __shared__ int tz[256], ty[256], tx[256];
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool valid = i < n;
-  const int64_t r2 = (md + 1) * (md + 1);
-  const int cz = valid ? (int)xyz[3 * i + 0] : 0, cy = valid ? (int)xyz[3 * i + 1] : 0,
-            cx = valid ? (int)xyz[3 * i + 2] : 0;
-  int k = 0;
-  const int64_t last = (int64_t)blockIdx.x * 256;  // the block's first row: earlier tiles are whole
-  for (int64_t j0 = 0; j0 <= last; j0 += 256) {
-    const int64_t j = j0 + threadIdx.x;
-    if (j < n) {
-      tz[threadIdx.x] = (int)xyz[3 * j + 0];
-      ty[threadIdx.x] = (int)xyz[3 * j + 1];
-      tx[threadIdx.x] = (int)xyz[3 * j + 2];
-    }
-    __syncthreads();
-    if (valid) {
-      const int64_t jn = i - j0 < 256 ? i - j0 : 256;  // rows j0 .. i-1 of this tile
-      for (int64_t q = 0; q < jn; ++q) {
-        const int64_t dz = llabs((int64_t)cz - tz[q]), dy = llabs((int64_t)cy - ty[q]),
-                      dx = llabs((int64_t)cx - tx[q]);
-        if (dz <= md && dy <= md && dx <= md && dz * dz + dy * dy + dx * dx < r2) {
-          if (k < PK_CF_MAX) cf[i * PK_CF_MAX + k] = (int)(j0 + q);
-          ++k;
-        }
-      }
-    }
-    __syncthreads();
+// earlier candidates within the distance of each candidate: ncf[i] counts
+// them (zeroed before), cf[i][0..PK_CF_MAX) lists up to PK_CF_MAX of them in
+// any order (the resolve step reads them as a set, and scans all earlier
+// rows itself when there are more).  Block (x, y) pairs rows 256 x .. 256 x
+// + 255 with the earlier rows 64 y .. 64 y + 63 (held in LDS), so the
+// triangle of pairs spreads over many blocks
+__global__ __launch_bounds__(256) void k_pf_conflicts(const int64_t* __restrict__ xyz, int64_t n_host, int64_t md,
+                                                      int* __restrict__ ncf, int* __restrict__ cf,
+                                                      const int64_t* __restrict__ n_dev) {
+  __shared__ int tz[64], ty[64], tx[64];
+  const int64_t n = n_dev ? *n_dev : n_host;  // rows: the host count or one on the device (grid sized for the most)
+  const int64_t i0 = (int64_t)blockIdx.x * 256, j0 = (int64_t)blockIdx.y * 64;
+  if (j0 >= i0 + 255 || i0 >= n) return;  // no earlier row in this tile for any row of the block (uniform)
+  const int t = threadIdx.x;
+  if (t < 64 && j0 + t < n) {
+    tz[t] = (int)xyz[3 * (j0 + t) + 0];
+    ty[t] = (int)xyz[3 * (j0 + t) + 1];
+    tx[t] = (int)xyz[3 * (j0 + t) + 2];
   }
-  if (valid) ncf[i] = k;
+  __syncthreads();
+  const int64_t i = i0 + t;
+  if (i >= n || i <= j0) return;
+  const int cz = (int)xyz[3 * i + 0], cy = (int)xyz[3 * i + 1], cx = (int)xyz[3 * i + 2];
+  const int64_t r2 = (md + 1) * (md + 1);
+  const int jn = (int)(i - j0 < 64 ? i - j0 : 64);  // rows j0 .. i-1 of this tile
+  for (int q = 0; q < jn; ++q) {
+    const int64_t dz = llabs((int64_t)cz - tz[q]), dy = llabs((int64_t)cy - ty[q]), dx = llabs((int64_t)cx - tx[q]);
+    if (dz <= md && dy <= md && dx <= md && dz * dz + dy * dy + dx * dx < r2) {
+      const int k = atomicAdd(&ncf[i], 1);
+      if (k < PK_CF_MAX) cf[i * PK_CF_MAX + k] = (int)(j0 + q);
+    }
+  }
+}
+
+__global__ void k_zero_i32(int* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 // rounds of decisions in LDS (0 open, 1 kept, 2 dropped), then the kept rows
 // in order, at most max_keep
-__global__ __launch_bounds__(1024) void k_pf_resolve(const int64_t* __restrict__ xyz, int64_t n, int64_t md,
+__global__ __launch_bounds__(1024) void k_pf_resolve(const int64_t* __restrict__ xyz, int64_t n_host, int64_t md,
                                                      const int* __restrict__ ncf, const int* __restrict__ cf,
                                                      int64_t max_keep, int64_t* __restrict__ keep,
-                                                     int64_t* __restrict__ nkeep) {
+                                                     int64_t* __restrict__ nkeep, const int64_t* __restrict__ n_dev) {
   __shared__ unsigned char st[PK_PAR_MAX];
+  const int64_t n = n_dev ? *n_dev : n_host;
   __shared__ int wsum[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t r2 = (md + 1) * (md + 1);
@@ -558,6 +622,204 @@ __global__ __launch_bounds__(PK_THREADS) void k_peak_filter(const int64_t* __res
 // ---------------------------------------------------------------------------
 // C-ABI
 
+// ---------------------------------------------------------------------------
+// one step of the peak caller on the device: the candidates of one score
+// volume folded into the caller state (PeakCaller.__call__ with
+// _get_peak_mask and _update, tme/analyzer/peaks.py:241-290, 327-384,
+// 531-583), sized by counts that stay on the device
+
+struct PkFold {
+  int64_t n0, n1, n2, mbd;
+  int has_min, has_max;
+  float lo, hi;
+};
+
+__device__ __forceinline__ bool pk_window(float s, const PkFold& f) {
+  return s == s && (!f.has_min || s >= f.lo) && (!f.has_max || s <= f.hi);
+}
+
+__device__ __forceinline__ bool pk_inside(int64_t z, int64_t y, int64_t x, const PkFold& f) {
+  return z >= f.mbd && y >= f.mbd && x >= f.mbd && z < f.n0 - f.mbd && y < f.n1 - f.mbd && x < f.n2 - f.mbd;
+}
+
+__global__ void k_fold_init(int64_t* __restrict__ counts, int64_t* __restrict__ m) {
+  if (threadIdx.x < 3) counts[threadIdx.x] = 0;
+  if (threadIdx.x == 0) *m = 0;
+}
+
+// candidate validity (boundary distance, NaN, score window), counted into
+// counts[0]; with key, the preselection key (score where valid, else 0)
+__global__ __launch_bounds__(256) void k_fold_valid(const int64_t* __restrict__ cand, int64_t n,
+                                                    const float* __restrict__ scores, PkFold f,
+                                                    float* __restrict__ key, int64_t* __restrict__ counts) {
+  __shared__ int wsum[4];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool v = false;
+  if (i < n) {
+    const int64_t z = cand[3 * i], y = cand[3 * i + 1], x = cand[3 * i + 2];
+    const float sc = scores[(z * f.n1 + y) * f.n2 + x];
+    v = pk_inside(z, y, x, f) && pk_window(sc, f);
+    if (key) key[i] = v ? sc : 0.f;
+  }
+  const unsigned long long b = __ballot(v);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (tot) atomicAdd((unsigned long long*)counts, (unsigned long long)tot);
+  }
+}
+
+// the new rows in candidate order: all valid candidates, or (presel) the
+// preselected ones (sel, any order) that pass NaN and the score window -
+// the reference re-applies only the score limits after preselecting
+// (peaks.py:275-288); positions and scores compacted, count in counts[1]
+__global__ __launch_bounds__(1024) void k_fold_new(const int64_t* __restrict__ cand, const int64_t* __restrict__ sel,
+                                                   int L, int np, int presel, const float* __restrict__ scores,
+                                                   PkFold f, int64_t* __restrict__ new_pos,
+                                                   float* __restrict__ new_score, int64_t* __restrict__ counts) {
+  extern __shared__ unsigned long long sk[];
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int i = t; i < np; i += 1024)
+    sk[i] = i < L ? (unsigned long long)(presel ? sel[i] : (int64_t)i) : ~0ull;
+  __syncthreads();
+  if (presel) pk_bitonic(sk, np);
+  int64_t base = 0;
+  for (int b0 = 0; b0 < L; b0 += 1024) {
+    const int r = b0 + t;
+    bool keep = false;
+    int64_t z = 0, y = 0, x = 0;
+    float sc = 0.f;
+    if (r < L) {
+      const int64_t c = (int64_t)sk[r];
+      z = cand[3 * c];
+      y = cand[3 * c + 1];
+      x = cand[3 * c + 2];
+      sc = scores[(z * f.n1 + y) * f.n2 + x];
+      keep = pk_window(sc, f) && (presel || pk_inside(z, y, x, f));
+    }
+    const unsigned long long b = __ballot(keep);
+    if (lane == 0) wsum[w] = __popcll(b);
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < w) wb += wsum[q];
+      tot += wsum[q];
+    }
+    const int64_t o = base + wb + __popcll(b & ((1ull << lane) - 1ull));
+    if (keep) {
+      new_pos[3 * o] = z;
+      new_pos[3 * o + 1] = y;
+      new_pos[3 * o + 2] = x;
+      new_score[o] = sc;
+    }
+    base += tot;
+    __syncthreads();
+  }
+  if (t == 0) counts[1] = base;
+}
+
+__device__ __forceinline__ bool pk_filler(const int64_t* __restrict__ pos, const float* __restrict__ sc, int64_t c) {
+  return pos[3 * c] == -1 && pos[3 * c + 1] == -1 && pos[3 * c + 2] == -1 && sc[c] == -1.f;
+}
+
+// state rows then new rows, sorted by score (descending, ties by lower row,
+// NaN first); with dedupe every placeholder row after the first is left out
+// (it lies at distance 0 from the first and shares its fate or is dropped).
+// The live rows are compacted first, so the sort covers the next power of
+// two of their count only.  ord[r] = row of rank r, ordpos its position, *m
+// the row count
+__global__ __launch_bounds__(1024) void k_fold_sort(const int64_t* __restrict__ st_pos,
+                                                    const float* __restrict__ st_score, int64_t n_state, int dedupe,
+                                                    const int64_t* __restrict__ new_pos,
+                                                    const float* __restrict__ new_score,
+                                                    const int64_t* __restrict__ counts, int np2, int* __restrict__ ord,
+                                                    int64_t* __restrict__ ordpos, int64_t* __restrict__ m) {
+  extern __shared__ unsigned long long sk[];
+  __shared__ int first;
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t P = counts[1];
+  const int rows = (int)(n_state + P);
+  if (t == 0) first = 0x7fffffff;
+  __syncthreads();
+  if (dedupe)
+    for (int64_t c = t; c < n_state; c += 1024)
+      if (pk_filler(st_pos, st_score, c)) atomicMin(&first, (int)c);
+  __syncthreads();
+  int live = 0;
+  for (int b0 = 0; b0 < rows; b0 += 1024) {
+    const int i = b0 + t;
+    bool in = false;
+    unsigned long long v = 0ull;
+    if (i < n_state) {
+      in = !(dedupe && i != first && pk_filler(st_pos, st_score, i));
+      v = ((unsigned long long)(~pk_key(st_score[i])) << 32) | (unsigned)i;
+    } else if (i < rows) {
+      in = true;
+      v = ((unsigned long long)(~pk_key(new_score[i - n_state])) << 32) | (unsigned)i;
+    }
+    const unsigned long long b = __ballot(in);
+    if (lane == 0) wsum[w] = __popcll(b);
+    __syncthreads();
+    int wb = 0, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < w) wb += wsum[q];
+      tot += wsum[q];
+    }
+    if (in) sk[live + wb + __popcll(b & ((1ull << lane) - 1ull))] = v;
+    live += tot;
+    __syncthreads();
+  }
+  int np = 1;
+  while (np < live) np <<= 1;
+  for (int i = live + t; i < np; i += 1024) sk[i] = ~0ull;
+  __syncthreads();
+  pk_bitonic(sk, np);
+  for (int r = t; r < live; r += 1024) {
+    const int c = (int)(sk[r] & 0xffffffffull);
+    ord[r] = c;
+    const int64_t* src = c < n_state ? st_pos + 3 * c : new_pos + 3 * (c - n_state);
+    ordpos[3 * r] = src[0];
+    ordpos[3 * r + 1] = src[1];
+    ordpos[3 * r + 2] = src[2];
+  }
+  if (t == 0) *m = live;
+}
+
+// output rows: rank keep[r] (or r without a distance filter) of the sorted
+// rows, gathered from the state or the new rows (rotation = this volume's,
+// details -1); counts[2] = output rows
+__global__ __launch_bounds__(256) void k_fold_out(const int* __restrict__ ord, const int64_t* __restrict__ keep,
+                                                  const int64_t* __restrict__ nkeep, const int64_t* __restrict__ m,
+                                                  int64_t num_peaks, const int64_t* __restrict__ st_pos,
+                                                  const float* __restrict__ st_rot,
+                                                  const float* __restrict__ st_score,
+                                                  const float* __restrict__ st_det, int64_t n_state,
+                                                  const float* __restrict__ rot, const int64_t* __restrict__ new_pos,
+                                                  const float* __restrict__ new_score, int64_t* __restrict__ out_pos,
+                                                  float* __restrict__ out_rot, float* __restrict__ out_score,
+                                                  float* __restrict__ out_det, int64_t* __restrict__ counts) {
+  const int64_t nout = keep ? *nkeep : (*m < num_peaks ? *m : num_peaks);
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r == 0) counts[2] = nout;
+  if (r >= nout) return;
+  const int c = ord[keep ? keep[r] : r];
+  if (c < n_state) {
+    for (int k = 0; k < 3; ++k) out_pos[3 * r + k] = st_pos[3 * c + k];
+    for (int k = 0; k < 9; ++k) out_rot[9 * r + k] = st_rot[9 * c + k];
+    out_score[r] = st_score[c];
+    out_det[r] = st_det[c];
+  } else {
+    const int j = c - (int)n_state;
+    for (int k = 0; k < 3; ++k) out_pos[3 * r + k] = new_pos[3 * j + k];
+    for (int k = 0; k < 9; ++k) out_rot[9 * r + k] = rot[k];
+    out_score[r] = new_score[j];
+    out_det[r] = -1.f;
+  }
+}
+
 extern "C" {
 
 // Peaks of a 3-D float32 volume under a box maximum filter of edge `size`
@@ -646,6 +908,14 @@ int tm_topk_f32(const float* v, int64_t n, int64_t k, int64_t* out, void* work, 
              PK_SORT_MAX, (long long)k);
   TM_REQUIRE(work_bytes >= tm_topk_workspace_bytes(n, k), "tm_topk_f32: workspace too small");
   hipStream_t s = (hipStream_t)stream;
+  if (n <= PK_SORT_LDS) {
+    int np = 1;
+    while (np < n) np <<= 1;
+    hipLaunchKernelGGL(k_topk_small, dim3(1), dim3(1024), sizeof(unsigned long long) * np, s, v, (int)n, np, (int)k,
+                       out);
+    TM_CHECK_LAUNCH();
+    return TM_OK;
+  }
   const int64_t nb = tm_ceil_div(n, PK_TOPK_CHUNK);
   int np2 = 1;
   while (np2 < k) np2 <<= 1;
@@ -705,17 +975,147 @@ int tm_peak_filter(const int64_t* coords, int64_t n, int64_t min_distance, int64
     TM_REQUIRE(work && work_bytes >= tm_peak_filter_workspace_bytes(n), "tm_peak_filter: workspace too small");
     int* ncf = (int*)work;
     int* cf = ncf + n;
-    hipLaunchKernelGGL(k_pf_conflicts, dim3((unsigned)tm_ceil_div(n, 256)), dim3(256), 0, s, coords, n,
-                       min_distance, ncf, cf);
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)std::min<int64_t>(tm_ceil_div(n, 256), 1024)), dim3(256), 0, s, ncf,
+                       n);
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pf_conflicts, dim3((unsigned)tm_ceil_div(n, 256), (unsigned)tm_ceil_div(n, 64)), dim3(256), 0,
+                       s, coords, n, min_distance, ncf, cf, (const int64_t*)nullptr);
     TM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_pf_resolve, dim3(1), dim3(1024), 0, s, coords, n, min_distance, ncf, cf, max_keep, keep,
-                       nkeep);
+                       nkeep, (const int64_t*)nullptr);
     TM_CHECK_LAUNCH();
     return TM_OK;
   }
   TM_REQUIRE(max_keep <= PK_KEEP_MAX, "tm_peak_filter: beyond %d rows at most %d can be kept", PK_PAR_MAX,
              PK_KEEP_MAX);
   hipLaunchKernelGGL(k_peak_filter, dim3(1), dim3(PK_THREADS), 0, s, coords, n, min_distance, max_keep, keep, nkeep);
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+
+// rows of the native caller step: state rows + new rows (<= 2 num_peaks)
+#define PK_FOLD_MAX PK_SORT_LDS
+
+static int64_t pk_align(int64_t b) { return (b + 255) / 256 * 256; }
+
+// Workspace (bytes) of tm_peak_caller_fold.
+int64_t tm_peak_caller_fold_workspace_bytes(int64_t n_cand, int64_t n_state, int64_t num_peaks) {
+  if (n_cand < 1) n_cand = 1;
+  const bool presel = n_cand > num_peaks;
+  const int64_t kk = presel ? std::min<int64_t>(n_cand, 2 * num_peaks) : n_cand;
+  const int64_t M = n_state + kk;
+  int64_t b = 0;
+  if (presel) b += pk_align(4 * n_cand) + pk_align(8 * kk) + pk_align(tm_topk_workspace_bytes(n_cand, kk));
+  b += pk_align(24 * kk) + pk_align(4 * kk);              // new rows
+  b += pk_align(4 * M) + pk_align(24 * M) + pk_align(8);  // sorted rows, count
+  b += pk_align(4 * M * (1 + PK_CF_MAX)) + pk_align(8 * num_peaks) + pk_align(8);  // distance filter
+  return b;
+}
+
+// One step of PeakCaller.__call__ after call_peaks (the reference's
+// _get_peak_mask and _update, tme/analyzer/peaks.py:241-290, 531-583): the
+// n_cand candidate positions (int64 [n][3], C order) of the score volume
+// (float32, shape[3]) are checked against the boundary distance and the
+// score window, preselected to the 2 num_peaks highest when there are more
+// than num_peaks, merged with the state (n_state rows: int64 [n][3]
+// positions, float32 [n][3][3] rotations, [n] scores and details), sorted
+// by score and distance-filtered in that order (trunc(sqrt(d2)) <=
+// min_distance drops a row); the first num_peaks rows go to out_* (row
+// capacity num_peaks).  counts (device int64 [3]): valid candidates (0: the
+// state is to be left unchanged), new rows, output rows.  Requires
+// n_state + min(n_cand, 2 num_peaks when n_cand > num_peaks) <= 8192.
+int tm_peak_caller_fold(const int64_t* cand, int64_t n_cand, const float* scores, const int64_t* shape,
+                        const int64_t* st_pos, const float* st_rot, const float* st_score, const float* st_det,
+                        int64_t n_state, const float* rotation, int64_t num_peaks, int64_t min_distance,
+                        int64_t min_boundary_distance, int has_min, float min_score, int has_max, float max_score,
+                        int64_t* out_pos, float* out_rot, float* out_score, float* out_det, int64_t* counts,
+                        void* work, int64_t work_bytes, void* stream) {
+  TM_REQUIRE(cand && scores && shape && rotation && out_pos && out_rot && out_score && out_det && counts && work,
+             "tm_peak_caller_fold: null pointer");
+  TM_REQUIRE(n_cand >= 1, "tm_peak_caller_fold: needs at least one candidate");
+  TM_REQUIRE(num_peaks >= 1, "tm_peak_caller_fold: num_peaks must be positive");
+  TM_REQUIRE(n_state >= 0 && (n_state == 0 || (st_pos && st_rot && st_score && st_det)),
+             "tm_peak_caller_fold: bad state");
+  TM_REQUIRE(min_distance < (1 << 20) && min_boundary_distance >= 0, "tm_peak_caller_fold: distance out of range");
+  const bool presel = n_cand > num_peaks;
+  const int64_t kk = presel ? std::min<int64_t>(n_cand, 2 * num_peaks) : n_cand;
+  const int64_t M = n_state + kk;
+  TM_REQUIRE(M <= PK_FOLD_MAX, "tm_peak_caller_fold: %lld state rows + %lld new rows exceed %d", (long long)n_state,
+             (long long)kk, PK_FOLD_MAX);
+  TM_REQUIRE(work_bytes >= tm_peak_caller_fold_workspace_bytes(n_cand, n_state, num_peaks),
+             "tm_peak_caller_fold: workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  PkFold f;
+  f.n0 = shape[0];
+  f.n1 = shape[1];
+  f.n2 = shape[2];
+  f.mbd = min_boundary_distance;
+  f.has_min = has_min;
+  f.has_max = has_max;
+  f.lo = min_score;
+  f.hi = max_score;
+  char* w = (char*)work;
+  auto take = [&](int64_t bytes) {
+    char* p = w;
+    w += pk_align(bytes);
+    return p;
+  };
+  float* key = nullptr;
+  int64_t* sel = nullptr;
+  char* tws = nullptr;
+  const int64_t tw_bytes = presel ? tm_topk_workspace_bytes(n_cand, kk) : 0;
+  if (presel) {
+    key = (float*)take(4 * n_cand);
+    sel = (int64_t*)take(8 * kk);
+    tws = take(tw_bytes);
+  }
+  int64_t* new_pos = (int64_t*)take(24 * kk);
+  float* new_score = (float*)take(4 * kk);
+  int* ord = (int*)take(4 * M);
+  int64_t* ordpos = (int64_t*)take(24 * M);
+  int64_t* m = (int64_t*)take(8);
+  int* ncf = (int*)take(4 * M * (1 + PK_CF_MAX));
+  int* cf = ncf + M;
+  int64_t* keep = (int64_t*)take(8 * num_peaks);
+  int64_t* nkeep = (int64_t*)take(8);
+
+  hipLaunchKernelGGL(k_fold_init, dim3(1), dim3(64), 0, s, counts, m);
+  TM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_fold_valid, dim3((unsigned)tm_ceil_div(n_cand, 256)), dim3(256), 0, s, cand, n_cand, scores, f,
+                     key, counts);
+  TM_CHECK_LAUNCH();
+  if (presel) {
+    const int rc = tm_topk_f32(key, n_cand, kk, sel, tws, tw_bytes, stream);
+    if (rc != TM_OK) return rc;
+  }
+  int np = 1;
+  while (np < kk) np <<= 1;
+  hipLaunchKernelGGL(k_fold_new, dim3(1), dim3(1024), sizeof(unsigned long long) * np, s, cand, sel, (int)kk, np,
+                     presel ? 1 : 0, scores, f, new_pos, new_score, counts);
+  TM_CHECK_LAUNCH();
+  int np2 = 1;
+  while (np2 < M) np2 <<= 1;
+  const int dedupe = min_distance > 0 && n_state > 1;
+  TM_TIMED("peaks_fold_sort", 40.0 * M, s,
+           hipLaunchKernelGGL(k_fold_sort, dim3(1), dim3(1024), sizeof(unsigned long long) * np2, s, st_pos, st_score,
+                              n_state, dedupe, new_pos, new_score, counts, np2, ord, ordpos, m));
+  TM_CHECK_LAUNCH();
+  const bool filter = min_distance > 0;
+  if (filter) {
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)tm_ceil_div(M, 256)), dim3(256), 0, s, ncf, M);
+    TM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pf_conflicts, dim3((unsigned)tm_ceil_div(M, 256), (unsigned)tm_ceil_div(M, 64)), dim3(256), 0,
+                       s, ordpos, M, min_distance, ncf, cf, (const int64_t*)m);
+    TM_CHECK_LAUNCH();
+    TM_TIMED("peaks_fold_filter", 60.0 * M, s,
+             hipLaunchKernelGGL(k_pf_resolve, dim3(1), dim3(1024), 0, s, ordpos, M, min_distance, ncf, cf, num_peaks,
+                                keep, nkeep, (const int64_t*)m));
+    TM_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_fold_out, dim3((unsigned)tm_ceil_div(num_peaks, 256)), dim3(256), 0, s, ord,
+                     filter ? keep : (const int64_t*)nullptr, nkeep, m, num_peaks, st_pos, st_rot, st_score, st_det,
+                     n_state, rotation, new_pos, new_score, out_pos, out_rot, out_score, out_det, counts);
   TM_CHECK_LAUNCH();
   return TM_OK;
 }

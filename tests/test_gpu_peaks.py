@@ -121,9 +121,18 @@ CALLER_CASES = [
 ]
 
 
+@pytest.fixture(params=["device_fold", "primitives"])
+def step_path(request, be, monkeypatch):
+    """Each caller step through tm_peak_caller_fold, or (fold disabled) the
+    primitive-by-primitive flow."""
+    if request.param == "primitives":
+        monkeypatch.setattr(be._active(), "fold_max_rows", 0)
+    return request.param
+
+
 @pytest.mark.parametrize("kind", ["maximum_filter", "sort"])
 @pytest.mark.parametrize("case", CALLER_CASES)
-def test_peak_callers_match_oracle(be, kind, case):
+def test_peak_callers_match_oracle(be, step_path, kind, case):
     """tests/test_analyzer.py:51-73's setup (100^3 uniform, min_distance 5)
     and variations; also the merge of the result."""
     import torch
@@ -136,7 +145,7 @@ def test_peak_callers_match_oracle(be, kind, case):
     cls = PeakCallerMaximumFilter if kind == "maximum_filter" else PeakCallerSort
     kw = dict(shape=data.shape, num_peaks=num_peaks, min_distance=md, min_boundary_distance=mbd, min_score=lo,
               max_score=hi)
-    print(f"peak caller {kind} {case}", flush=True)
+    print(f"peak caller {kind} {case} {step_path}", flush=True)
     pc = cls(**kw)
     state = pc(pc.init_state(), torch.as_tensor(data).cuda(), rotation_matrix=np.eye(3))
     res = pc.result(state)
@@ -151,7 +160,48 @@ def test_peak_callers_match_oracle(be, kind, case):
     _compare_state(merged, ref.merge([want]))
 
 
-def test_two_volumes_into_one_state(be):
+EDGE_CASES = [
+    # (volume, num_peaks, min_distance, min_boundary_distance, min_score, max_score)
+    ("uniform", 100, 5, 0, 2.0, None),     # no valid candidate: the state stays as it is
+    ("negative", 50, 3, 10, None, None),   # preselection keys 0 > negative scores: boundary peaks come back
+    ("nan", 200, 4, 2, None, None),        # NaN scores never become peaks
+    ("sparse", 500, 6, 0, None, None),     # fewer candidates than num_peaks: no preselection
+    ("uniform", 2000, 1, 3, 0.3, 0.9),     # 2 num_peaks + state rows close to the device limit
+]
+
+
+@pytest.mark.parametrize("kind", ["maximum_filter", "sort"])
+@pytest.mark.parametrize("case", EDGE_CASES)
+def test_peak_caller_edge_cases(be, step_path, kind, case):
+    import torch
+
+    from pytme_amd.analyzer import PeakCallerMaximumFilter, PeakCallerSort
+
+    vol, num_peaks, md, mbd, lo, hi = case
+    rng = np.random.default_rng(77)
+    data = rng.random((60, 50, 40)).astype(np.float32)
+    if vol == "negative":
+        data = data - 1.5
+    elif vol == "nan":
+        data[rng.random(data.shape) < 0.01] = np.nan
+    elif vol == "sparse":
+        data = np.zeros((60, 50, 40), dtype=np.float32)
+        data[tuple(rng.integers(0, 40, size=(3, 60)))] = rng.random(60).astype(np.float32) + 0.5
+    cls = PeakCallerMaximumFilter if kind == "maximum_filter" else PeakCallerSort
+    kw = dict(shape=data.shape, num_peaks=num_peaks, min_distance=md, min_boundary_distance=mbd, min_score=lo,
+              max_score=hi)
+    print(f"peak caller edge case {kind} {case} {step_path}", flush=True)
+    pc = cls(**kw)
+    state = pc(pc.init_state(), torch.as_tensor(data).cuda(), rotation_matrix=np.eye(3))
+    ref = op.PeakCallerOracle(kind, **kw)
+    want_state = ref(ref.init_state(), data)
+    # the raw state too (placeholder rows included), not only the result
+    for g, w in zip(state, want_state):
+        np.testing.assert_array_equal(np.asarray(g.cpu() if hasattr(g, "cpu") else g), np.asarray(w))
+    _compare_state(pc.result(state), ref.result(want_state))
+
+
+def test_two_volumes_into_one_state(be, step_path):
     """Two score volumes (two rotations) folded into one state, then result:
     the state logic across calls."""
     import torch
@@ -164,7 +214,7 @@ def test_two_volumes_into_one_state(be):
     ra = np.eye(3, dtype=np.float32)
     rb = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]], dtype=np.float32)
     kw = dict(shape=a.shape, num_peaks=200, min_distance=4)
-    print("two volumes into one state", flush=True)
+    print(f"two volumes into one state ({step_path})", flush=True)
     pc = PeakCallerMaximumFilter(**kw)
     st = pc(pc.init_state(), torch.as_tensor(a).cuda(), rotation_matrix=ra)
     st = pc(st, torch.as_tensor(b).cuda(), rotation_matrix=rb)
