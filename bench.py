@@ -273,7 +273,8 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps, cpu_rate=None
         for i in range(4):
             obj = FLC(target, template, mask)
             optimize_match(obj, optimization_method="basinhopping", maxiter=3,
-                           bounds_translation=tuple((x - 5, x + 5) for x in t0[i]), x0=[*t0[i], *a0[i]])
+                           bounds_translation=tuple((x - 5, x + 5) for x in t0[i]), x0=[*t0[i], *a0[i]],
+                           rng=np.random.default_rng(200 + i))
             evals.append(obj.evaluations)
         rate, cores = cpu_rate
         per_peak = float(np.mean(evals))

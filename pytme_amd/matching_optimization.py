@@ -106,12 +106,15 @@ def create_score_object(score: str, **kwargs) -> object:
 
 def optimize_match(score_object: object, bounds_translation: Tuple[Tuple[float]] = None,
                    bounds_rotation: Tuple[Tuple[float]] = None, optimization_method: str = "basinhopping",
-                   maxiter: int = 50, x0: Tuple[float] = None) -> Tuple[np.ndarray, np.ndarray, float]:
+                   maxiter: int = 50, x0: Tuple[float] = None,
+                   rng=None) -> Tuple[np.ndarray, np.ndarray, float]:
     """Minimise ``score_object.score`` over (translation, zyz angles)
     (tme/matching_optimization.py:1143-1312): scipy basinhopping with COBYLA
     steps, differential evolution or minimize, with the same bounds, linear
     constraints, initial-score fallback and return values
-    ``(translation, rotation_matrix, score)``."""
+    ``(translation, rotation_matrix, score)``.  ``rng`` (not in the
+    reference, default unseeded as there) seeds the random steps of
+    basinhopping and differential evolution, for repeatable runs."""
     ndim = 3
     methods = {"differential_evolution": differential_evolution, "basinhopping": basinhopping, "minimize": minimize}
     if optimization_method not in methods:
@@ -139,10 +142,10 @@ def optimize_match(score_object: object, bounds_translation: Tuple[Tuple[float]]
         initial = initial[0]
     if optimization_method == "basinhopping":
         result = basinhopping(x0=x0, func=score_object.score, niter=maxiter,
-                              minimizer_kwargs={"method": "COBYLA", "constraints": linear_constraint})
+                              minimizer_kwargs={"method": "COBYLA", "constraints": linear_constraint}, rng=rng)
     elif optimization_method == "differential_evolution":
         result = differential_evolution(func=score_object.score, bounds=bounds, constraints=linear_constraint,
-                                        maxiter=maxiter)
+                                        maxiter=maxiter, rng=rng)
     else:
         result = minimize(x0=x0, fun=score_object.score, jac=return_gradient, bounds=bounds,
                           constraints=linear_constraint, options={"maxiter": maxiter})

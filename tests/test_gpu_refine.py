@@ -230,7 +230,8 @@ def test_gradient_refinement_vs_derivative_free():
     for i in range(2):
         obj = FLC(target, template, mask)
         tr, rot, fun = optimize_match(obj, optimization_method="basinhopping", maxiter=3,
-                                      bounds_translation=tuple((x - 5, x + 5) for x in t0[i]), x0=[*t0[i], *a0[i]])
+                                      bounds_translation=tuple((x - 5, x + 5) for x in t0[i]), x0=[*t0[i], *a0[i]],
+                                      rng=np.random.default_rng(100 + i))
         evals.append(obj.evaluations)
         s_df.append(-fun)
     print(f"  derivative-free {np.round(s_df, 3)} ({evals} evaluations)", flush=True)
