@@ -23,7 +23,7 @@
 #include "tm_common.h"
 
 #define PK_THREADS 256                 // threads per compaction block (4 waves)
-#define PK_MF_TX 64                    // max-filter plane tile: 64 x 16
+#define PK_MF_TX 64                    // max-filter plane tile: up to 64 x 16
 #define PK_MF_TY 16
 #define PK_MF_TZ 8                     // z planes per block of the xy pass
 #define PK_MAX_SIZE 64                 // largest filter edge
@@ -88,16 +88,17 @@ __device__ __forceinline__ int pk_block_rank(bool flag, int* wsum, int* total) {
 // ---------------------------------------------------------------------------
 // maximum filter
 
-// x and y maxima over a 64 x 16 plane tile with a (d-1) halo; outside the
-// volume the input is the constant 0
+// x and y maxima over a tx x 16 plane tile with a (d-1) halo, tx <= 64
+// chosen so the x tiles split the rows evenly; outside the volume the input
+// is the constant 0
 __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, float* __restrict__ out, int n0, int n1,
-                                               int n2, int d) {
+                                               int n2, int d, int tx) {
   extern __shared__ float sm[];
   const int lo = d / 2;
-  const int hx = PK_MF_TX + d - 1, hy = PK_MF_TY + d - 1;
+  const int hx = tx + d - 1, hy = PK_MF_TY + d - 1;
   float* in = sm;             // hy x hx
-  float* mx = sm + hy * hx;   // hy x 64
-  const int x0 = blockIdx.x * PK_MF_TX, y0 = blockIdx.y * PK_MF_TY;
+  float* mx = sm + hy * hx;   // hy x tx
+  const int x0 = blockIdx.x * tx, y0 = blockIdx.y * PK_MF_TY;
   const int zb = blockIdx.z * PK_MF_TZ, ze = min(zb + PK_MF_TZ, n0);
   for (int64_t z = zb; z < ze; ++z) {
   __syncthreads();  // the previous plane's tiles are consumed
@@ -108,8 +109,8 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
     in[i] = (y >= 0 && y < n1 && x >= 0 && x < n2) ? plane[(int64_t)y * n2 + x] : 0.f;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < hy * PK_MF_TX; i += blockDim.x) {
-    const int r = i / PK_MF_TX, c = i - r * PK_MF_TX;
+  for (int i = threadIdx.x; i < hy * tx; i += blockDim.x) {
+    const int r = i / tx, c = i - r * tx;
     const float* row = in + r * hx + c;
     float m = row[0];
     for (int k = 1; k < d; ++k) m = fmaxf(m, row[k]);
@@ -117,13 +118,13 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
   }
   __syncthreads();
   float* oplane = out + z * (int64_t)n1 * n2;
-  for (int i = threadIdx.x; i < PK_MF_TY * PK_MF_TX; i += blockDim.x) {
-    const int r = i / PK_MF_TX, c = i - r * PK_MF_TX;
+  for (int i = threadIdx.x; i < PK_MF_TY * tx; i += blockDim.x) {
+    const int r = i / tx, c = i - r * tx;
     const int y = y0 + r, x = x0 + c;
     if (y >= n1 || x >= n2) continue;
-    const float* col = mx + r * PK_MF_TX + c;
+    const float* col = mx + r * tx + c;
     float m = col[0];
-    for (int k = 1; k < d; ++k) m = fmaxf(m, col[k * PK_MF_TX]);
+    for (int k = 1; k < d; ++k) m = fmaxf(m, col[k * tx]);
     oplane[(int64_t)y * n2 + x] = m;
   }
   }
@@ -840,11 +841,14 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
              "tm_peaks_max_filter: shape must be positive with z < 65536 and y, x < 2^20");
   const int64_t F = n0 * n1 * n2, nblk = tm_ceil_div(F, PK_MF_CHUNK);
   hipStream_t s = (hipStream_t)stream;
-  const int hx = PK_MF_TX + size - 1, hy = PK_MF_TY + size - 1;
-  const size_t shm = sizeof(float) * (size_t)hy * (hx + PK_MF_TX);
-  dim3 g1((unsigned)tm_ceil_div(n2, PK_MF_TX), (unsigned)tm_ceil_div(n1, PK_MF_TY), (unsigned)tm_ceil_div(n0, PK_MF_TZ));
+  // x tiles of at most 64 columns that split the rows evenly (200 -> 4 x 50)
+  const int ntx = (int)tm_ceil_div(n2, PK_MF_TX);
+  const int tx = (int)tm_ceil_div(n2, ntx);
+  const int hx = tx + size - 1, hy = PK_MF_TY + size - 1;
+  const size_t shm = sizeof(float) * (size_t)hy * (hx + tx);
+  dim3 g1((unsigned)ntx, (unsigned)tm_ceil_div(n1, PK_MF_TY), (unsigned)tm_ceil_div(n0, PK_MF_TZ));
   TM_TIMED("peaks_max_filter_xy", 8.0 * F, s,
-           hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n0, (int)n1, (int)n2, size));
+           hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n0, (int)n1, (int)n2, size, tx));
   TM_CHECK_LAUNCH();
   if (size <= PK_ZCOL_MAX) {
     const int64_t words = 64 * nblk, plane = n1 * n2;
