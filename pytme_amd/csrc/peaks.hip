@@ -26,6 +26,7 @@
 #define PK_MF_TX 64                    // max-filter plane tile: up to 64 x 16
 #define PK_MF_TY 16
 #define PK_MF_TZ 8                     // z planes per block of the xy pass
+#define PK_XY_PF 7                     // prefetched tile values per thread (tiles of edges <= 8)
 #define PK_MAX_SIZE 64                 // largest filter edge
 #define PK_SORT_LDS 8192               // bitonic sort in LDS up to this many keys
 #define PK_SORT_MAX 65536              // larger sorts run on a global buffer
@@ -100,13 +101,37 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
   float* mx = sm + hy * hx;   // hy x tx
   const int x0 = blockIdx.x * tx, y0 = blockIdx.y * PK_MF_TY;
   const int zb = blockIdx.z * PK_MF_TZ, ze = min(zb + PK_MF_TZ, n0);
+  // edges up to 8: the next plane's tile is loaded into registers while the
+  // current one is filtered (at most PK_XY_PF values per thread)
+  const bool pf = hy * hx <= PK_XY_PF * 256;
+  float pre[PK_XY_PF];
+  auto fetch = [&](int64_t z) {
+    const float* plane = s + z * (int64_t)n1 * n2;
+#pragma unroll
+    for (int k = 0; k < PK_XY_PF; ++k) {
+      const int i = threadIdx.x + k * 256;
+      const int r = i / hx, c = i - r * hx;
+      const int y = y0 - lo + r, x = x0 - lo + c;
+      pre[k] = (i < hy * hx && y >= 0 && y < n1 && x >= 0 && x < n2) ? plane[(int64_t)y * n2 + x] : 0.f;
+    }
+  };
+  if (pf) fetch(zb);
   for (int64_t z = zb; z < ze; ++z) {
   __syncthreads();  // the previous plane's tiles are consumed
-  const float* plane = s + z * (int64_t)n1 * n2;
-  for (int i = threadIdx.x; i < hy * hx; i += blockDim.x) {
-    const int r = i / hx, c = i - r * hx;
-    const int y = y0 - lo + r, x = x0 - lo + c;
-    in[i] = (y >= 0 && y < n1 && x >= 0 && x < n2) ? plane[(int64_t)y * n2 + x] : 0.f;
+  if (pf) {
+#pragma unroll
+    for (int k = 0; k < PK_XY_PF; ++k) {
+      const int i = threadIdx.x + k * 256;
+      if (i < hy * hx) in[i] = pre[k];
+    }
+    if (z + 1 < ze) fetch(z + 1);
+  } else {
+    const float* plane = s + z * (int64_t)n1 * n2;
+    for (int i = threadIdx.x; i < hy * hx; i += blockDim.x) {
+      const int r = i / hx, c = i - r * hx;
+      const int y = y0 - lo + r, x = x0 - lo + c;
+      in[i] = (y >= 0 && y < n1 && x >= 0 && x < n2) ? plane[(int64_t)y * n2 + x] : 0.f;
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < hy * tx; i += blockDim.x) {
@@ -190,17 +215,30 @@ __global__ __launch_bounds__(256) void k_mf_zcol(const float* __restrict__ s, co
   float w[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) w[k] = 0.f;
-  // window slots hold planes zi - D + 1 .. zi; planes outside the volume are 0
-  for (int zi = z0 - LO; zi <= zend - 1 + HI; ++zi) {
+  // the xy maximum of plane zi (0 outside the volume) and the score of the
+  // output plane zi - HI (needed inside the chunk only), loaded two planes
+  // ahead so a wave keeps loads in flight while it tests the current plane
+  auto ld_m = [&](int zi) { return (col && zi >= 0 && zi < n0) ? mxy[(int64_t)zi * plane + q] : 0.f; };
+  auto ld_s = [&](int zo) { return (col && zo >= z0 && zo < zend) ? s[(int64_t)zo * plane + q] : 0.f; };
+  const int zfirst = z0 - LO;
+  float pm0 = ld_m(zfirst), pm1 = ld_m(zfirst + 1);
+  float ps0 = ld_s(zfirst - HI), ps1 = ld_s(zfirst + 1 - HI);
+  // window slots hold planes zi - D + 1 .. zi
+  for (int zi = zfirst; zi <= zend - 1 + HI; ++zi) {
+    const float cm = pm0, cs = ps0;
+    pm0 = pm1;
+    ps0 = ps1;
+    pm1 = ld_m(zi + 2);
+    ps1 = ld_s(zi + 2 - HI);
 #pragma unroll
     for (int k = 0; k < D - 1; ++k) w[k] = w[k + 1];
-    w[D - 1] = (col && zi >= 0 && zi < n0) ? mxy[(int64_t)zi * plane + q] : 0.f;
+    w[D - 1] = cm;
     const int zo = zi - HI;
     if (zo < z0) continue;  // uniform
     float m = w[0];
 #pragma unroll
     for (int k = 1; k < D; ++k) m = fmaxf(m, w[k]);
-    const bool peak = col && s[(int64_t)zo * plane + q] == m;
+    const bool peak = col && cs == m;
     const unsigned long long b = __ballot(peak);
     if (lane == 0 && b) {
       const int64_t L0 = (int64_t)zo * plane + (q - lane);
