@@ -100,10 +100,10 @@ class LocalFLC:
         self.evaluations = 0  # poses scored (forward kernel)
         self.gradients = 0  # poses differentiated (backward kernel)
         #: gradients from the backward kernel after the moments kernel
-        #: (default: it recomputes the interpolation weights of every sample;
-        #: both kernels run at 4 waves/SIMD or more, 1.84 + 1.79 ms for 4096
-        #: poses of a 48^3 template) or from the one-pass moments-and-Jacobian
-        #: kernel (78 sums per thread, 2 waves/SIMD, 4.02 ms)
+        #: (default: it recomputes the interpolation weights of every
+        #: sample; 1.71 + 1.77 ms for 4096 poses of a 48^3 template) or from
+        #: the one-pass moments-and-Jacobian kernel (78 sums per thread,
+        #: 2 waves/SIMD, 4.02 ms)
         self.one_pass_gradient = bool(one_pass_gradient)
 
     def __del__(self):
