@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP32_PEAK_GFLOPS = 157300.0  # MI355X FP32 (the kernels issue non-packed FP32: half of it is their ceiling)
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the default workload (scaled by
 # a 1 GiB copy); its traffic_bytes_per_launch line feeds roofline.traffic
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48_b64.md")
@@ -300,6 +301,19 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps, cpu_rate=None
     }
 
 
+def fft_flops(score, F, rotations_per_s):
+    """Nominal FFT throughput: 2.5 F log2 F flops per real 3-D transform
+    (5 N log2 N per complex transform, the benchFFT convention) on the
+    fast shape, 5 transforms per FLC rotation (template and mask forward,
+    S1, S2 and X inverse), 2 per FLCSphericalMask / CC / LCC rotation;
+    the pruned template transforms are counted at full size."""
+    per = float((5 if score == "FLC" else 2) * 2.5 * F * np.log2(F))
+    achieved = per * rotations_per_s / 1e9
+    return {"convention": "2.5 F log2 F per real 3-D transform, unpruned", "transforms_per_rotation":
+            5 if score == "FLC" else 2, "per_rotation_gflop": round(per / 1e9, 3), "achieved": round(achieved, 1),
+            "unit": "GFLOP/s", "peak": FP32_PEAK_GFLOPS, "frac": round(achieved / FP32_PEAK_GFLOPS, 4)}
+
+
 def bench_multi_template(target, spec, n_rot, steps):
     """The multi-template schedule: the same tomogram against four templates
     of uneven size (24^3, 32^3, 48^3, 64^3 blob densities, thresholded and
@@ -533,6 +547,7 @@ def main():
             "path": args.path,
             "filters": "wedge x CTF template filter, continuous-wedge target filter" if spec["filters"] else None,
         },
+        "fft_flops": fft_flops(spec["score"], F, value),
         "whole_step_model_GBps": round(model_bytes * value / 1e9, 1),
         "whole_step_model_frac": round(model_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
