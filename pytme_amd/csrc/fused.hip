@@ -42,6 +42,7 @@ int tm_rotate_launch(const float* src, const float* srcm, const int* shape, cons
                      float* dst, float* dstm, int64_t ds0, int64_t ds1, int64_t dbatch, double* n_obs,
                      double* workspace, hipStream_t s);
 extern "C" int tm_cmul_cr(const void* a, const float* f, void* out, int64_t n, int64_t nf, void* stream);
+extern "C" int64_t tm_rotate_workspace(const int* shape, int nbatch);
 
 // --------------------------------------------------------------------------
 // batched normalize_template (tme/matching_utils.py:61-99)
@@ -847,7 +848,7 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
     if (mode == 0) TM_ALLOC(p->ft2p, sizeof(float2) * N0 * N1 * N2s);
   }
   TM_ALLOC(p->n_obs, sizeof(double) * B);
-  TM_ALLOC(p->rot_ws, sizeof(double) * (tm_ceil_div(T3, 256) * B + 8));
+  TM_ALLOC(p->rot_ws, sizeof(double) * (tm_rotate_workspace(p->T, B) + 8));
   TM_ALLOC(p->mom_ws, sizeof(double) * 2 * TM_MOM_BLOCKS * B);
   TM_ALLOC(p->ms, sizeof(double) * 2 * B);
   // the spectra may still be in flight on the caller's stream

@@ -16,6 +16,8 @@
 // The rotated mask's sum (the FLC n_obs) is reduced deterministically in
 // FP64: one partial per workgroup, summed in a fixed order by
 // k_sum_partials.
+#include <math.h>
+
 #include "tm_common.h"
 
 #define TM_ROT_THREADS 256
@@ -78,46 +80,96 @@ __device__ __forceinline__ void pose_matrix(const float* __restrict__ R, const R
   }
 }
 
+// Output bricks of 8 (x) x 8 (y) x 4 (z) voxels, one per block of 256
+// threads: the block stages the source region its brick samples (the
+// bounding box of the brick's mapped corners, one voxel of margin, clipped
+// to the template) in LDS and interpolates from there; a region above
+// TM_ROT_LDS voxels, or a corner outside it, is read from global memory.
+// The per-voxel arithmetic is the same either way.
+#define TM_ROT_BX 8
+#define TM_ROT_BY 8
+#define TM_ROT_BZ 4
+#define TM_ROT_LDS 4096
+
+__device__ __forceinline__ double map_coord(const double* M, int i, double d0, double d1, double d2) {
+  double t = 0.0;
+  t = dadd(t, dmul(d0, M[4 * i + 0]));
+  t = dadd(t, dmul(d1, M[4 * i + 1]));
+  t = dadd(t, dmul(d2, M[4 * i + 2]));
+  return dadd(t, M[4 * i + 3]);
+}
+
 __global__ void __launch_bounds__(TM_ROT_THREADS)
 k_rotate_trilinear(const float* __restrict__ src, const float* __restrict__ srcm,
                    const float* __restrict__ rots, const double* __restrict__ mats, RotPose pose, RotBox box,
                    float* __restrict__ dst, float* __restrict__ dstm, double* __restrict__ partials) {
   __shared__ double sh[TM_ROT_THREADS];
   __shared__ double Msh[12];
+  __shared__ int lo[3], ext[3];
+  __shared__ float st[TM_ROT_LDS], sm[TM_ROT_LDS];
   const int b = blockIdx.y;
+  const int nbx = (box.s2 + TM_ROT_BX - 1) / TM_ROT_BX, nby = (box.s1 + TM_ROT_BY - 1) / TM_ROT_BY;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+  const int x0 = bx * TM_ROT_BX, y0 = by * TM_ROT_BY, z0 = bz * TM_ROT_BZ;
+  const int sz[3] = {box.s0, box.s1, box.s2};
   if (threadIdx.x == 0) {
     if (rots) {
       pose_matrix(rots + 9 * b, pose, Msh);
     } else {
       for (int k = 0; k < 12; ++k) Msh[k] = mats[12 * b + k];
     }
+    const int zc[2] = {z0, min(z0 + TM_ROT_BZ, box.s0) - 1}, yc[2] = {y0, min(y0 + TM_ROT_BY, box.s1) - 1},
+              xc[2] = {x0, min(x0 + TM_ROT_BX, box.s2) - 1};
+    for (int i = 0; i < 3; ++i) {
+      double mn = INFINITY, mx = -INFINITY;
+      for (int p = 0; p < 2; ++p)
+        for (int q = 0; q < 2; ++q)
+          for (int r = 0; r < 2; ++r) {
+            const double c = map_coord(Msh, i, zc[p], yc[q], xc[r]);
+            mn = fmin(mn, c);
+            mx = fmax(mx, c);
+          }
+      // corners used: floor(c) and floor(c) + 1 for c in [mn, mx], inside [0, n-1]
+      int l = 0, h = -1;
+      if (mn == mn && mx == mx && fabs(mn) < 1e9 && fabs(mx) < 1e9) {
+        l = max((int)floor(mn) - 1, 0);
+        h = min((int)floor(mx) + 2, sz[i] - 1);
+      }
+      lo[i] = l;
+      ext[i] = h >= l ? h - l + 1 : 0;
+    }
   }
   __syncthreads();
-  const int64_t nvox = (int64_t)box.s0 * box.s1 * box.s2;
-  const int64_t v = (int64_t)blockIdx.x * TM_ROT_THREADS + threadIdx.x;
+  const int e0 = ext[0], e1 = ext[1], e2 = ext[2];
+  const int vol = e0 * e1 * e2;
+  const bool staged = vol > 0 && vol <= TM_ROT_LDS;  // uniform
+  if (staged) {
+    for (int i = threadIdx.x; i < vol; i += TM_ROT_THREADS) {
+      const int z = i / (e1 * e2), r = i - z * e1 * e2, y = r / e2, x = r - y * e2;
+      const int64_t g = ((int64_t)(lo[0] + z) * box.s1 + (lo[1] + y)) * box.s2 + (lo[2] + x);
+      st[i] = src[g];
+      if (srcm) sm[i] = srcm[g];
+    }
+  }
+  __syncthreads();
+  const int o2 = x0 + (threadIdx.x & 7), o1 = y0 + ((threadIdx.x >> 3) & 7), o0 = z0 + (threadIdx.x >> 6);
   float val = 0.f, valm = 0.f;
-  if (v < nvox) {
-    const int o2 = (int)(v % box.s2);
-    const int o1 = (int)((v / box.s2) % box.s1);
-    const int o0 = (int)(v / ((int64_t)box.s2 * box.s1));
+  if (o0 < box.s0 && o1 < box.s1 && o2 < box.s2) {
     const double d0 = o0, d1 = o1, d2 = o2;
     double c[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double t = 0.0;
-      t = dadd(t, dmul(d0, Msh[4 * i + 0]));
-      t = dadd(t, dmul(d1, Msh[4 * i + 1]));
-      t = dadd(t, dmul(d2, Msh[4 * i + 2]));
-      c[i] = dadd(t, Msh[4 * i + 3]);
-    }
-    int a0, a1, b0, b1, e0, e1;
+    for (int i = 0; i < 3; ++i) c[i] = map_coord(Msh, i, d0, d1, d2);
+    int a0, a1, b0, b1, f0, f1;
     double wa0, wa1, wb0, wb1, we0, we1;
     const bool in = axis_weights(c[0], box.s0, a0, a1, wa0, wa1) &&
                     axis_weights(c[1], box.s1, b0, b1, wb0, wb1) &&
-                    axis_weights(c[2], box.s2, e0, e1, we0, we1);
+                    axis_weights(c[2], box.s2, f0, f1, we0, we1);
     if (in) {
-      const int ia[2] = {a0, a1}, ib[2] = {b0, b1}, ie[2] = {e0, e1};
+      const int ia[2] = {a0, a1}, ib[2] = {b0, b1}, ie[2] = {f0, f1};
       const double wa[2] = {wa0, wa1}, wb[2] = {wb0, wb1}, we[2] = {we0, we1};
+      // the staged region holds every corner of this voxel (else: global)
+      const bool lds = staged && a0 >= lo[0] && a1 < lo[0] + e0 && b0 >= lo[1] && b1 < lo[1] + e1 &&
+                       f0 >= lo[2] && f1 < lo[2] + e2;
       double acc = 0.0, accm = 0.0;
 #pragma unroll
       for (int p = 0; p < 2; ++p)
@@ -125,12 +177,21 @@ k_rotate_trilinear(const float* __restrict__ src, const float* __restrict__ srcm
         for (int q = 0; q < 2; ++q)
 #pragma unroll
           for (int r = 0; r < 2; ++r) {
-            const int64_t idx = ((int64_t)ia[p] * box.s1 + ib[q]) * box.s2 + ie[r];
-            double x = (double)src[idx];
+            float sv, mv = 0.f;
+            if (lds) {
+              const int li = ((ia[p] - lo[0]) * e1 + (ib[q] - lo[1])) * e2 + (ie[r] - lo[2]);
+              sv = st[li];
+              if (srcm) mv = sm[li];
+            } else {
+              const int64_t idx = ((int64_t)ia[p] * box.s1 + ib[q]) * box.s2 + ie[r];
+              sv = src[idx];
+              if (srcm) mv = srcm[idx];
+            }
+            double x = (double)sv;
             x = dmul(dmul(dmul(x, wa[p]), wb[q]), we[r]);
             acc = dadd(acc, x);
             if (srcm) {
-              double xm = (double)srcm[idx];
+              double xm = (double)mv;
               xm = dmul(dmul(dmul(xm, wa[p]), wb[q]), we[r]);
               accm = dadd(accm, xm);
             }
@@ -146,6 +207,11 @@ k_rotate_trilinear(const float* __restrict__ src, const float* __restrict__ srcm
     const double s = block_sum_d((double)valm, sh);
     if (threadIdx.x == 0) partials[(int64_t)b * gridDim.x + blockIdx.x] = s;
   }
+}
+
+// blocks (output bricks) per rotation
+static int64_t rot_blocks(const int* shape) {
+  return tm_ceil_div(shape[2], TM_ROT_BX) * tm_ceil_div(shape[1], TM_ROT_BY) * tm_ceil_div(shape[0], TM_ROT_BZ);
 }
 
 // out[b] = sum_{i < nparts} partials[b*nparts + i], in index order.
@@ -177,7 +243,7 @@ int tm_rotate_launch(const float* src, const float* srcm, const int* shape, cons
     pose.translation[k] = 0.0;
   }
   const int64_t nvox = (int64_t)box.s0 * box.s1 * box.s2;
-  const int64_t nblk = tm_ceil_div(nvox, TM_ROT_THREADS);
+  const int64_t nblk = rot_blocks(shape);
   TM_TIMED("rotate_trilinear", (srcm ? 8.0 : 4.0) * nvox * nbatch, s,
            hipLaunchKernelGGL(k_rotate_trilinear, dim3((unsigned)nblk, nbatch), dim3(TM_ROT_THREADS), 0, s, src,
                               srcm, rots, (const double*)nullptr, pose, box, dst, dstm, n_obs ? workspace : nullptr));
@@ -194,10 +260,7 @@ extern "C" {
 
 // Number of doubles the workspace of tm_rotate_trilinear must hold when
 // n_obs is requested.
-int64_t tm_rotate_workspace(const int* shape, int nbatch) {
-  const int64_t nvox = (int64_t)shape[0] * shape[1] * shape[2];
-  return tm_ceil_div(nvox, TM_ROT_THREADS) * (int64_t)nbatch;
-}
+int64_t tm_rotate_workspace(const int* shape, int nbatch) { return rot_blocks(shape) * (int64_t)nbatch; }
 
 // Rotate src (and optionally the mask srcm) of shape[3] by nbatch poses and
 // write the shape[0..2] box at dst + b*dst_batch_stride with row strides
@@ -231,8 +294,7 @@ int tm_rotate_trilinear(const float* src, const float* srcm, const int* shape, c
     pose.center[k] = center ? center[k] : 0.0;
     pose.translation[k] = translation ? translation[k] : 0.0;
   }
-  const int64_t nvox = (int64_t)box.s0 * box.s1 * box.s2;
-  const int64_t nblk = tm_ceil_div(nvox, TM_ROT_THREADS);
+  const int64_t nblk = rot_blocks(shape);
   TM_REQUIRE(nblk <= 0x7fffffff, "template too large");
   hipLaunchKernelGGL(k_rotate_trilinear, dim3((unsigned)nblk, nbatch), dim3(TM_ROT_THREADS), 0, s, src, srcm,
                      rots_dev, mats_dev, pose, box, dst, dstm, n_obs_out ? workspace : nullptr);
