@@ -191,7 +191,7 @@ def build_scan(workload, target, template, mask, filters, rotations, path, batch
                                        and k[3] == tshape][-1]
         tuned = hip_fused.TUNED[key]
         batch = tuned["batch"]
-        be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch)
+        be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch, scan_tiles=tuned["tiles"])
     torch.cuda.synchronize()
     return scoring, setup, proxy, rot_dev, batch, tuned
 
@@ -351,6 +351,7 @@ def bench_multi_template(target, spec, n_rot, steps):
             "rotations_per_s": round(steps * batch / elapsed, 2),
             "batch": batch,
             "autotune_ms_per_rotation": None if tuned is None else tuned["ms_per_rotation"],
+            "tiles": None if tuned is None else list(tuned["tiles"]),
             "setup_s": round(setup_s, 3),
         }
         total_rot += steps * batch
@@ -544,6 +545,8 @@ def main():
             "fast_shape": list(fast),
             "rotations_per_step": batch,
             "autotune_ms_per_rotation": None if tuned is None else tuned["ms_per_rotation"],
+            "autotune_tiles": None if tuned is None else list(tuned["tiles"]),
+            "autotune_tiles_ms_per_rotation": None if tuned is None else tuned["tiles_ms_per_rotation"],
             "path": args.path,
             "filters": "wedge x CTF template filter, continuous-wedge target filter" if spec["filters"] else None,
         },

@@ -133,6 +133,12 @@ int tm_scan_batch(tm_scan_plan* plan, const float* rotations, const int* rotatio
  * the first batch of a (batch size, state arrays) configuration runs eagerly,
  * the second is captured.  Off by default; per-kernel timing forces eager. */
 int tm_scan_plan_use_graph(tm_scan_plan* plan, int enable);
+/* No counterpart in the reference: tile shapes of the batch passes, chosen
+ * by the backend's autotuner.  pass_a_columns 16 or 8 (0: 16 where the rows
+ * are padded to 16 bins), pass_c_rows 5 or 10 (0: 5; 10 only for an x length
+ * of 200); unsupported values return an error.  Results do not depend on
+ * them. */
+int tm_scan_plan_set_tiles(tm_scan_plan* plan, int pass_a_columns, int pass_c_rows);
 
 /* ---- local FLC refinement --------------------------------------------------
  * tm_flc_local_fwd replaces the score evaluation of the derivative-free

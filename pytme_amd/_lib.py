@@ -74,6 +74,7 @@ SIGNATURES = {
     "tm_scan_plan_destroy": (c_int, [c_void_p]),
     "tm_scan_plan_set_target": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_scan_plan_use_graph": (c_int, [c_void_p, c_int]),
+    "tm_scan_plan_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     # local FLC refinement
     "tm_flc_local_plan_create": (c_int, [c_void_p, c_void_p, c_int_p, c_int, POINTER(c_void_p)]),
     "tm_flc_local_plan_destroy": (c_int, [c_void_p]),

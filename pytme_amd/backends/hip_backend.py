@@ -96,6 +96,10 @@ class HipBackend(MatchingBackend):
         and keeps the fastest (``hip_fused.autotune_batch``).
     scan_graph : bool, optional
         Replay each fused batch's kernel sequence as a hipGraph (default True).
+    scan_tiles : tuple of int, optional
+        ``(pass_a_columns, pass_c_rows)`` of the fused passes
+        (``tm_scan_plan_set_tiles``); by default the kernels' own choice, or
+        with ``rotation_batch="auto"`` the autotuner's.
     """
 
     #: spline order the rotation kernel implements (reference CLI default for
@@ -112,6 +116,7 @@ class HipBackend(MatchingBackend):
         fused: bool = True,
         rotation_batch: int = None,
         scan_graph: bool = True,
+        scan_tiles=None,
         **kwargs,
     ):
         import torch
@@ -129,8 +134,9 @@ class HipBackend(MatchingBackend):
         self.fused = bool(fused)
         self.rotation_batch = rotation_batch
         self.scan_graph = bool(scan_graph)
+        self.scan_tiles = None if scan_tiles is None else tuple(int(x) for x in scan_tiles)
         self._init_kwargs = dict(device=device, fused=fused, rotation_batch=rotation_batch, scan_graph=scan_graph,
-                                 **kwargs)
+                                 scan_tiles=scan_tiles, **kwargs)
 
     # ------------------------------------------------------------------ infra
     def _require_gpu(self, what: str):

@@ -61,12 +61,33 @@ def choose_batch(fast_shape, template_shape, mode: int, with_filter: bool, free_
     return int(max(1, min(b, n_rotations)))
 
 
+#: tile shapes the autotuner times at the chosen batch: (pass A columns,
+#: pass C rows); 0 is the kernels' default, combinations a plan does not
+#: support are skipped
+TILE_CANDIDATES = ((0, 0), (8, 0), (0, 10), (8, 10))
+
+
+def _time_plan(t, handle, stream, rots, idx_dev, B, scores, rot):
+    """ms per rotation of one warm-up and two timed batches on scratch state."""
+    start, stop = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    for rep in range(3):
+        if rep == 1:
+            start.record()
+        _lib.call("tm_scan_batch", handle, rots[0].data_ptr(), idx_dev[0].data_ptr(), B, scores.data_ptr(),
+                  rot.data_ptr(), stream)
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / (2 * B)
+
+
 def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, candidates):
-    """Time each candidate batch size on scratch max/argmax state (the
-    caller's state is not touched) with the call's own rotations: one
-    warm-up batch and two timed batches per candidate, HIP events on the
-    launch stream; keep the fastest per rotation.  ``create(B)`` returns a
-    plan handle for batch B.  Results are cached in :data:`TUNED`."""
+    """Time each candidate batch size, then the tile shapes of the passes
+    (``TILE_CANDIDATES``, ``tm_scan_plan_set_tiles``) at the fastest size,
+    on scratch max/argmax state (the caller's state is not touched) with the
+    call's own rotations: one warm-up batch and two timed batches each, HIP
+    events on the launch stream; keep the fastest per rotation.  ``create(B)``
+    returns a plan handle for batch B.  Results are cached in :data:`TUNED`
+    (``batch`` and ``tiles``)."""
     t = backend._torch
     if key in TUNED:
         return TUNED[key]["batch"]
@@ -79,19 +100,30 @@ def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, can
             continue
         handle = create(B)
         try:
-            start, stop = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
-            for rep in range(3):
-                if rep == 1:
-                    start.record()
-                _lib.call("tm_scan_batch", handle, rots[0].data_ptr(), idx_dev[0].data_ptr(), B, scores.data_ptr(),
-                          rot.data_ptr(), stream)
-            stop.record()
-            stop.synchronize()
-            timings[B] = start.elapsed_time(stop) / (2 * B)
+            timings[B] = _time_plan(t, handle, stream, rots, idx_dev, B, scores, rot)
         finally:
             _lib.call("tm_scan_plan_destroy", handle)
     best = min(timings, key=timings.get) if timings else min(candidates)
-    TUNED[key] = {"batch": int(best), "ms_per_rotation": {int(b): round(v, 5) for b, v in timings.items()}}
+    tiles = {}
+    if best in timings:
+        tiles[TILE_CANDIDATES[0]] = timings[best]
+        for combo in TILE_CANDIDATES[1:]:
+            handle = create(best)
+            try:
+                try:
+                    _lib.call("tm_scan_plan_set_tiles", handle, *combo)
+                except ValueError:
+                    continue  # not a tiling of this plan
+                tiles[combo] = _time_plan(t, handle, stream, rots, idx_dev, best, scores, rot)
+            finally:
+                _lib.call("tm_scan_plan_destroy", handle)
+    best_tiles = min(tiles, key=tiles.get) if tiles else TILE_CANDIDATES[0]
+    TUNED[key] = {
+        "batch": int(best),
+        "tiles": tuple(int(x) for x in best_tiles),
+        "ms_per_rotation": {int(b): round(v, 5) for b, v in timings.items()},
+        "tiles_ms_per_rotation": {f"{a}x{c}": round(v, 5) for (a, c), v in tiles.items()},
+    }
     return int(best)
 
 
@@ -198,11 +230,18 @@ def run(backend, score: str, **kw):
         t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,
         tuple(0 if x is None else x.data_ptr() for x in keep), bool(getattr(backend, "scan_graph", True)),
     )
+    # tile shapes: the backend's, the autotuner's, or the kernels' defaults
+    tiles = getattr(backend, "scan_tiles", None)
+    if tiles is None and auto:
+        tiles = TUNED.get(tune_key, {}).get("tiles")
+    tiles = (0, 0) if tiles is None else tiles
     if _PLAN["key"] != key:
         _free_plan()
         _PLAN.update(key=key, handle=create(B), keep=keep)
         _lib.call("tm_scan_plan_use_graph", _PLAN["handle"], int(getattr(backend, "scan_graph", True)))
+        _lib.call("tm_scan_plan_set_tiles", _PLAN["handle"], *tiles)
     else:
+        _lib.call("tm_scan_plan_set_tiles", _PLAN["handle"], *tiles)
         # the plan holds a row-padded copy of the target spectra: refresh it,
         # the caller's arrays may hold new values at the same addresses
         _lib.call("tm_scan_plan_set_target", _PLAN["handle"], ptr(ft_target), ptr(ft2), stream)

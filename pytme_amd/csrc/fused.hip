@@ -705,6 +705,8 @@ struct tm_scan_plan {
   float2 *specT, *Q1, *P, *Q;
   double *n_obs, *rot_ws, *mom_ws, *ms;
   // graph replay of the batch sequence (kernels only) on a plan stream
+  int a_cols;  // pass A column tile: 16 (default where the rows allow it) or 8
+  int c_rows;  // pass C rows per block for N2 = 200: 5 (default, 128 threads) or 10 (256 threads)
   int use_graph;
   hipStream_t gs;
   hipEvent_t ev_in, ev_out;
@@ -963,6 +965,30 @@ int tm_scan_plan_use_graph(tm_scan_plan* p, int enable) {
   return TM_OK;
 }
 
+// Tile shapes of the scan's passes (chosen by the batch autotuner):
+// pass_a_columns 16 or 8 (0: 16 where the padded rows allow it), pass_c_rows
+// 5 or 10 (0: 5; other than 5 only for an x length of 200).  A change drops
+// the captured batch graph.  Results do not depend on the tiles.
+int tm_scan_plan_set_tiles(tm_scan_plan* p, int pass_a_columns, int pass_c_rows) {
+  TM_REQUIRE(p, "null plan");
+  TM_REQUIRE(pass_a_columns == 0 || pass_a_columns == 8 || pass_a_columns == 16,
+             "pass A column tiles are 8 or 16, got %d", pass_a_columns);
+  TM_REQUIRE(pass_a_columns != 16 || tm_ceil_div(p->N2c, 16) * 16 <= p->N2s,
+             "16-column pass A tiles need rows padded to 16 bins");
+  TM_REQUIRE(pass_c_rows == 0 || pass_c_rows == 5 || (pass_c_rows == 10 && p->N[2] == 200),
+             "pass C takes 5 rows per block (10 for an x length of 200), got %d", pass_c_rows);
+  if (p->a_cols != pass_a_columns || p->c_rows != pass_c_rows) {
+    p->a_cols = pass_a_columns;
+    p->c_rows = pass_c_rows;
+    if (p->gexec) {
+      TM_CHECK_HIP(hipGraphExecDestroy(p->gexec));
+      p->gexec = nullptr;
+    }
+    p->g_warm = 0;
+  }
+  return TM_OK;
+}
+
 }  // extern "C"
 
 static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch,
@@ -1027,7 +1053,7 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
                          8.0 * C_ * p->nq_out * B;
   bool done_a = true;
   // 16-bin column tiles when they fit the padded rows, else 8-bin tiles
-  const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s;
+  const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s && p->a_cols != 8;
 #define TM_PASS_A(N0_, L_, ...) \
   launch_pass_a_ct<N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s)
   switch (N0) {
@@ -1077,7 +1103,12 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     done_c = true;
     switch (N2) {
       case 128: rc = launch_pass_c_ct<64, 32, 256, 1, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
-      case 200: rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 200:
+        if (p->c_rows == 10)
+          rc = launch_pass_c_ct<100, 10, 256, 3, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        else
+          rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        break;
       case 256: rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
       default: done_c = false; break;
     }

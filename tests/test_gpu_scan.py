@@ -347,6 +347,7 @@ def test_autotuned_batch_matches_fixed_batches():
     tuned = next(iter(hip_fused.TUNED.values()))
     print(f"  tuned {tuned}", flush=True)
     assert tuned["batch"] in tuned["ms_per_rotation"] and len(tuned["ms_per_rotation"]) >= 2
+    assert len(tuned["tiles_ms_per_rotation"]) >= 1 and len(tuned["tiles"]) == 2
     for b in (5, 24):
         s_b, _, r_b, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=b)
         np.testing.assert_array_equal(s_b, s_auto)
@@ -355,6 +356,22 @@ def test_autotuned_batch_matches_fixed_batches():
                               return_raw=True)
     n_diff, err = assert_scan_parity(s_auto, r_auto, raws, oc, -1.0)
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+@pytest.mark.parametrize("tiles", [(8, 0), (0, 10), (8, 10)])
+def test_tile_shapes_give_identical_results(tiles):
+    """The tile shapes the autotuner chooses between (pass A 16- or 8-column
+    z tiles, pass C 5- or 10-row blocks) change how lines are spread over
+    threads, not the arithmetic: the scan state is bit-identical to the
+    default tiles on a 128x96x200 target (both pass A and the 200-point
+    pass C have both variants there)."""
+    target, template, mask, rots = _workload(seed=81, shape=(128, 96, 200), n_rot=8)
+    print(f"tile shapes {tiles} vs default", flush=True)
+    s0, _, r0, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=4)
+    s1, _, r1, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=4,
+                        scan_tiles=tiles)
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(r1, r0)
 
 
 def test_graph_replays_match_eager_batches():
