@@ -88,7 +88,7 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
         int l, e;
         const bool ok = PL::in_pos(tid, i, r, l, e);
         if (FULL)
-          dstv[i * PL::R_IN + r] = src[e * ld + l];
+          dstv[i * PL::R_IN + r] = tm_ld_stream(src + e * ld + l);
         else
           dstv[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * ld + l] : make_float2(0.f, 0.f);
       }
@@ -113,7 +113,10 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
       for (int r = 0; r < PL::R_OUT; ++r) {
         int l, e;
         const bool ok = PL::out_pos(tid, i, r, l, e);
-        if (FULL || (ok && l < ti)) dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
+        if (FULL)
+          tm_st_stream(dst + e * ld + l, c_scale(v[i * PL::R_OUT + r], scale));
+        else if (ok && l < ti)
+          dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
       }
 #pragma unroll
     for (int s = 0; s < EI; ++s) v[s] = nx[s];

@@ -403,7 +403,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       for (int r = 0; r < PI::R_OUT; ++r) {
         int l, e;
         PI::out_pos(tid, i, r, l, e);
-        dst[e * plane + l] = w[i * PI::R_OUT + r];
+        tm_st_stream(dst + e * plane + l, w[i * PI::R_OUT + r]);
       }
   };
 
@@ -502,7 +502,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
       int f = tid + p * NT;
       f = f < tile_n ? f : tile_n - 1;  // unpredicated: out-of-tile slots re-read a valid bin
       const int l = f / N2c, e = f - l * N2c;
-      pf[p] = src[l * ldq + e];
+      pf[p] = tm_ld_stream(src + l * ldq + e);
     }
   };
   auto step = [&](int it, int o, float2* pf) {

@@ -54,6 +54,31 @@ __device__ __forceinline__ float2 c_scale(float2 a, float s) {
 }
 __device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
 
+// ---- streaming global accesses --------------------------------------------
+// Non-temporal cache policy for data that is larger than every cache and
+// touched once per pass (the correlation spectra of the fused scan, full
+// volumes of the plain FFT passes).  A hint only: results are unchanged.
+#ifndef TM_STREAM_HINTS
+#define TM_STREAM_HINTS 1
+#endif
+typedef float tm_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 tm_ld_stream(const float2* p) {
+#if TM_STREAM_HINTS
+  const tm_f2v v = __builtin_nontemporal_load(reinterpret_cast<const tm_f2v*>(p));
+  return make_float2(v.x, v.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void tm_st_stream(float2* p, float2 a) {
+#if TM_STREAM_HINTS
+  const tm_f2v v = {a.x, a.y};
+  __builtin_nontemporal_store(v, reinterpret_cast<tm_f2v*>(p));
+#else
+  *p = a;
+#endif
+}
+
 // ---- optional per-kernel timing (HIP events around launches) -------------
 // When enabled with tm_timing_enable(1), TM_TIMED launches record a start and
 // stop event on the launch stream and remember the kernel's algorithmic HBM
