@@ -69,6 +69,7 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
          int64_t tiles_per_outer, int64_t ntiles, float scale) {
   using PL = CtPlan<N, L, NT, true, false>;
   constexpr int EI = PL::E_IN, EO = PL::E_OUT, EW = EI > EO ? EI : EO;
+  constexpr bool WIDE = L * sizeof(float2) >= 128;  // whole-line row segments: streaming hints
   extern __shared__ float2 lds[];
   __shared__ float2 tws[N];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
@@ -88,7 +89,7 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
         int l, e;
         const bool ok = PL::in_pos(tid, i, r, l, e);
         if (FULL)
-          dstv[i * PL::R_IN + r] = tm_ld_stream(src + e * ld + l);
+          dstv[i * PL::R_IN + r] = tm_ld_stream<WIDE>(src + e * ld + l);
         else
           dstv[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * ld + l] : make_float2(0.f, 0.f);
       }
@@ -114,7 +115,7 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
         int l, e;
         const bool ok = PL::out_pos(tid, i, r, l, e);
         if (FULL)
-          tm_st_stream(dst + e * ld + l, c_scale(v[i * PL::R_OUT + r], scale));
+          tm_st_stream<WIDE>(dst + e * ld + l, c_scale(v[i * PL::R_OUT + r], scale));
         else if (ok && l < ti)
           dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
       }

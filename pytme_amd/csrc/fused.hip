@@ -403,7 +403,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       for (int r = 0; r < PI::R_OUT; ++r) {
         int l, e;
         PI::out_pos(tid, i, r, l, e);
-        tm_st_stream(dst + e * plane + l, w[i * PI::R_OUT + r]);
+        tm_st_stream<L * sizeof(float2) >= 128>(dst + e * plane + l, w[i * PI::R_OUT + r]);
       }
   };
 
