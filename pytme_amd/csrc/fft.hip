@@ -94,7 +94,9 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
           dstv[i * PL::R_IN + r] = (ok && l < ti && e < lin) ? src[e * ld + l] : make_float2(0.f, 0.f);
       }
   };
-  int64_t t = blockIdx.x;
+  // the blocks of one XCD take consecutive tiles (neighbouring column tiles
+  // of the same rows share cache lines and DRAM pages in that XCD's L2)
+  int64_t t = tm_xcd_block();
   if (t >= ntiles) return;
   load(t, v);
   ct_ready<EI>(v);

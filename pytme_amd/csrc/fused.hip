@@ -304,17 +304,6 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
   }
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (block i on XCD i % 8).
-// Renumbering so that consecutive logical tiles run on one XCD keeps the
-// neighbouring column tiles of a row (which share DRAM pages) together.
-#ifndef TM_XCD_REMAP
-#define TM_XCD_REMAP 1
-#endif
-__device__ __forceinline__ int tm_xcd_block() {
-  const int n = gridDim.x, b = blockIdx.x;
-  if (!TM_XCD_REMAP || (n & 7)) return b;
-  return (b & 7) * (n >> 3) + (b >> 3);
-}
 
 // --------------------------------------------------------------------------
 // compile-time specialised pass A / pass C

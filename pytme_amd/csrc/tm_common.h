@@ -87,6 +87,18 @@ __device__ __forceinline__ void tm_st_stream(float2* p, float2 a) {
   *p = a;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block i on XCD i % 8).
+// Renumbering so that consecutive logical tiles run on one XCD keeps the
+// neighbouring column tiles of a row (which share DRAM pages) together.
+#ifndef TM_XCD_REMAP
+#define TM_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int tm_xcd_block() {
+  const int n = gridDim.x, b = blockIdx.x;
+  if (!TM_XCD_REMAP || (n & 7)) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // ---- optional per-kernel timing (HIP events around launches) -------------
 // When enabled with tm_timing_enable(1), TM_TIMED launches record a start and
 // stop event on the launch stream and remember the kernel's algorithmic HBM
