@@ -1048,7 +1048,7 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   switch (N0) {
     case 128: rc = l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128); break;
     case 256: rc = l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256); break;
-    case 512: rc = l16 ? TM_PASS_A(512, 16, 512) : TM_PASS_A(512, 8, 512); break;
+    case 512: rc = l16 ? TM_PASS_A(512, 16, 512, 1024) : TM_PASS_A(512, 8, 512); break;
     case 1024: rc = TM_PASS_A(1024, 8, 512); break;
 #undef TM_PASS_A
     default: done_a = false; break;
