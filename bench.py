@@ -330,6 +330,7 @@ def bench_multi_template(target, spec, n_rot, steps):
     lo, hi, step = spec["tilt"]
     angles = np.arange(lo, hi + step / 2, step, dtype=np.float64)
     per, total_rot, total_s = {}, 0, 0.0
+    entries = []
     for k, (T, defocus) in enumerate(((24, 20000.0), (32, 25000.0), (48, 30000.0), (64, 35000.0))):
         tmpl = blob_template(T, seed=2000 + k)
         m = threshold_mask(tmpl)
@@ -342,6 +343,7 @@ def bench_multi_template(target, spec, n_rot, steps):
                                           opening_axis=2, tilt_axis=0)
         filters = (Compose((wedge, ctf)), Compose((target_wedge,)))
         rots = uniform_rotations(n_rot, seed=2100 + k, include_identity=True)
+        entries.append((tmpl, m, filters, rots))
         t0 = time.perf_counter()
         scoring, setup, proxy, rot_dev, batch, tuned = build_scan(spec, target, tmpl, m, filters, rots, "fused",
                                                                   None)
@@ -365,6 +367,51 @@ def bench_multi_template(target, spec, n_rot, steps):
         "steps_per_template": steps,
         "note": "one fused plan per template, batches replayed as graphs; setup_s (target and filter "
                 "setup, plan creation, batch autotuning) is outside the timed region",
+        "one_graph_schedule": bench_group_schedule(target, spec, entries, steps),
+    }
+
+
+def bench_group_schedule(target, spec, entries, rounds):
+    """The same four templates as one schedule (``backend.fused_group``, the
+    engine of ``scan_templates(schedule="graph")``): all setups first, then
+    every round scans one rotation batch of each template as one replayed
+    hipGraph.  The first round runs eagerly; the following ``rounds`` are
+    timed (host wall clock between stream synchronisations)."""
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+    from pytme_amd.backends import backend as be
+    from pytme_amd.backends import hip_fused
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+
+    hip_fused._free_plan()
+    be.change_backend("hip", device="cuda", fused=True, rotation_batch="auto")
+    setup_fn, scoring = MATCHING_EXHAUSTIVE_REGISTER[spec["score"]]
+    free = be.get_available_memory()
+    jobs = []
+    t0 = time.perf_counter()
+    for tmpl, m, filters, rots in entries:
+        fast = tuple(target.shape)
+        B = hip_fused.choose_batch(fast, tmpl.shape, 0, True, 2 * free // len(entries))
+        md = MatchingData(target, tmpl, template_mask=m, rotations=rots[: (rounds + 1) * B])
+        md.template_filter, md.target_filter = filters
+        setup, dca, chunks = prepare_scan(md, setup_fn, n_jobs=1, callback_class_args={"score_threshold": -1.0},
+                                          pad_target=False, pad_template_filter=False, interpolation_order=1)
+        jobs.append((setup, chunks[0], SharedAnalyzerProxy(MaxScoreOverRotations, dca)))
+    setup_s = time.perf_counter() - t0
+    stats = {}
+    be.fused_group(scoring, jobs, stats=stats)
+    del jobs
+    hip_fused._free_plan()
+    return {
+        "value": round(stats["rotations_timed"] / stats["seconds"], 2),
+        "unit": "rotations/s",
+        "batches": stats["batches"],
+        "rounds_timed": stats["rounds_timed"],
+        "rotations_timed": stats["rotations_timed"],
+        "setup_s": round(setup_s, 3),
+        "note": "all four templates' batches of a round in one replayed hipGraph (tm_scan_group_batch); "
+                "each plan's batch fits its share of half the free HBM",
     }
 
 

@@ -139,6 +139,23 @@ int tm_scan_plan_use_graph(tm_scan_plan* plan, int enable);
  * of 200); unsupported values return an error.  Results do not depend on
  * them. */
 int tm_scan_plan_set_tiles(tm_scan_plan* plan, int pass_a_columns, int pass_c_rows);
+/* A group of scan plans (several templates of uneven size against one
+ * target, each with its own state arrays) stepped one batch per plan per
+ * call: tm_scan_group_batch runs, in plan order, one batch of every plan
+ * whose nbatch[i] > 0 (rotations[i], rotation_indices[i] into max_scores[i],
+ * max_rotations[i]; the per-plan arguments of tm_scan_batch); the first call
+ * of a configuration (batch sizes, state arrays, tile shapes) runs eagerly,
+ * the second captures all plans' kernel sequences into one hipGraph, later
+ * calls replay it on a group stream event-ordered with the caller's stream.
+ * Replaces the per-template invocations of scripts/match_template.py (one
+ * template each, tme/matching_exhaustive.py:298-465) for a multi-template
+ * schedule.  The plans stay owned by the caller and must outlive the group;
+ * a plan is not to be used through tm_scan_batch while its group runs. */
+typedef struct tm_scan_group tm_scan_group;
+int tm_scan_group_create(tm_scan_plan* const* plans, int n, tm_scan_group** group);
+int tm_scan_group_destroy(tm_scan_group* group);
+int tm_scan_group_batch(tm_scan_group* group, const float* const* rotations, const int* const* rotation_indices,
+                        const int* nbatch, float* const* max_scores, int* const* max_rotations, void* stream);
 
 /* ---- local FLC refinement --------------------------------------------------
  * tm_flc_local_fwd replaces the score evaluation of the derivative-free

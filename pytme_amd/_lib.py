@@ -75,6 +75,11 @@ SIGNATURES = {
     "tm_scan_plan_set_target": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_scan_plan_use_graph": (c_int, [c_void_p, c_int]),
     "tm_scan_plan_set_tiles": (c_int, [c_void_p, c_int, c_int]),
+    "tm_scan_group_create": (c_int, [POINTER(c_void_p), c_int, POINTER(c_void_p)]),
+    "tm_scan_group_destroy": (c_int, [c_void_p]),
+    "tm_scan_group_batch": (
+        c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), c_int_p, POINTER(c_void_p), POINTER(c_void_p), c_void_p]
+    ),
     # local FLC refinement
     "tm_flc_local_plan_create": (c_int, [c_void_p, c_void_p, c_int_p, c_int, POINTER(c_void_p)]),
     "tm_flc_local_plan_destroy": (c_int, [c_void_p]),

@@ -940,4 +940,32 @@ class HipBackend(MatchingBackend):
             return NotImplemented
         from . import hip_fused
 
+        collect = getattr(self, "_fused_collect", None)
+        if collect is not None:
+            # fused_group: record the scoring function's arguments, scan later
+            collect.append((kwargs.pop("score"), kwargs))
+            return kwargs["callback"]
         return hip_fused.run(self, **kwargs)
+
+    def fused_group(self, scoring, jobs, stats=None):
+        """Several templates' scans as one schedule (no counterpart in the
+        reference, which runs one template per scripts/match_template.py
+        call): ``jobs`` are ``(setup, rotations, callback)`` triples of the
+        registered ``scoring`` function; each round runs one rotation batch
+        of every template as one replayed hipGraph (``hip_fused.run_group``;
+        ``stats``: its batch sizes and timing).  Raises ``ValueError`` where
+        the fused scan does not apply."""
+        if not (self.fused and self._gpu):
+            raise ValueError("fused_group needs the fused scan on a GPU device")
+        from . import hip_fused
+
+        collected = []
+        self._fused_collect = collected
+        try:
+            for setup, rotations, callback in jobs:
+                scoring(rotations=rotations, callback=callback, **setup)
+        finally:
+            self._fused_collect = None
+        if len(collected) != len(jobs):
+            raise ValueError("fused_group: a scoring function did not take the fused path")
+        hip_fused.run_group(self, collected, stats)

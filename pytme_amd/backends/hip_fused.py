@@ -17,6 +17,7 @@ per-rotation loop on the same GPU kernels.
 """
 
 import ctypes
+import time
 from typing import Optional
 
 import numpy as np
@@ -139,8 +140,10 @@ def _proxy_state(callback):
     return analyzer, callback._state
 
 
-def run(backend, score: str, **kw):
-    """Fused scan of ``kw['rotations']`` into ``kw['callback']`` or NotImplemented."""
+def _job(backend, score: str, kw):
+    """Device arrays and plan arguments of one fused scan (``kw`` are the
+    scoring function's arguments), or NotImplemented when the fused path
+    does not apply."""
     t = backend._torch
     if int(kw.get("interpolation_order", 1)) != 1:
         return NotImplemented
@@ -200,6 +203,38 @@ def run(backend, score: str, **kw):
     if rotations.ndim != 3 or rotations.shape[0] == 0:
         return NotImplemented
     rots = backend.to_backend_array(rotations).to(t.float32).contiguous()
+    return dict(
+        mode=mode, fast_shape=fast_shape, tshape=tshape, templ=templ, mask=mask, filt=filt, ft_target=ft_target,
+        ft2=ft2, inv_den=inv_den, normalize=normalize, mask_sum=mask_sum, rots=rots, analyzer=analyzer, state=state,
+        max_scores=max_scores, max_rot=max_rot,
+    )
+
+
+def _ptr(x):
+    return None if x is None else x.data_ptr()
+
+
+def _create_plan(job, batch):
+    handle = ctypes.c_void_p()
+    _lib.call(
+        "tm_scan_plan_create", job["mode"], _lib.i64_array(job["fast_shape"]), _lib.i32_array(job["tshape"]),
+        _ptr(job["templ"]), _ptr(job["mask"]), _ptr(job["filt"]), _ptr(job["ft_target"]), _ptr(job["ft2"]),
+        _ptr(job["inv_den"]), job["normalize"], job["mask_sum"], batch, ctypes.byref(handle),
+    )
+    return handle.value
+
+
+def run(backend, score: str, **kw):
+    """Fused scan of ``kw['rotations']`` into ``kw['callback']`` or NotImplemented."""
+    t = backend._torch
+    job = _job(backend, score, kw)
+    if job is NotImplemented:
+        return NotImplemented
+    mode, fast_shape, tshape, rots = job["mode"], job["fast_shape"], job["tshape"], job["rots"]
+    templ, mask, filt, ft_target, ft2, inv_den = (job[k] for k in ("templ", "mask", "filt", "ft_target", "ft2",
+                                                                   "inv_den"))
+    normalize, mask_sum = job["normalize"], job["mask_sum"]
+    max_scores, max_rot = job["max_scores"], job["max_rot"]
     n = int(rots.shape[0])
 
     free = backend.get_available_memory()
@@ -207,18 +242,13 @@ def run(backend, score: str, **kw):
     B = choose_batch(fast_shape, tshape, mode, filt is not None, free, None if auto else backend.rotation_batch, n)
 
     keep = (templ, mask, filt, ft_target, ft2, inv_den)
-    ptr = lambda x: None if x is None else x.data_ptr()  # noqa: E731
+    ptr = _ptr
     stream = backend._stream()
 
     def create(batch):
-        handle = ctypes.c_void_p()
-        _lib.call(
-            "tm_scan_plan_create", mode, _lib.i64_array(fast_shape), _lib.i32_array(tshape), ptr(templ), ptr(mask),
-            ptr(filt), ptr(ft_target), ptr(ft2), ptr(inv_den), normalize, mask_sum, batch, ctypes.byref(handle),
-        )
-        return handle.value
+        return _create_plan(job, batch)
 
-    idx = analyzer.rotation_indices(state, rots)
+    idx = job["analyzer"].rotation_indices(job["state"], rots)
     idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
     if auto:
         candidates = tuple(b for b in (B, B // 2, B // 4) if b >= 1)
@@ -253,3 +283,72 @@ def run(backend, score: str, **kw):
             max_scores.data_ptr(), max_rot.data_ptr(), stream,
         )
     return kw["callback"]
+
+
+def run_group(backend, kws, stats=None):
+    """Several fused scans (``kws``: one scoring function's arguments per
+    template, each with its own analyzer) stepped together: every round runs
+    one batch of each template as one replayed hipGraph
+    (``tm_scan_group_batch``).  Every template gets a plan whose batch is the
+    largest of 64, 32, ... fitting its share of half the free HBM (or the
+    backend's fixed ``rotation_batch``) and the backend's or the default tile
+    shapes.  Results equal the templates' separate scans.  ``stats`` (a
+    dict) receives the batch sizes and the wall time of the rounds after the
+    first (the first runs eagerly and settles every launch attribute)."""
+    t = backend._torch
+    jobs = []
+    for i, (score, kw) in enumerate(kws):
+        job = _job(backend, score, kw)
+        if job is NotImplemented:
+            raise ValueError(f"template {i}: the fused scan does not apply to these arguments")
+        job["idx_dev"] = t.tensor(np.asarray(job["analyzer"].rotation_indices(job["state"], job["rots"]),
+                                             dtype=np.int32), device=job["rots"].device)
+        jobs.append(job)
+    if not jobs:
+        return
+    _free_plan()  # the group's workspaces need the memory
+    fixed = None if backend.rotation_batch == "auto" else backend.rotation_batch
+    free = backend.get_available_memory()
+    share = 2 * free // len(jobs)  # choose_batch keeps each plan under a quarter of its share
+    tiles = getattr(backend, "scan_tiles", None) or (0, 0)
+    stream = backend._stream()
+    plans, group = [], None
+    try:
+        for job in jobs:
+            job["B"] = choose_batch(job["fast_shape"], job["tshape"], job["mode"], job["filt"] is not None, share,
+                                    fixed, int(job["rots"].shape[0]))
+            plans.append(_create_plan(job, job["B"]))
+            _lib.call("tm_scan_plan_set_tiles", plans[-1], *tiles)
+        n = len(plans)
+        handles = (ctypes.c_void_p * n)(*plans)
+        gh = ctypes.c_void_p()
+        _lib.call("tm_scan_group_create", handles, n, ctypes.byref(gh))
+        group = gh.value
+        rounds = max(-(-int(j["rots"].shape[0]) // j["B"]) for j in jobs)
+        t0, timed_rot = None, 0
+        for r in range(rounds):
+            if r == 1 and stats is not None:
+                _lib.call("tm_stream_sync", stream)
+                t0 = time.perf_counter()
+            rp, ip, nb = (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)(), (ctypes.c_int * n)()
+            ms, mr = (ctypes.c_void_p * n)(), (ctypes.c_void_p * n)()
+            for i, job in enumerate(jobs):
+                start = r * job["B"]
+                cnt = max(0, min(job["B"], int(job["rots"].shape[0]) - start))
+                nb[i] = cnt
+                timed_rot += cnt if r >= 1 else 0
+                if cnt:
+                    rp[i] = job["rots"][start].data_ptr()
+                    ip[i] = job["idx_dev"][start].data_ptr()
+                    ms[i] = job["max_scores"].data_ptr()
+                    mr[i] = job["max_rot"].data_ptr()
+            _lib.call("tm_scan_group_batch", group, rp, ip, nb, ms, mr, stream)
+        _lib.call("tm_stream_sync", stream)
+        if stats is not None:
+            stats.update(batches=[j["B"] for j in jobs], rounds=rounds, rounds_timed=max(0, rounds - 1),
+                         rotations_timed=timed_rot, seconds=None if t0 is None else time.perf_counter() - t0)
+    finally:
+        if group is not None:
+            _lib.call("tm_scan_group_destroy", group)
+        for h in plans:
+            _lib.call("tm_scan_plan_destroy", h)

@@ -954,6 +954,188 @@ int tm_scan_plan_use_graph(tm_scan_plan* p, int enable) {
   return TM_OK;
 }
 
+// --------------------------------------------------------------------------
+// plan groups: one batch of several plans (templates of uneven size against
+// one target) per call, replayed as one graph
+
+struct tm_scan_group {
+  int n;
+  tm_scan_plan** plans;  // not owned
+  hipStream_t gs;
+  hipEvent_t ev_in, ev_out;
+  hipGraphExec_t gexec;
+  int warm;
+  // key of the warm / captured configuration, per plan: batch size (0 =
+  // no batch this call), state arrays and tile shapes
+  int* k_nb;
+  float** k_ms;
+  int** k_mr;
+  int* k_tiles;
+};
+
+int tm_scan_group_destroy(tm_scan_group* g);
+
+int tm_scan_group_create(tm_scan_plan* const* plans, int n, tm_scan_group** out) {
+  TM_REQUIRE(plans && out && n >= 1 && n <= 64, "plan group: 1 to 64 plans, got %d", n);
+  for (int i = 0; i < n; ++i) {
+    TM_REQUIRE(plans[i], "plan group: null plan %d", i);
+    for (int j = 0; j < i; ++j) TM_REQUIRE(plans[j] != plans[i], "plan group: plan %d repeated", i);
+  }
+  tm_scan_group* g = (tm_scan_group*)calloc(1, sizeof(tm_scan_group));
+  if (!g) {
+    tm_set_error("plan group: out of host memory");
+    return TM_ERR_OOM;
+  }
+  g->n = n;
+  g->plans = (tm_scan_plan**)calloc(n, sizeof(tm_scan_plan*));
+  g->k_nb = (int*)calloc(n, sizeof(int));
+  g->k_ms = (float**)calloc(n, sizeof(float*));
+  g->k_mr = (int**)calloc(n, sizeof(int*));
+  g->k_tiles = (int*)calloc(2 * n, sizeof(int));
+  if (!g->plans || !g->k_nb || !g->k_ms || !g->k_mr || !g->k_tiles) {
+    tm_scan_group_destroy(g);
+    tm_set_error("plan group: out of host memory");
+    return TM_ERR_OOM;
+  }
+  for (int i = 0; i < n; ++i) g->plans[i] = plans[i];
+  int rc = TM_OK;
+  // the graph reads each plan's rotation buffers (allocated with its own graph state)
+  for (int i = 0; i < n && rc == TM_OK; ++i) {
+    const int was = plans[i]->use_graph;
+    rc = tm_scan_plan_use_graph(plans[i], 1);
+    plans[i]->use_graph = was;
+  }
+  if (rc == TM_OK && hipStreamCreateWithFlags(&g->gs, hipStreamNonBlocking) != hipSuccess) rc = TM_ERR_HIP;
+  if (rc == TM_OK && hipEventCreateWithFlags(&g->ev_in, hipEventDisableTiming) != hipSuccess) rc = TM_ERR_HIP;
+  if (rc == TM_OK && hipEventCreateWithFlags(&g->ev_out, hipEventDisableTiming) != hipSuccess) rc = TM_ERR_HIP;
+  if (rc != TM_OK) {
+    if (rc == TM_ERR_HIP) tm_set_error("plan group: creating its stream and events failed");
+    tm_scan_group_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return TM_OK;
+}
+
+int tm_scan_group_destroy(tm_scan_group* g) {
+  if (!g) return TM_OK;
+  if (g->gs) hipStreamSynchronize(g->gs);
+  if (g->gexec) hipGraphExecDestroy(g->gexec);
+  if (g->ev_in) hipEventDestroy(g->ev_in);
+  if (g->ev_out) hipEventDestroy(g->ev_out);
+  if (g->gs) hipStreamDestroy(g->gs);
+  free(g->plans);
+  free(g->k_nb);
+  free(g->k_ms);
+  free(g->k_mr);
+  free(g->k_tiles);
+  free(g);
+  return TM_OK;
+}
+
+// One batch of every plan with nbatch[i] > 0, in plan order, into that
+// plan's state arrays.  The first call of a configuration (batch sizes,
+// state arrays, tile shapes) runs eagerly on the group stream, the second
+// captures the whole sequence into one graph, later ones replay it; the
+// group stream is event-ordered after and before the caller's stream.
+// Per-kernel timing forces eager launches on the caller's stream.
+int tm_scan_group_batch(tm_scan_group* g, const float* const* rots_dev, const int* const* rot_index_dev,
+                        const int* nbatch, float* const* max_scores, int* const* max_rot, void* stream) {
+  TM_REQUIRE(g && rots_dev && rot_index_dev && nbatch && max_scores && max_rot, "null argument to tm_scan_group_batch");
+  int active = 0;
+  for (int i = 0; i < g->n; ++i) {
+    TM_REQUIRE(nbatch[i] >= 0 && nbatch[i] <= g->plans[i]->max_batch, "plan %d: batch %d outside [0, %d]", i,
+               nbatch[i], g->plans[i]->max_batch);
+    if (nbatch[i] == 0) continue;
+    TM_REQUIRE(rots_dev[i] && rot_index_dev[i] && max_scores[i] && max_rot[i], "plan %d: null batch argument", i);
+    ++active;
+  }
+  if (!active) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (tm_timing_on()) {
+    for (int i = 0; i < g->n; ++i) {
+      if (!nbatch[i]) continue;
+      const int rc = scan_batch_launch(g->plans[i], rots_dev[i], rot_index_dev[i], nbatch[i], max_scores[i],
+                                       max_rot[i], s);
+      if (rc != TM_OK) return rc;
+    }
+    return TM_OK;
+  }
+  TM_CHECK_HIP(hipEventRecord(g->ev_in, s));
+  TM_CHECK_HIP(hipStreamWaitEvent(g->gs, g->ev_in, 0));
+  bool same = true;
+  for (int i = 0; i < g->n; ++i) {
+    tm_scan_plan* p = g->plans[i];
+    if (nbatch[i]) {
+      TM_CHECK_HIP(hipMemcpyAsync(p->rbuf, rots_dev[i], sizeof(float) * 9 * nbatch[i], hipMemcpyDeviceToDevice,
+                                  g->gs));
+      TM_CHECK_HIP(hipMemcpyAsync(p->ibuf, rot_index_dev[i], sizeof(int) * nbatch[i], hipMemcpyDeviceToDevice,
+                                  g->gs));
+    }
+    float* ms = nbatch[i] ? max_scores[i] : nullptr;
+    int* mr = nbatch[i] ? max_rot[i] : nullptr;
+    if (g->k_nb[i] != nbatch[i] || g->k_ms[i] != ms || g->k_mr[i] != mr || g->k_tiles[2 * i] != p->a_cols ||
+        g->k_tiles[2 * i + 1] != p->c_rows)
+      same = false;
+    g->k_nb[i] = nbatch[i];
+    g->k_ms[i] = ms;
+    g->k_mr[i] = mr;
+    g->k_tiles[2 * i] = p->a_cols;
+    g->k_tiles[2 * i + 1] = p->c_rows;
+  }
+  auto launch_all = [&]() -> int {
+    for (int i = 0; i < g->n; ++i) {
+      if (!nbatch[i]) continue;
+      tm_scan_plan* p = g->plans[i];
+      const int rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch[i], max_scores[i], max_rot[i], g->gs);
+      if (rc != TM_OK) return rc;
+    }
+    return TM_OK;
+  };
+  int rc = TM_OK;
+  if (!same || !g->warm) {
+    // a new configuration: drop the old graph and run eagerly once
+    if (g->gexec) {
+      TM_CHECK_HIP(hipGraphExecDestroy(g->gexec));
+      g->gexec = nullptr;
+    }
+    g->warm = 1;
+    rc = launch_all();
+  } else if (!g->gexec) {
+    hipGraph_t graph = nullptr;
+    TM_CHECK_HIP(hipStreamBeginCapture(g->gs, hipStreamCaptureModeThreadLocal));
+    rc = launch_all();
+    const hipError_t e = hipStreamEndCapture(g->gs, &graph);
+    if (rc == TM_OK && e != hipSuccess) {
+      tm_set_error("capturing the group graph: %s", hipGetErrorString(e));
+      rc = TM_ERR_HIP;
+    }
+    if (rc == TM_OK) {
+      const hipError_t ie = hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        tm_set_error("instantiating the group graph: %s", hipGetErrorString(ie));
+        g->gexec = nullptr;
+        rc = TM_ERR_HIP;
+      }
+    }
+    if (graph) hipGraphDestroy(graph);
+    if (rc == TM_OK && hipGraphLaunch(g->gexec, g->gs) != hipSuccess) {
+      tm_set_error("launching the group graph failed");
+      rc = TM_ERR_HIP;
+    }
+  } else if (hipGraphLaunch(g->gexec, g->gs) != hipSuccess) {
+    tm_set_error("launching the group graph failed");
+    rc = TM_ERR_HIP;
+  }
+  if (rc != TM_OK) {
+    g->warm = 0;  // start over with an eager run on the next call
+    return rc;
+  }
+  TM_CHECK_HIP(hipEventRecord(g->ev_out, g->gs));
+  TM_CHECK_HIP(hipStreamWaitEvent(s, g->ev_out, 0));
+  return TM_OK;
+}
+
 // Tile shapes of the scan's passes (chosen by the batch autotuner):
 // pass_a_columns 16 or 8 (0: 16 where the padded rows allow it), pass_c_rows
 // 5 or 10 (0: 5; other than 5 only for an x length of 200).  A change drops

@@ -310,19 +310,27 @@ def scan_templates(
     score: str = "FLC",
     callback_class=None,
     callback_class_args: Dict = None,
+    schedule: str = "sequential",
     **scan_kwargs,
 ) -> List[Optional[Tuple]]:
     """Several templates of uneven size against one target, one result per
-    template (the multi-template schedule the survey's config C2 describes;
-    the reference runs one template per invocation of
-    scripts/match_template.py).
+    template (a multi-template schedule; the reference runs one template per
+    invocation of scripts/match_template.py, tme/matching_exhaustive.py:
+    298-465 each).
 
     Each entry of ``templates`` is a dict with ``template`` and optional
     ``template_mask``, ``rotations``, ``template_filter`` and
-    ``target_filter`` (a ``Compose``).  Every template runs through
-    :func:`scan_subsets` with its own analyzer; on the hip backend each
-    template's fused plan is created once and kept for all its rotations.
-    ``scan_kwargs`` are passed to :func:`scan_subsets`.
+    ``target_filter`` (a ``Compose``).  ``schedule="sequential"``: every
+    template runs through :func:`scan_subsets` with its own analyzer (on the
+    hip backend each template's fused plan is created once and kept for all
+    its rotations); ``scan_kwargs`` are passed to :func:`scan_subsets`.
+    ``schedule="graph"`` (hip backend, fused scan, whole target: no
+    ``target_splits`` / ``template_splits``): the templates' setups run
+    first, then every round scans one rotation batch of each template as one
+    replayed hipGraph (``backend.fused_group``); the results equal the
+    sequential schedule's.  ``pad_target_edges``, ``pad_template_filter`` and
+    ``interpolation_order`` are honoured there, other ``scan_kwargs`` raise
+    ``ValueError``.
     """
     from .analyzer import MaxScoreOverRotations
     from .matching_data import MatchingData
@@ -332,6 +340,11 @@ def scan_templates(
     setup, scoring = MATCHING_EXHAUSTIVE_REGISTER[score]
     callback_class = MaxScoreOverRotations if callback_class is None else callback_class
     callback_class_args = {} if callback_class_args is None else callback_class_args
+    if schedule == "graph":
+        return _scan_templates_graph(target, templates, setup, scoring, callback_class, callback_class_args,
+                                     **scan_kwargs)
+    if schedule != "sequential":
+        raise ValueError(f"schedule has to be 'sequential' or 'graph', got {schedule!r}")
     results = []
     for entry in templates:
         if "template" not in entry:
@@ -346,6 +359,50 @@ def scan_templates(
             scan_subsets(matching_data=md, matching_setup=setup, matching_score=scoring,
                          callback_class=callback_class, callback_class_args=callback_class_args, **scan_kwargs)
         )
+    return results
+
+
+def _template_data(target, entry):
+    from .matching_data import MatchingData
+
+    if "template" not in entry:
+        raise ValueError("every template entry needs a 'template'")
+    md = MatchingData(target, entry["template"], template_mask=entry.get("template_mask"),
+                      rotations=entry.get("rotations"))
+    if entry.get("template_filter") is not None:
+        md.template_filter = entry["template_filter"]
+    if entry.get("target_filter") is not None:
+        md.target_filter = entry["target_filter"]
+    return md
+
+
+def _scan_templates_graph(target, templates, setup_fn, scoring, callback_class, callback_class_args,
+                          pad_target_edges: bool = False, pad_template_filter: bool = True,
+                          interpolation_order: int = None, **unsupported):
+    """``scan_templates(schedule="graph")``: per template the setup half of
+    :func:`scan` (one split, one rotation job), then all templates' scans
+    through ``backend.fused_group``, then each analyzer's result merged as
+    :func:`scan` and :func:`scan_subsets` merge a single split."""
+    bad = sorted(k for k, v in unsupported.items() if v not in (None, {}, (1, 1), False))
+    if bad:
+        raise ValueError(f"schedule='graph' does not take {', '.join(bad)}")
+    if not hasattr(be, "fused_group"):
+        raise ValueError("schedule='graph' needs the hip backend")
+    jobs, callback_args = [], []
+    for entry in templates:
+        md = _template_data(target, entry)
+        setup, default_callback_args, rotation_chunks = prepare_scan(
+            md, setup_fn, n_jobs=1, callback_class_args=callback_class_args, pad_target=pad_target_edges,
+            pad_template_filter=pad_template_filter, interpolation_order=interpolation_order,
+        )
+        jobs.append((setup, rotation_chunks[0], SharedAnalyzerProxy(callback_class, default_callback_args)))
+        callback_args.append(default_callback_args)
+    be.fused_group(scoring, jobs)
+    results = []
+    for (_, _, proxy), dca in zip(jobs, callback_args):
+        one = callback_class.merge([proxy.result(**dca)], **dca)
+        results.append(callback_class.merge([one], **callback_class_args))
+    be.free_cache()
     return results
 
 

@@ -320,6 +320,44 @@ def test_multi_template_schedule():
             assert s[win].max() >= np.sort(s.ravel())[-200], (k, ctr)
 
 
+@pytest.mark.parametrize("counts", [(28, 28, 28, 28), (28, 12, 20, 5)])
+def test_multi_template_one_graph_schedule(counts):
+    """scan_templates(schedule="graph"): four templates of uneven size whose
+    batches of a round run as one replayed hipGraph equal the sequential
+    schedule bit for bit.  Batches of 8: with 28 rotations each the rounds
+    run eagerly, capture, replay and end with a partial batch; with 28 / 12
+    / 20 / 5 rotations templates drop out of later rounds (batch 0)."""
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.backends import backend as be
+    from pytme_amd.filters import Compose, CTFReconstructed
+    from pytme_amd.matching_exhaustive import scan_templates
+    from pytme_amd.rotations import uniform_rotations
+    from pytme_amd.synthetic import blob_template, threshold_mask
+
+    rng = np.random.default_rng(43)
+    target = rng.standard_normal((96, 80, 72)).astype(np.float32)
+    entries = []
+    for k, (T, dz) in enumerate(((12, 20000.0), (16, 25000.0), (20, 30000.0), (24, 35000.0))):
+        tmpl = blob_template(T, n_blobs=8, seed=70 + k)
+        ctf = CTFReconstructed(defocus_x=dz, sampling_rate=6.8, acceleration_voltage=300e3,
+                               spherical_aberration=2.7e7, amplitude_contrast=0.07, flip_phase=True)
+        entries.append(dict(template=tmpl, template_mask=threshold_mask(tmpl), template_filter=Compose((ctf,)),
+                            rotations=uniform_rotations(counts[k], seed=80 + k, include_identity=True)))
+    kw = dict(callback_class=MaxScoreOverRotations, callback_class_args={"score_threshold": -1.0},
+              interpolation_order=1, pad_template_filter=False)
+    be.change_backend("hip", device="cuda", fused=True, rotation_batch=8)
+    print(f"one-graph multi-template schedule, rotations {counts}", flush=True)
+    graph = scan_templates(target, entries, score="FLC", schedule="graph", **kw)
+    seq = scan_templates(target, entries, score="FLC", job_schedule=(1, 1), **kw)
+    for k in range(len(entries)):
+        assert np.array_equal(graph[k][0], seq[k][0]), k
+        assert np.array_equal(graph[k][2], seq[k][2]), k
+        assert np.array_equal(graph[k][1], seq[k][1]), k
+        assert sorted(graph[k][3]) == sorted(seq[k][3]), k
+    with pytest.raises(ValueError):
+        scan_templates(target, entries[:1], score="FLC", schedule="graph", target_splits={0: 2}, **kw)
+
+
 def test_fused_partial_batches_and_plan_reuse():
     """20 rotations in batches of 8 (8, 8, 4): the kept plan, the partial
     last batch and the rotation index assignment against the FP64 oracle."""
