@@ -386,8 +386,8 @@ def _scan_templates_graph(target, templates, setup_fn, scoring, callback_class, 
     bad = sorted(k for k, v in unsupported.items() if v not in (None, {}, (1, 1), False))
     if bad:
         raise ValueError(f"schedule='graph' does not take {', '.join(bad)}")
-    if not hasattr(be, "fused_group"):
-        raise ValueError("schedule='graph' needs the hip backend")
+    if not hasattr(be, "fused_group") or not (getattr(be, "fused", False) and getattr(be, "_gpu", False)):
+        raise ValueError("schedule='graph' needs the hip backend with the fused scan on a GPU device")
     jobs, callback_args = [], []
     for entry in templates:
         md = _template_data(target, entry)

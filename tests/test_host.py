@@ -56,6 +56,26 @@ class TestBackendManager:
             assert _next_fast_len(n) == next_fast_len(n, real=True)
 
 
+class TestMultiTemplateSchedule:
+    def test_schedule_arguments(self, cpu_backend):
+        """scan_templates refuses an unknown schedule, and the one-graph
+        schedule refuses target/template splits and a backend without the
+        fused GPU scan (here: the hip backend on the CPU device) before any
+        setup work."""
+        from pytme_amd.matching_exhaustive import scan_templates
+
+        target = np.zeros((16, 16, 16), np.float32)
+        entries = [dict(template=np.ones((4, 4, 4), np.float32), template_mask=np.ones((4, 4, 4), np.float32))]
+        with pytest.raises(ValueError, match="schedule"):
+            scan_templates(target, entries, score="FLC", schedule="bogus")
+        with pytest.raises(ValueError, match="target_splits"):
+            scan_templates(target, entries, score="FLC", schedule="graph", target_splits={0: 2})
+        with pytest.raises(ValueError, match="GPU"):
+            scan_templates(target, entries, score="FLC", schedule="graph")
+        with pytest.raises(ValueError):
+            scan_templates(target, entries, score="NoSuchScore", schedule="graph")
+
+
 class TestRegistry:
     def test_register_and_duplicate(self):
         from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, register_matching_exhaustive
