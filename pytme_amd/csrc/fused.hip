@@ -323,11 +323,17 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
 // The X kernel (KIND 1) is held to 128 VGPRs for the 512-point case so two
 // blocks share a CU (a couple of spilled registers are cheaper than half the
 // occupancy).
-template <int N0, int L, int NT, int KIND, bool PR>
+//
+// Split rows (TAIL): the output rows hold ldo bins (a multiple of L) and the
+// last ntl bins of every row go to the dense tail array Qt[o][b][ky][z][ntl];
+// the tail tiles are their own launch (kx_base = ldo) whose stores are
+// predicated to the valid columns, so the main launch stays unpredicated; a
+// tail tile packs L / ntl rows (ky) of ntl columns each.
+template <int N0, int L, int NT, int KIND, bool PR, bool TAIL = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((KIND == 1 && PR && N0 <= 512) ? 4 : 1, 8)))
 k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
-            float2* __restrict__ Q, const float2* __restrict__ tw, int T0, int N1, int N2s, int B,
-            int tiles_per_row, int q_in, int o_x) {
+            float2* __restrict__ Q, float2* __restrict__ Qt, const float2* __restrict__ tw, int T0, int N1, int N2s,
+            int ldo, int B, int tiles_per_row, int kx_base, int ntl, int q_in, int o_x) {
   using PF = CtPlan<N0, L, NT, true, false>;
   using PI = CtPlan<N0, L, NT, true, true>;
   static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
@@ -342,10 +348,24 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   const int tid = threadIdx.x;
   for (int i = tid; i < N0; i += NT) tws[i] = tw[i];
   const int bid = tm_xcd_block();
-  const int ky = bid / tiles_per_row;
-  const int kx0 = (bid - ky * tiles_per_row) * L;
+  const int ky = TAIL ? bid * (L / ntl) : bid / tiles_per_row;
+  const int kx0 = kx_base + (TAIL ? 0 : (bid - ky * tiles_per_row) * L);
+  // TAIL: lane column l holds row ky + l / ntl, column kx0 + l % ntl; its
+  // input offset from col is tl; lanes past the packed rows read a valid
+  // column and store nothing
+  int tl = 0, tky = ky, tc = 0;
+  bool tok = true;
+  if constexpr (TAIL) {
+    const int lane = tid % L, j = lane / ntl;
+    tok = j < L / ntl && ky + j < N1;
+    tky = tok ? ky + j : ky;
+    tc = tok ? lane - j * ntl : 0;
+    tl = (tky - ky) * N2s + tc;
+  }
   const int plane = N1 * N2s;  // per-slab offsets fit in 32 bits (checked by the launcher)
   const int64_t col = (int64_t)ky * N2s + kx0;
+  const int plane_o = N1 * ldo;  // output rows of ldo bins
+  const int64_t col_o = (int64_t)ky * ldo + kx0;
   const float2* Ftc = Ft + col;
   const float2* Ft2c = KIND == 0 ? Ft2 + col : nullptr;
 
@@ -357,8 +377,9 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       int l, e;
       PF::out_pos(tid, i, r, l, e);
       const int s = i * PF::R_OUT + r;
-      F[s] = Ftc[e * plane + l];
-      if constexpr (KIND == 0) F2[s] = Ft2c[e * plane + l];
+      const int lo = TAIL ? tl : l;
+      F[s] = Ftc[e * plane + lo];
+      if constexpr (KIND == 0) F2[s] = Ft2c[e * plane + lo];
     }
 
   // input element of load slot k: rows e >= T0 are clamped to a valid row
@@ -379,21 +400,33 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
     for (int k = 0; k < NX; ++k) {
       int l, e;
       in_elem(k, l, e);
-      dst[k] = src[(e < T0 ? e : T0 - 1) * plane + l];
+      dst[k] = src[(e < T0 ? e : T0 - 1) * plane + (TAIL ? tl : l)];
     }
   };
 
   auto inv_store = [&](int o, int b, float2* w) {
     ct_fft<PI, 1, L>(lds, tws, w, w, tid);
-    float2* dst = Q + ((int64_t)(o * B + b) * N0) * (int64_t)plane + col;
+    if constexpr (TAIL) {
+      float2* dst = Qt + ((int64_t)(o * B + b) * N1 + tky) * (int64_t)(N0 * ntl) + tc;
 #pragma unroll
-    for (int i = 0; i < PI::E_OUT / PI::R_OUT; ++i)
+      for (int i = 0; i < PI::E_OUT / PI::R_OUT; ++i)
 #pragma unroll
-      for (int r = 0; r < PI::R_OUT; ++r) {
-        int l, e;
-        PI::out_pos(tid, i, r, l, e);
-        tm_st_stream<L * sizeof(float2) >= 128>(dst + e * plane + l, w[i * PI::R_OUT + r]);
-      }
+        for (int r = 0; r < PI::R_OUT; ++r) {
+          int l, e;
+          PI::out_pos(tid, i, r, l, e);
+          if (tok) dst[e * ntl] = w[i * PI::R_OUT + r];
+        }
+    } else {
+      float2* dst = Q + ((int64_t)(o * B + b) * N0) * (int64_t)plane_o + col_o;
+#pragma unroll
+      for (int i = 0; i < PI::E_OUT / PI::R_OUT; ++i)
+#pragma unroll
+        for (int r = 0; r < PI::R_OUT; ++r) {
+          int l, e;
+          PI::out_pos(tid, i, r, l, e);
+          tm_st_stream<L * sizeof(float2) >= 128>(dst + e * plane_o + l, w[i * PI::R_OUT + r]);
+        }
+    }
   };
 
   load_in(0, x);
@@ -452,9 +485,13 @@ constexpr int pass_c_ld(int M) { return M + 1; }
 // are prefetched PD deep into registers; every prefetch is issued (the tail
 // re-reads the last tile), so waiting for the oldest one never waits for the
 // younger ones.
-template <int M, int RL, int NT, int NQ, int PD>
+//
+// SPLIT: rows hold ldq main bins in Q and the remaining N2c - ldq bins in
+// the tail array Qt[o][b][y][z][ntl] (row r = z * N1 + y).
+template <int M, int RL, int NT, int NQ, int PD, bool SPLIT = false>
 __global__ void __launch_bounds__(NT)
-k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_t nrows, int ldq, int B,
+k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const float2* __restrict__ tw2,
+            int64_t nrows, int ldq, int N0, int N1, int ntl, int B,
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
             float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
   using PI = CtPlan<M, RL, NT, false, true>;
@@ -469,7 +506,9 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
   __shared__ float2 tws[N2];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
   for (int i = tid; i < N2; i += NT) tws[i] = tw2[i];
-  const int64_t r0 = (int64_t)blockIdx.x * RL;
+  // XCD-grouped row blocks (the grid is a multiple of 8; blocks past the rows exit)
+  const int64_t r0 = (int64_t)tm_xcd_block() * RL;
+  if (r0 >= nrows) return;
   const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
   const int tile_n = rl * N2c;
   const int iters = B * NQ;
@@ -482,16 +521,41 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
     arg0[s] = arg1[s] = -1;
     d0[s] = d1[s] = 0.f;
   }
-  // rows of slab (o, b) are consecutive at stride ldq: one tile = rl rows
-  auto prefetch = [&](int it, float2* pf) {
-    const int b = it / NQ, o = it - (it / NQ) * NQ;
-    const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;
+  // rows of slab (o, b) are consecutive at stride ldq: one tile = rl rows.
+  // Each load slot's offset within a slab is the same for every tile:
+  // computed once (split rows: >= 0 main offset, < 0 the tail offset -1-t).
+  int offs[NPF];
+  {
+    const int z0 = SPLIT ? (int)(r0 / N1) : 0;
+    const int y0 = SPLIT ? (int)(r0 - (int64_t)z0 * N1) : 0;
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
       int f = tid + p * NT;
       f = f < tile_n ? f : tile_n - 1;  // unpredicated: out-of-tile slots re-read a valid bin
       const int l = f / N2c, e = f - l * N2c;
-      pf[p] = tm_ld_stream(src + l * ldq + e);
+      if constexpr (SPLIT) {
+        // row r0 + l = (z, y); its tail bins sit at Qt[y][z][e - ldq] (RL <= N1)
+        int y = y0 + l, z = z0;
+        if (y >= N1) {
+          y -= N1;
+          ++z;
+        }
+        offs[p] = e < ldq ? l * ldq + e : -1 - ((y * N0 + z) * ntl + (e - ldq));
+      } else {
+        offs[p] = l * ldq + e;
+      }
+    }
+  }
+  auto prefetch = [&](int it, float2* pf) {
+    const int b = it / NQ, o = it - (it / NQ) * NQ;
+    const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;
+    const float2* tsrc = SPLIT ? Qt + (int64_t)(o * B + b) * nrows * ntl : nullptr;
+#pragma unroll
+    for (int p = 0; p < NPF; ++p) {
+      if constexpr (SPLIT)
+        pf[p] = tm_ld_stream(offs[p] >= 0 ? src + offs[p] : tsrc + (-1 - offs[p]));
+      else
+        pf[p] = tm_ld_stream(src + offs[p]);
     }
   };
   auto step = [&](int it, int o, float2* pf) {
@@ -608,62 +672,89 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ tw2, int64_
     }
 }
 
+// ntl > 0: split rows (main rows of ldo bins in Q, ntl tail bins in Qt); the
+// tail tile (kx_base = ldo) is a second launch of each kind.
 template <int N0, int L, int NT, int NT0, bool PR>
 static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
-                               const float2* tw, int T0, int N1, int N2s, int tpr, int B, double bytes,
-                               hipStream_t s) {
+                               float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int ntl, int tpr,
+                               int B, double bytes, hipStream_t s) {
   const int64_t blocks = (int64_t)N1 * tpr;
   const size_t shm = (size_t)N0 * L * sizeof(float2);
+  const double fm = ntl > 0 ? (double)ldo / (ldo + ntl) : 1.0;  // the main launch's share of the bytes
+  const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
   if (flc) {
-    TM_TIMED("fused_pass_a", bytes * 0.6, s,
+    TM_TIMED("fused_pass_a", bytes * 0.6 * fm, s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
-                                Q, tw, T0, N1, N2s, B, tpr, 0, 0));
+                                Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, 0, 0));
+    TM_CHECK_LAUNCH();
+    if (ntl > 0) {
+      TM_TIMED("fused_pass_a_tail", bytes * 0.6 * (1.0 - fm), s,
+               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true>), dim3(tail_blocks), dim3(NT0), shm, s, P,
+                                  ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, ldo, ntl, 0, 0));
+      TM_CHECK_LAUNCH();
+    }
+  }
+  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", (flc ? bytes * 0.4 : bytes) * fm, s,
+           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
+                              (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, flc ? 1 : 0,
+                              flc ? 2 : 0));
+  TM_CHECK_LAUNCH();
+  if (ntl > 0) {
+    TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", (flc ? bytes * 0.4 : bytes) * (1.0 - fm), s,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true>), dim3(tail_blocks), dim3(NT), shm, s, P, ft,
+                                (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, ldo, ntl, flc ? 1 : 0,
+                                flc ? 2 : 0));
     TM_CHECK_LAUNCH();
   }
-  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", flc ? bytes * 0.4 : bytes, s,
-           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
-                              (const float2*)nullptr, Q, tw, T0, N1, N2s, B, tpr, flc ? 1 : 0, flc ? 2 : 0));
-  TM_CHECK_LAUNCH();
   return TM_OK;
 }
 
 // NT: threads of the X kernel, NT0: threads of the FLC mask kernel
 template <int N0, int L, int NT, int NT0 = NT>
-static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
-                            const float2* tw, int T0, int N1, int N2c, int N2s, int B, double bytes, hipStream_t s) {
+static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q, float2* Qt,
+                            const float2* tw, int T0, int N1, int N2c, int N2s, int ldo, int ntl, int B, double bytes,
+                            hipStream_t s) {
   TM_REQUIRE((int64_t)N0 * N1 * N2s < (1LL << 31), "pass A: z column stride too large");
-  const int tpr = (int)tm_ceil_div(N2c, L);
+  const int tpr = ntl > 0 ? ldo / L : (int)tm_ceil_div(N2c, L);
   TM_REQUIRE((int64_t)tpr * L <= N2s, "pass A: spectra rows not padded to whole tiles");
+  TM_REQUIRE(ntl == 0 || (ldo % L == 0 && ldo + ntl == N2c && ntl <= L && ldo + L <= N2s),
+             "pass A: split rows need main rows of whole tiles and one tail tile inside the padded input rows");
   using PF = CtPlan<N0, L, NT, true, false>;
   if (T0 <= N0 / PF::R_IN)
-    return launch_pass_a_ct_pr<N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
-  return launch_pass_a_ct_pr<N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, tw, T0, N1, N2s, tpr, B, bytes, s);
+    return launch_pass_a_ct_pr<N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, ntl, tpr, B,
+                                                     bytes, s);
+  return launch_pass_a_ct_pr<N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, ntl, tpr, B,
+                                                    bytes, s);
 }
 
-template <int M, int RL, int NT, int PD3, int PD1>
-static int launch_pass_c_ct(bool flc, const float2* Q, const float2* tw2, int64_t nrows, int ldq, int B,
-                            const double* n_obs, const float* inv_den, const int* rot_idx, float scale, float eps,
-                            float* max_scores, int* max_rot, double bytes, hipStream_t s) {
+template <int M, int RL, int NT, int PD3, int PD1, bool SPLIT = false>
+static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const float2* tw2, int64_t nrows, int ldq,
+                            int N0, int N1, int ntl, int B, const double* n_obs, const float* inv_den,
+                            const int* rot_idx, float scale, float eps, float* max_scores, int* max_rot, double bytes,
+                            hipStream_t s) {
+  TM_REQUIRE(!SPLIT || (N1 >= RL && ldq + ntl == M + 1 && Qt), "pass C: split rows need N1 >= %d rows", RL);
   // PD3 / PD1: prefetch depth of the FLC (3 spectra) / correlation kernels;
   // the step count B * NQ must be a multiple of the unroll lcm(NQ, PD)
-  const int64_t blocks = tm_ceil_div(nrows, RL);
+  const int64_t blocks = tm_ceil_div(tm_ceil_div(nrows, RL), 8) * 8;  // a multiple of 8: XCD-grouped blocks
   TM_REQUIRE(blocks <= 0x7fffffff, "pass C grid too large");
+  TM_REQUIRE(!SPLIT || (int64_t)N0 * N1 * ntl < (1LL << 31), "pass C: tail offsets exceed 32 bits");
   const size_t shm = (size_t)(RL + 1) * pass_c_ld(M) * sizeof(float2);  // + the junk line
   if (flc) {
     static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
-                                nrows, ldq, B, n_obs, (const float*)nullptr, rot_idx, scale, eps, max_scores, max_rot));
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3, SPLIT>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+                                Qt, tw2, nrows, ldq, N0, N1, ntl, B, n_obs, (const float*)nullptr, rot_idx, scale, eps,
+                                max_scores, max_rot));
   } else if (B % PD1 == 0) {
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
-                                nrows, ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores,
-                                max_rot));
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1, SPLIT>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+                                Qt, tw2, nrows, ldq, N0, N1, ntl, B, (const double*)nullptr, inv_den, rot_idx, scale,
+                                eps, max_scores, max_rot));
   } else {
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1>), dim3((unsigned)blocks), dim3(NT), shm, s, Q, tw2,
-                                nrows, ldq, B, (const double*)nullptr, inv_den, rot_idx, scale, eps, max_scores,
-                                max_rot));
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1, SPLIT>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+                                Qt, tw2, nrows, ldq, N0, N1, ntl, B, (const double*)nullptr, inv_den, rot_idx, scale,
+                                eps, max_scores, max_rot));
   }
   TM_CHECK_LAUNCH();
   return TM_OK;
@@ -676,6 +767,7 @@ struct tm_scan_plan {
   int mode;  // 0 FLC, 1 CORR
   int nq_in, nq_out;
   int N[3], N2c, N2s;  // N2s: row stride of the workspace spectra (N2c rounded up to 16 bins = 128 B)
+  int N2m, ntl;        // correlation spectra rows: N2m bins in Q, ntl tail bins in Qt (split rows) or N2s, 0
   int T[3], T2c;
   int max_batch;
   int normalize;
@@ -691,7 +783,7 @@ struct tm_scan_plan {
   tm_fft_plan* tplan;  // template-size plan (filter)
   // workspace
   float *TR, *MR;
-  float2 *specT, *Q1, *P, *Q;
+  float2 *specT, *Q1, *P, *Q, *Qt;
   double *n_obs, *rot_ws, *mom_ws, *ms;
   // graph replay of the batch sequence (kernels only) on a plan stream
   int a_cols;  // pass A column tile: 16 (default where the rows allow it) or 8
@@ -712,7 +804,7 @@ extern "C" int tm_fft_plan_destroy(tm_fft_plan* p);
 
 static void scan_plan_free(tm_scan_plan* p) {
   if (!p) return;
-  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->n_obs, p->rot_ws, p->mom_ws, p->ms, p->ftp, p->ft2p,
+  void* bufs[] = {p->TR, p->MR, p->specT, p->Q1, p->P, p->Q, p->Qt, p->n_obs, p->rot_ws, p->mom_ws, p->ms, p->ftp, p->ft2p,
                   p->rbuf, p->ibuf};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -741,6 +833,20 @@ static void scan_plan_free(tm_scan_plan* p) {
 // (8-bin rows would save the padding traffic but 64-byte row segments
 // stream measurably slower than 128-byte ones on MI355X.)
 static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
+
+// Row layout of the correlation spectra.  Split rows where the compile-time
+// passes run (x length 200: 101 bins; z length 128, 256 or 512): 96 bins per
+// row in Q and the last 5 in the dense tail array, so no pass moves the 11
+// padding bins of a 112-bin row; otherwise rows padded to N2s bins.
+#ifndef TM_SPLIT_ROWS
+#define TM_SPLIT_ROWS 1
+#endif
+static inline void row_layout(int64_t N0, int64_t N1, int64_t N2, int64_t& N2m, int64_t& ntl) {
+  const int64_t N2c = N2 / 2 + 1;
+  const bool split = TM_SPLIT_ROWS && N2 == 200 && (N0 == 128 || N0 == 256 || N0 == 512) && N1 >= 16;
+  N2m = split ? N2c / 16 * 16 : pad_bins(N2c);
+  ntl = split ? N2c - N2m : 0;
+}
 
 // Copy the caller's target spectra (rows of N2c bins) into the padded copies.
 static int scan_copy_target(tm_scan_plan* p, const void* ft, const void* ft2, hipStream_t s) {
@@ -771,7 +877,9 @@ int64_t tm_scan_workspace_bytes(int mode, const int64_t* fast_shape, const int* 
   if (with_filter) bytes += 8 * B * T0 * T1 * (T2 / 2 + 1);
   bytes += 8 * nq_in * B * T0 * T1 * N2s;
   bytes += 8 * nq_in * B * T0 * N1 * N2s;
-  bytes += 8 * nq_out * B * N0 * N1 * N2s;
+  int64_t N2m = 0, ntl = 0;
+  row_layout(N0, N1, fast_shape[2], N2m, ntl);
+  bytes += 8 * nq_out * B * N0 * N1 * (N2m + ntl);
   if (N2s != N2c) bytes += 8 * (mode == 0 ? 2 : 1) * N0 * N1 * N2s;  // padded target spectra
   return bytes;
 }
@@ -803,6 +911,12 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
   }
   p->N2c = p->N[2] / 2 + 1;
   p->N2s = (int)pad_bins(p->N2c);
+  {
+    int64_t n2m = 0, ntl = 0;
+    row_layout(p->N[0], p->N[1], p->N[2], n2m, ntl);
+    p->N2m = (int)n2m;
+    p->ntl = (int)ntl;
+  }
   p->T2c = p->T[2] / 2 + 1;
   p->max_batch = max_batch;
   p->normalize = mode == 0 ? 1 : normalize;
@@ -833,7 +947,8 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
   if (filter) TM_ALLOC(p->specT, sizeof(float2) * B * T0 * T1 * p->T2c);
   TM_ALLOC(p->Q1, sizeof(float2) * p->nq_in * B * T0 * T1 * N2s);
   TM_ALLOC(p->P, sizeof(float2) * p->nq_in * B * T0 * N1 * N2s);
-  TM_ALLOC(p->Q, sizeof(float2) * p->nq_out * B * N0 * N1 * N2s);
+  TM_ALLOC(p->Q, sizeof(float2) * p->nq_out * B * N0 * N1 * p->N2m);
+  if (p->ntl) TM_ALLOC(p->Qt, sizeof(float2) * p->nq_out * B * N0 * N1 * p->ntl);
   if (p->N2s != p->N2c) {
     TM_ALLOC(p->ftp, sizeof(float2) * N0 * N1 * N2s);
     if (mode == 0) TM_ALLOC(p->ft2p, sizeof(float2) * N0 * N1 * N2s);
@@ -1226,7 +1341,8 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   // 16-bin column tiles when they fit the padded rows, else 8-bin tiles
   const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s && p->a_cols != 8;
 #define TM_PASS_A(N0_, L_, ...) \
-  launch_pass_a_ct<N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, lz.tw, T0, N1, N2c, N2s, B, bytes_a, s)
+  launch_pass_a_ct<N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, p->Qt, lz.tw, T0, N1, N2c, N2s, p->N2m, \
+                                         p->ntl, B, bytes_a, s)
   switch (N0) {
     case 128: rc = l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128); break;
     case 256: rc = l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256); break;
@@ -1260,7 +1376,16 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   }
 
   // 6. pass B: inverse y in place
-  rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, N2s, 1, 1.0f, s, "fused_pass_b");
+  if (p->ntl) {
+    // split rows: the main rows (whole 16-bin tiles), then the tail array
+    // (lines (z, c) of N0 * ntl, whole 128-byte segments)
+    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, p->N2m, p->N2m, 1, 1.0f, s, "fused_pass_b");
+    if (rc != TM_OK) return rc;
+    rc = tm_launch_c2c(ly, p->Qt, N1, p->Qt, (int64_t)p->nq_out * B, (int64_t)N0 * p->ntl, (int64_t)N0 * p->ntl, 1,
+                       1.0f, s, "fused_pass_b_tail");
+  } else {
+    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, N2s, 1, 1.0f, s, "fused_pass_b");
+  }
   if (rc != TM_OK) return rc;
 
   // 7. pass C: inverse x, score, max/argmax
@@ -1273,14 +1398,20 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     const float eps = 1.1920928955078125e-07f;
     done_c = true;
     switch (N2) {
-      case 128: rc = launch_pass_c_ct<64, 32, 256, 1, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 128: rc = launch_pass_c_ct<64, 32, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
       case 200:
-        if (p->c_rows == 10)
-          rc = launch_pass_c_ct<100, 10, 256, 3, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
-        else
-          rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        if (p->ntl) {  // split rows: main bins in Q (row stride N2m), the tail bins in Qt
+          if (p->c_rows == 10)
+            rc = launch_pass_c_ct<100, 10, 256, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          else
+            rc = launch_pass_c_ct<100, 5, 128, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        } else if (p->c_rows == 10) {
+          rc = launch_pass_c_ct<100, 10, 256, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        } else {
+          rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        }
         break;
-      case 256: rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, lx.tw, nrows, N2s, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 256: rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
       default: done_c = false; break;
     }
     if (done_c && rc != TM_OK) return rc;
