@@ -540,7 +540,8 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
           y -= N1;
           ++z;
         }
-        offs[p] = e < ldq ? l * ldq + e : -1 - ((y * N0 + z) * ntl + (e - ldq));
+        const int nm = N2c - ntl;  // main bins; ldq is the row stride (>= nm)
+        offs[p] = e < nm ? l * ldq + e : -1 - ((y * N0 + z) * ntl + (e - nm));
       } else {
         offs[p] = l * ldq + e;
       }
@@ -676,11 +677,11 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
 // tail tile (kx_base = ldo) is a second launch of each kind.
 template <int N0, int L, int NT, int NT0, bool PR>
 static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
-                               float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int ntl, int tpr,
-                               int B, double bytes, hipStream_t s) {
+                               float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int nm, int ntl,
+                               int tpr, int B, double bytes, hipStream_t s) {
   const int64_t blocks = (int64_t)N1 * tpr;
   const size_t shm = (size_t)N0 * L * sizeof(float2);
-  const double fm = ntl > 0 ? (double)ldo / (ldo + ntl) : 1.0;  // the main launch's share of the bytes
+  const double fm = ntl > 0 ? (double)nm / (nm + ntl) : 1.0;  // the main launch's share of the bytes
   const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
   if (flc) {
     TM_TIMED("fused_pass_a", bytes * 0.6 * fm, s,
@@ -690,7 +691,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
     if (ntl > 0) {
       TM_TIMED("fused_pass_a_tail", bytes * 0.6 * (1.0 - fm), s,
                hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true>), dim3(tail_blocks), dim3(NT0), shm, s, P,
-                                  ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, ldo, ntl, 0, 0));
+                                  ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 0));
       TM_CHECK_LAUNCH();
     }
   }
@@ -702,7 +703,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   if (ntl > 0) {
     TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", (flc ? bytes * 0.4 : bytes) * (1.0 - fm), s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true>), dim3(tail_blocks), dim3(NT), shm, s, P, ft,
-                                (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, ldo, ntl, flc ? 1 : 0,
+                                (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
                                 flc ? 2 : 0));
     TM_CHECK_LAUNCH();
   }
@@ -715,15 +716,16 @@ static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const f
                             const float2* tw, int T0, int N1, int N2c, int N2s, int ldo, int ntl, int B, double bytes,
                             hipStream_t s) {
   TM_REQUIRE((int64_t)N0 * N1 * N2s < (1LL << 31), "pass A: z column stride too large");
-  const int tpr = ntl > 0 ? ldo / L : (int)tm_ceil_div(N2c, L);
+  const int nm = N2c - ntl;  // main bins (split rows) or N2c
+  const int tpr = ntl > 0 ? nm / L : (int)tm_ceil_div(N2c, L);
   TM_REQUIRE((int64_t)tpr * L <= N2s, "pass A: spectra rows not padded to whole tiles");
-  TM_REQUIRE(ntl == 0 || (ldo % L == 0 && ldo + ntl == N2c && ntl <= L && ldo + L <= N2s),
+  TM_REQUIRE(ntl == 0 || (nm % L == 0 && nm <= ldo && ntl <= L && nm + L <= N2s),
              "pass A: split rows need main rows of whole tiles and one tail tile inside the padded input rows");
   using PF = CtPlan<N0, L, NT, true, false>;
   if (T0 <= N0 / PF::R_IN)
-    return launch_pass_a_ct_pr<N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, ntl, tpr, B,
+    return launch_pass_a_ct_pr<N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr, B,
                                                      bytes, s);
-  return launch_pass_a_ct_pr<N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, ntl, tpr, B,
+  return launch_pass_a_ct_pr<N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr, B,
                                                     bytes, s);
 }
 
@@ -732,7 +734,8 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const f
                             int N0, int N1, int ntl, int B, const double* n_obs, const float* inv_den,
                             const int* rot_idx, float scale, float eps, float* max_scores, int* max_rot, double bytes,
                             hipStream_t s) {
-  TM_REQUIRE(!SPLIT || (N1 >= RL && ldq + ntl == M + 1 && Qt), "pass C: split rows need N1 >= %d rows", RL);
+  TM_REQUIRE(!SPLIT || (N1 >= RL && ntl >= 1 && ldq >= M + 1 - ntl && Qt), "pass C: split rows need N1 >= %d rows",
+             RL);
   // PD3 / PD1: prefetch depth of the FLC (3 spectra) / correlation kernels;
   // the step count B * NQ must be a multiple of the unroll lcm(NQ, PD)
   const int64_t blocks = tm_ceil_div(tm_ceil_div(nrows, RL), 8) * 8;  // a multiple of 8: XCD-grouped blocks
@@ -835,17 +838,23 @@ static void scan_plan_free(tm_scan_plan* p) {
 static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
 
 // Row layout of the correlation spectra.  Split rows where the compile-time
-// passes run (x length 200: 101 bins; z length 128, 256 or 512): 96 bins per
-// row in Q and the last 5 in the dense tail array, so no pass moves the 11
-// padding bins of a 112-bin row; otherwise rows padded to N2s bins.
+// passes run (x length 200: 101 bins, 96 per row in Q and the last 5 in the
+// dense tail array instead of 112-bin rows; x length 256: 129 bins, 128 + 1
+// instead of 144; z length 128, 256 or 512, or 1024 for x length 256), so no
+// pass moves the padding bins; otherwise rows padded to N2s bins.
 #ifndef TM_SPLIT_ROWS
 #define TM_SPLIT_ROWS 1
 #endif
+// N2m is the row stride of Q: the main bins, plus 16 unused bins where they
+// are a multiple of 64 (with a 1 KB stride the 1024-point pass B over 64-byte
+// segments ran 27.3 ms, with 1088 B 28.5 ms: lines split between rows).
 static inline void row_layout(int64_t N0, int64_t N1, int64_t N2, int64_t& N2m, int64_t& ntl) {
   const int64_t N2c = N2 / 2 + 1;
-  const bool split = TM_SPLIT_ROWS && N2 == 200 && (N0 == 128 || N0 == 256 || N0 == 512) && N1 >= 16;
-  N2m = split ? N2c / 16 * 16 : pad_bins(N2c);
-  ntl = split ? N2c - N2m : 0;
+  const bool z_ct = N0 == 128 || N0 == 256 || N0 == 512;
+  const bool split = TM_SPLIT_ROWS && N1 >= 16 && ((N2 == 200 && z_ct) || (N2 == 256 && (z_ct || N0 == 1024)));
+  const int64_t nm = N2c / 16 * 16;
+  N2m = split ? (nm % 64 == 0 ? nm + 16 : nm) : pad_bins(N2c);
+  ntl = split ? N2c - nm : 0;
 }
 
 // Copy the caller's target spectra (rows of N2c bins) into the padded copies.
@@ -1379,7 +1388,8 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   if (p->ntl) {
     // split rows: the main rows (whole 16-bin tiles), then the tail array
     // (lines (z, c) of N0 * ntl, whole 128-byte segments)
-    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, p->N2m, p->N2m, 1, 1.0f, s, "fused_pass_b");
+    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c - p->ntl, p->N2m, 1, 1.0f, s,
+                       "fused_pass_b");
     if (rc != TM_OK) return rc;
     rc = tm_launch_c2c(ly, p->Qt, N1, p->Qt, (int64_t)p->nq_out * B, (int64_t)N0 * p->ntl, (int64_t)N0 * p->ntl, 1,
                        1.0f, s, "fused_pass_b_tail");
@@ -1411,7 +1421,12 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
           rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         }
         break;
-      case 256: rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 256:
+        if (p->ntl)  // split rows: 128 bins in Q, the Nyquist bin in Qt
+          rc = launch_pass_c_ct<128, 16, 256, 1, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        else
+          rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+        break;
       default: done_c = false; break;
     }
     if (done_c && rc != TM_OK) return rc;

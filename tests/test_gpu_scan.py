@@ -124,12 +124,13 @@ def test_filtered_parity(score, fused):
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
 
 
-@pytest.mark.parametrize("shape", [(128, 128, 200), (128, 128, 128)])
+@pytest.mark.parametrize("shape", [(128, 128, 200), (128, 128, 128), (128, 64, 256)])
 @pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
 @pytest.mark.parametrize("fused", [False, True])
 def test_specialised_sizes_parity(shape, score, fused):
     """Tomogram sizes that take the compile-time FFT kernels (512/128-point
-    axes, half-length 200/128-point C2R) against the FP64 oracle."""
+    axes, half-length 200/128-point C2R) against the FP64 oracle; x lengths
+    200 and 256 run the fused scan on split rows (96 + 5 and 128 + 1 bins)."""
     target, template, mask, rots = _workload(seed=31, shape=shape, T=16, n_rot=4, copies=4)
     if score == "FLCSphericalMask":
         from pytme_amd.synthetic import spherical_mask
