@@ -26,6 +26,7 @@
 // The target spectra are read once per batch instead of once per rotation
 // and the score volume never exists in HBM.
 #include <math.h>
+#include <algorithm>
 #include <stdlib.h>
 #include <string.h>
 
@@ -429,12 +430,15 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
     }
   };
 
-  load_in(0, x);
+  // rotations of this block: all of them, or a share of them where the
+  // launch splits the batch over gridDim.y (the few tail blocks)
+  const int b0 = (int)((int64_t)blockIdx.y * B / gridDim.y), b1 = (int)((int64_t)(blockIdx.y + 1) * B / gridDim.y);
+  load_in(b0, x);
   ct_ready<EO>(F);
   if constexpr (KIND == 0) ct_ready<EO>(F2);
   ct_ready<NX>(x);
-  for (int b = 0; b < B; ++b) {
-    load_in(b + 1 < B ? b + 1 : b, xn);  // the last rotation reloads itself, keeping the loop unpredicated
+  for (int b = b0; b < b1; ++b) {
+    load_in(b + 1 < b1 ? b + 1 : b, xn);  // the last rotation reloads itself, keeping the loop unpredicated
     float2 w[EW];
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
@@ -683,6 +687,8 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   const size_t shm = (size_t)N0 * L * sizeof(float2);
   const double fm = ntl > 0 ? (double)nm / (nm + ntl) : 1.0;  // the main launch's share of the bytes
   const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
+  // the tail blocks are fewer than the CUs: split their rotations over gridDim.y
+  const unsigned tail_split = ntl > 0 ? (unsigned)std::min<int64_t>(B, std::max<int64_t>(1, tm_ceil_div(512, tail_blocks))) : 1;
   if (flc) {
     TM_TIMED("fused_pass_a", bytes * 0.6 * fm, s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
@@ -690,7 +696,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
     TM_CHECK_LAUNCH();
     if (ntl > 0) {
       TM_TIMED("fused_pass_a_tail", bytes * 0.6 * (1.0 - fm), s,
-               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true>), dim3(tail_blocks), dim3(NT0), shm, s, P,
+               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true>), dim3(tail_blocks, tail_split), dim3(NT0), shm, s, P,
                                   ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 0));
       TM_CHECK_LAUNCH();
     }
@@ -702,7 +708,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   TM_CHECK_LAUNCH();
   if (ntl > 0) {
     TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", (flc ? bytes * 0.4 : bytes) * (1.0 - fm), s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true>), dim3(tail_blocks), dim3(NT), shm, s, P, ft,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true>), dim3(tail_blocks, tail_split), dim3(NT), shm, s, P, ft,
                                 (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
                                 flc ? 2 : 0));
     TM_CHECK_LAUNCH();
