@@ -30,17 +30,29 @@ FP32_PEAK_GFLOPS = 157300.0  # MI355X FP32 (the kernels issue non-packed FP32: h
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48_b64.md")
 
 
+#: kernels whose per-launch traffic is proportional to the batch (their
+#: work is 3 spectra x B rotations of the same lines), so the profiled
+#: figure at 64 rotations scales to the autotuner's other batch sizes
+TRAFFIC_SCALES_WITH_BATCH = ("fused_pass_b", "fused_pass_b_tail")
+PROFILED_BATCH = 64
+
+
 def measured_traffic(kernel, workload, batch, path):
-    """HBM bytes per launch of `kernel` from the committed counter summary,
-    or None when the run is not the profiled configuration."""
-    if workload != "flc_512x512x200_t48" or batch != 64 or path != "fused":
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiled at 64 rotations per batch; scaled by batch / 64 for the
+    kernels whose traffic is proportional to the batch), or None when the
+    run is not the profiled workload."""
+    if workload != "flc_512x512x200_t48" or path != "fused":
+        return None
+    if batch != PROFILED_BATCH and kernel not in TRAFFIC_SCALES_WITH_BATCH:
         return None
     try:
         with open(TRAFFIC_PROFILE) as fh:
             for line in fh:
                 line = line.strip()
                 if line.startswith("traffic_bytes_per_launch = "):
-                    return json.loads(line.split("=", 1)[1]).get(kernel)
+                    v = json.loads(line.split("=", 1)[1]).get(kernel)
+                    return None if v is None else int(round(v * batch / PROFILED_BATCH))
     except OSError:
         return None
     return None
@@ -547,7 +559,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(name, args.workload, batch, args.path),
-            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + (
+                "" if batch == PROFILED_BATCH else f" (profiled at {PROFILED_BATCH} rotations, scaled by {batch}/{PROFILED_BATCH})"),
             "launches": count,
             "avg_launch_us": round(ms * 1e3 / max(count, 1), 2),
             "bytes_per_launch": by / max(count, 1),
