@@ -148,3 +148,99 @@ class TestPadding:
         conv, fast, ft, shift = orc.fourier_padding((128,) * 3, (32,) * 3)
         assert shift == (-15, -15, -15)
         assert orc.fourier_padding((80,) * 3, (41, 41, 35))[3] == (-20, -20, -17)
+
+
+def _kernel(shape, taps):
+    k = np.zeros(shape)
+    for idx, v in taps:
+        k[idx] = v
+    return k
+
+
+# one-voxel shift, an anisotropic 3-tap blur, and an asymmetric mixture: each
+# filter is the rfft half spectrum of the kernel at the array's shape
+_KERNELS = {
+    "shift": [((1, 0, 0), 1.0)],
+    "blur": [((0, 0, 0), 0.5), ((0, 0, 1), 0.25), ((0, 0, -1), 0.25)],
+    "mixed": [((0, 0, 0), 0.6), ((1, 0, 0), 0.3), ((0, -1, 2), -0.2), ((-2, 1, 0), 0.1)],
+}
+
+
+class TestFilterStep:
+    """The oracle's Fourier-space filter steps against direct circular
+    convolution (tme/matching_scores.py:121-135 template-size filter,
+    tme/matching_exhaustive.py:106-117 target filter)."""
+
+    @pytest.mark.parametrize("name", sorted(_KERNELS))
+    @pytest.mark.parametrize("shape", [(6, 6, 6), (5, 6, 7), (7, 4, 5)])
+    def test_template_filter_is_circular_convolution(self, name, shape):
+        rng = np.random.default_rng(21)
+        t = rng.standard_normal(shape)
+        k = _kernel(shape, _KERNELS[name])
+        filt = np.fft.rfftn(k)  # half spectrum along the last axis
+        orc_ = orc.ScanOracle("CC", rng.standard_normal((12, 12, 12)), t, template_filter=filt)
+        assert np.abs(orc_._apply_filter(t) - bf.circular_convolve(t, k)).max() < 1e-12
+
+    def test_all_ones_filter_is_identity(self):
+        t = np.random.default_rng(22).standard_normal((5, 6, 7))
+        orc_ = orc.ScanOracle("CC", np.zeros((9, 9, 9)), t, template_filter=np.ones((5, 6, 4)))
+        assert np.abs(orc_._apply_filter(t) - t).max() < 1e-12
+        # a size-1 filter is the reference's "no filter" (tme/matching_scores.py:90-91)
+        assert orc.ScanOracle("CC", np.zeros((9, 9, 9)), t, template_filter=np.ones(1)).filter is None
+
+    def test_kernel_flip_is_not_a_no_op(self):
+        # the reversed-frame kernel differs from the kernel for asymmetric
+        # filters, so the filtered-scan pin below tells a filter applied in
+        # the reversed frame from one applied in the user's frame
+        rng = np.random.default_rng(23)
+        shape = (5, 6, 7)
+        t = rng.standard_normal(shape)
+        k = _kernel(shape, _KERNELS["mixed"])
+        assert np.abs(bf.circular_convolve(t, bf.flip_kernel(k)) - bf.circular_convolve(t, k)).max() > 0.1
+        assert np.array_equal(bf.flip_kernel(bf.flip_kernel(k)), k)
+
+    @pytest.mark.parametrize("name", sorted(_KERNELS))
+    @pytest.mark.parametrize("tshape", [(13, 11, 9), (12, 10, 16)])
+    def test_target_filter_at_fast_shape_then_crop(self, name, tshape):
+        rng = np.random.default_rng(24)
+        target = rng.standard_normal(tshape)
+        template = rng.standard_normal((4, 5, 3))
+        _, fast, _, _ = orc.fourier_padding(tshape, template.shape)
+        k = _kernel(fast, _KERNELS[name])
+        o = orc.ScanOracle("CC", target, template, target_filter=np.fft.rfftn(k))
+        got = np.fft.irfftn(o.ft_target, s=fast)
+        want = bf.filtered_target(target, k, fast)
+        assert np.abs(got[tuple(slice(0, n) for n in tshape)] - want).max() < 1e-12
+        # the filtered target is cropped back and zero-extended again
+        outside = ~np.pad(np.ones(tshape, bool), [(0, f - n) for f, n in zip(fast, tshape)])
+        assert np.all(np.abs(got[outside]) < 1e-12)
+
+    @pytest.mark.parametrize("score", ["CC", "FLCSphericalMask", "FLC"])
+    @pytest.mark.parametrize("name", ["shift", "mixed"])
+    def test_filtered_scan_vs_bruteforce(self, score, name):
+        rng = np.random.default_rng(25)
+        target = rng.standard_normal((11, 10, 9))
+        tshape = (5, 6, 4)
+        template = rng.standard_normal(tshape)
+        mask = (rng.random(tshape) > 0.3).astype(float)
+        _, fast, _, _ = orc.fourier_padding(target.shape, tshape)
+        kt = _kernel(tshape, _KERNELS[name])
+        kf = _kernel(fast, _KERNELS["blur"])
+        r = _rot(31)
+        s, _ = orc.scan(score, target, template, r[None], template_mask=mask, template_filter=np.fft.rfftn(kt),
+                        target_filter=np.fft.rfftn(kf), score_threshold=-1e9)
+        b = bf.local_correlation(bf.filtered_target(target, kf, fast), template, mask, r, score=score,
+                                 template_kernel=bf.flip_kernel(kt))
+        assert np.abs(s - b).max() < 1e-10
+
+    def test_filter_after_rotation_matters(self):
+        # filtering before the rotation (a plausible mistake) changes the scores
+        rng = np.random.default_rng(26)
+        target = rng.standard_normal((11, 10, 9))
+        template = rng.standard_normal((5, 6, 4))
+        kt = _kernel(template.shape, _KERNELS["mixed"])
+        r = _rot(32)
+        s, _ = orc.scan("CC", target, template, r[None], template_filter=np.fft.rfftn(kt), score_threshold=-1e9)
+        pre = bf.circular_convolve(template, kt)
+        b = bf.local_correlation(target, pre, None, r, score="CC")
+        assert np.abs(s - b).max() > 1e-3
