@@ -31,31 +31,43 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48
 
 
 #: kernels whose per-launch traffic is proportional to the batch (their
-#: work is 3 spectra x B rotations of the same lines), so the profiled
-#: figure at 64 rotations scales to the autotuner's other batch sizes
+#: work is 3 spectra x B rotations of the same lines), so a figure profiled
+#: at another batch size scales to the autotuner's choice as an estimate
 TRAFFIC_SCALES_WITH_BATCH = ("fused_pass_b", "fused_pass_b_tail")
-PROFILED_BATCH = 64
 
 
-def measured_traffic(kernel, workload, batch, path):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiled at 64 rotations per batch; scaled by batch / 64 for the
-    kernels whose traffic is proportional to the batch), or None when the
-    run is not the profiled workload."""
-    if workload != "flc_512x512x200_t48" or path != "fused":
-        return None
-    if batch != PROFILED_BATCH and kernel not in TRAFFIC_SCALES_WITH_BATCH:
-        return None
+def _profiled_traffic():
+    """{batch: {kernel: HBM bytes per launch}} from the committed counter
+    summary (lines ``traffic_bytes_per_launch@B = {json}``)."""
+    out = {}
     try:
         with open(TRAFFIC_PROFILE) as fh:
             for line in fh:
                 line = line.strip()
-                if line.startswith("traffic_bytes_per_launch = "):
-                    v = json.loads(line.split("=", 1)[1]).get(kernel)
-                    return None if v is None else int(round(v * batch / PROFILED_BATCH))
-    except OSError:
-        return None
-    return None
+                if line.startswith("traffic_bytes_per_launch@"):
+                    head, body = line.split("=", 1)
+                    out[int(head.split("@", 1)[1])] = json.loads(body)
+    except (OSError, ValueError):
+        return {}
+    return out
+
+
+def measured_traffic(kernel, workload, batch, path):
+    """(measured, estimated, profiled_batch): the HBM bytes per launch of
+    `kernel` from the committed counter summary when it was profiled at this
+    batch size; otherwise None and, for kernels whose traffic is
+    proportional to the batch, an estimate scaled from the nearest profiled
+    batch."""
+    if workload != "flc_512x512x200_t48" or path != "fused":
+        return None, None, None
+    prof = _profiled_traffic()
+    if batch in prof and kernel in prof[batch]:
+        return int(prof[batch][kernel]), None, batch
+    have = [b for b in prof if kernel in prof[b]]
+    if not have or kernel not in TRAFFIC_SCALES_WITH_BATCH:
+        return None, None, None
+    b0 = min(have, key=lambda b: abs(b - batch))
+    return None, int(round(prof[b0][kernel] * batch / b0)), b0
 
 
 def parse():
@@ -69,7 +81,8 @@ def parse():
     p.add_argument("--copies", type=int, default=None)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-rotations", type=int, default=None, help="rotations in the CPU sample")
-    p.add_argument("--compare-plain", action="store_true", help="also time the plain per-rotation path")
+    p.add_argument("--no-compare-plain", action="store_true",
+                   help="skip timing the plain per-rotation path on the same GPU")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a 1 GiB device copy first (reference dispatch for HBM byte counters)")
     p.add_argument("--no-refinement", action="store_true", help="skip the gradient-refinement measurement")
@@ -326,36 +339,28 @@ def fft_flops(score, F, rotations_per_s):
             "unit": "GFLOP/s", "peak": FP32_PEAK_GFLOPS, "frac": round(achieved / FP32_PEAK_GFLOPS, 4)}
 
 
-def bench_multi_template(target, spec, n_rot, steps):
-    """The multi-template schedule: the same tomogram against four templates
-    of uneven size (24^3, 32^3, 48^3, 64^3 blob densities, thresholded and
-    dilated masks, wedge x CTF filters at defocus 2.0 / 2.5 / 3.0 / 3.5 um).
-    Each template gets its own fused plan with an autotuned rotation batch
-    and graph replay; per template, ``steps`` batches of the public scoring
-    function are timed after one warm-up batch (setup outside the timed
-    region, as for the main record).  ``value`` is the schedule's
-    rotations/s: all timed rotations over the sum of the timed regions."""
-    from pytme_amd.filters import Compose, CTFReconstructed, WedgeReconstructed
-    from pytme_amd.rotations import uniform_rotations
-    from pytme_amd.synthetic import blob_template, threshold_mask
+def bench_multi_template(n_rot, steps):
+    """The multi-template schedule on its own workload: one 512x512x200
+    tomogram with 50 embedded copies of each of four templates of uneven
+    size (24^3, 32^3, 48^3, 64^3 blob densities, thresholded and dilated
+    masks, wedge x CTF filters at defocus 2.0 / 2.5 / 3.0 / 3.5 um,
+    continuous-wedge target filter).  Each template gets its own fused plan
+    with an autotuned rotation batch and graph replay; per template,
+    ``steps`` batches of the public scoring function are timed after one
+    warm-up batch (setup outside the timed region, as for the main record).
+    ``value`` is the schedule's rotations/s: all timed rotations over the
+    sum of the timed regions."""
+    from pytme_amd.synthetic import make_multi_workload
 
-    lo, hi, step = spec["tilt"]
-    angles = np.arange(lo, hi + step / 2, step, dtype=np.float64)
+    target, entries, mspec = make_multi_workload(n_rotations=n_rot)
+    spec = {"score": mspec["score"]}
     per, total_rot, total_s = {}, 0, 0.0
-    entries = []
-    for k, (T, defocus) in enumerate(((24, 20000.0), (32, 25000.0), (48, 30000.0), (64, 35000.0))):
-        tmpl = blob_template(T, seed=2000 + k)
-        m = threshold_mask(tmpl)
-        wedge = WedgeReconstructed(angles=angles, weight_wedge=False, create_continuous_wedge=False,
-                                   opening_axis=2, tilt_axis=0)
-        ctf = CTFReconstructed(defocus_x=defocus, sampling_rate=spec["sampling"],
-                               acceleration_voltage=spec["voltage"], spherical_aberration=spec["cs"],
-                               amplitude_contrast=spec["amplitude_contrast"], flip_phase=True)
-        target_wedge = WedgeReconstructed(angles=(lo, hi), create_continuous_wedge=True, weight_wedge=False,
-                                          opening_axis=2, tilt_axis=0)
-        filters = (Compose((wedge, ctf)), Compose((target_wedge,)))
-        rots = uniform_rotations(n_rot, seed=2100 + k, include_identity=True)
-        entries.append((tmpl, m, filters, rots))
+    group_entries = []
+    for entry in entries:
+        tmpl, m, rots = entry["template"], entry["template_mask"], entry["rotations"]
+        filters = (entry["template_filter"], entry["target_filter"])
+        group_entries.append((tmpl, m, filters, rots))
+        T = tmpl.shape[0]
         t0 = time.perf_counter()
         scoring, setup, proxy, rot_dev, batch, tuned = build_scan(spec, target, tmpl, m, filters, rots, "fused",
                                                                   None)
@@ -364,6 +369,7 @@ def bench_multi_template(target, spec, n_rot, steps):
         per[f"{T}^3"] = {
             "rotations_per_s": round(steps * batch / elapsed, 2),
             "batch": batch,
+            "defocus_A": entry["defocus"],
             "autotune_ms_per_rotation": None if tuned is None else tuned["ms_per_rotation"],
             "tiles": None if tuned is None else list(tuned["tiles"]),
             "setup_s": round(setup_s, 3),
@@ -375,11 +381,12 @@ def bench_multi_template(target, spec, n_rot, steps):
         "metric": "FLC rotations/s of the multi-template schedule (4 templates, 512x512x200 tomogram)",
         "value": round(total_rot / total_s, 2),
         "unit": "rotations/s",
+        "workload": "flc_multi_512x512x200 (50 copies of each template at SNR 0.1)",
         "templates": per,
         "steps_per_template": steps,
         "note": "one fused plan per template, batches replayed as graphs; setup_s (target and filter "
                 "setup, plan creation, batch autotuning) is outside the timed region",
-        "one_graph_schedule": bench_group_schedule(target, spec, entries, steps),
+        "one_graph_schedule": bench_group_schedule(target, spec, group_entries, steps),
     }
 
 
@@ -551,6 +558,7 @@ def main():
     if dominant:
         name, (ms, count, by) = dominant
         achieved = by / (ms * 1e-3) / 1e9
+        measured, estimated, prof_batch = measured_traffic(name, args.workload, batch, args.path)
         roofline = {
             "bound": "hbm",
             "kernel": name,
@@ -558,13 +566,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(name, args.workload, batch, args.path),
-            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + (
-                "" if batch == PROFILED_BATCH else f" (profiled at {PROFILED_BATCH} rotations, scaled by {batch}/{PROFILED_BATCH})"),
+            "traffic": measured,
+            "traffic_source": None if measured is None else
+            f"{os.path.relpath(TRAFFIC_PROFILE, ROOT)} (rocprofv3 FETCH_SIZE + WRITE_SIZE at {prof_batch} rotations)",
             "launches": count,
             "avg_launch_us": round(ms * 1e3 / max(count, 1), 2),
             "bytes_per_launch": by / max(count, 1),
         }
+        if estimated is not None:
+            roofline["traffic_estimated"] = estimated
+            roofline["traffic_scaled_from_batch"] = prof_batch
     kernels = {k: {"ms": round(v[0], 3), "launches": v[1], "GB/s": round(v[2] / max(v[0], 1e-9) / 1e6, 1)}
                for k, v in sorted(report.items(), key=lambda kv: -kv[1][0])} if report else {}
 
@@ -575,13 +586,20 @@ def main():
         # backend's scan_graph option is on)
         g_el, _ = time_steps(scoring, setup, proxy, rot_dev, batch, args.steps, 1)
         extra["untimed_kernels_rotations_per_s"] = round(n_rot / g_el, 2)
-    if args.compare_plain and args.path == "fused":
+    if not args.no_compare_plain and args.path == "fused":
+        # the per-rotation loop (flc_scoring without the fused hook) on the
+        # same GPU kernels, same workload and analyzer state
         from pytme_amd.backends import backend as be
 
         be.change_backend("hip", device="cuda", fused=False)
-        p_steps = max(2, args.steps // 4)
+        p_steps = max(4, args.steps // 2)
         p_el, _ = time_steps(scoring, setup, proxy, rot_dev, 4, p_steps, 1)
-        extra["plain_path_rotations_per_s"] = round(p_steps * 4 / p_el, 2)
+        plain = p_steps * 4 / p_el
+        extra["plain_path"] = {"rotations_per_s": round(plain, 2), "rotations_timed": p_steps * 4,
+                               "fused_over_plain": round(value / plain, 3)}
+        extra["plain_path_rotations_per_s"] = round(plain, 2)
+        be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch,
+                          scan_tiles=None if tuned is None else tuned["tiles"])
 
     out = {
         "metric": "FLC rotations/s, 512x512x200 tomogram vs 48^3 template" if args.workload.startswith("flc_512")
@@ -624,7 +642,7 @@ def main():
         out["refinement"] = bench_refinement(target, template, mask, spec, args.refine_peaks, args.refine_steps,
                                              cpu_refine)
     if not args.no_multi_template and args.workload.startswith("flc_512") and args.path == "fused":
-        out["multi_template"] = bench_multi_template(target, spec, 512, 4)
+        out["multi_template"] = bench_multi_template(512, 4)
     out.update(extra)
     print(json.dumps(out), flush=True)
 

@@ -166,7 +166,15 @@ def check(status: int, what: str = "") -> None:
     raise RuntimeError(msg)
 
 
+#: number of entry-point calls made through :func:`call`; a caller holding
+#: a device copy of some array compares it to tell whether any library call
+#: (which may have written that array) ran in between
+CALLS = 0
+
+
 def call(name: str, *args) -> None:
+    global CALLS
+    CALLS += 1
     check(getattr(lib(), name)(*args), name)
 
 

@@ -196,7 +196,11 @@ class HipBackend(MatchingBackend):
         return int
 
     def free_cache(self):
+        """Release the kept fused-scan plan and torch's cached blocks."""
         if self._gpu:
+            from . import hip_fused
+
+            hip_fused.release()
             self._torch.cuda.empty_cache()
 
     # ------------------------------------------------------------- arithmetic

@@ -24,9 +24,12 @@ import numpy as np
 
 from .. import _lib
 
-__all__ = ["run", "choose_batch", "autotune_batch", "TUNED"]
+__all__ = ["run", "choose_batch", "autotune_batch", "release", "invalidate", "TUNED"]
 
-_PLAN = {"key": None, "handle": None, "keep": None}
+#: the kept plan: ``key`` identifies its arguments, ``base`` the same without
+#: the batch size, ``keep`` holds its input arrays, ``seen`` the state of the
+#: target spectra when the plan last copied them
+_PLAN = {"key": None, "base": None, "handle": None, "keep": None, "seen": None}
 
 #: measured batch choices: (device, mode, fast_shape, template_shape, filter,
 #: normalize, candidates) -> {"batch": B, "ms_per_rotation": {B: ms}}
@@ -36,7 +39,30 @@ TUNED = {}
 def _free_plan():
     if _PLAN["handle"] is not None:
         _lib.call("tm_scan_plan_destroy", _PLAN["handle"])
-    _PLAN.update(key=None, handle=None, keep=None)
+    _PLAN.update(key=None, base=None, handle=None, keep=None, seen=None)
+
+
+def release():
+    """Free the kept fused plan (its batch workspace: tens of GB for a
+    512x512x200 target).  The hip backend's ``free_cache`` calls this, so
+    :func:`~pytme_amd.matching_exhaustive.scan` releases it when a scan
+    ends."""
+    _free_plan()
+
+
+def invalidate():
+    """Make the kept plan copy the target spectra again at its next batch.
+
+    The plan keeps a row-padded copy of the spectra it was created with and
+    refreshes it when their tensors' version counters changed or any
+    library call ran since its last batch.  Writes through another library
+    (raw pointers that neither torch nor this library sees) need this call.
+    """
+    _PLAN["seen"] = None
+
+
+def _spectra_state(ft_target, ft2):
+    return (_lib.CALLS, ft_target._version, None if ft2 is None else ft2._version)
 
 
 def choose_batch(fast_shape, template_shape, mode: int, with_filter: bool, free_bytes: int,
@@ -68,11 +94,15 @@ def choose_batch(fast_shape, template_shape, mode: int, with_filter: bool, free_
 TILE_CANDIDATES = ((0, 0), (8, 0), (0, 10), (8, 10))
 
 
-def _time_plan(t, handle, stream, rots, idx_dev, B, scores, rot):
-    """ms per rotation of one warm-up and two timed batches on scratch state."""
+def _time_plan(t, handle, stream, rots, idx_dev, B, scores, rot, graph):
+    """ms per rotation of two timed batches on scratch state, after the
+    warm-up batches that settle the launch mode production uses (with graph
+    replay: the eager first batch and the captured second one)."""
+    _lib.call("tm_scan_plan_use_graph", handle, int(graph))
+    warm = 2 if graph else 1
     start, stop = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
-    for rep in range(3):
-        if rep == 1:
+    for rep in range(warm + 2):
+        if rep == warm:
             start.record()
         _lib.call("tm_scan_batch", handle, rots[0].data_ptr(), idx_dev[0].data_ptr(), B, scores.data_ptr(),
                   rot.data_ptr(), stream)
@@ -85,15 +115,17 @@ def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, can
     """Time each candidate batch size, then the tile shapes of the passes
     (``TILE_CANDIDATES``, ``tm_scan_plan_set_tiles``) at the fastest size,
     on scratch max/argmax state (the caller's state is not touched) with the
-    call's own rotations: one warm-up batch and two timed batches each, HIP
-    events on the launch stream; keep the fastest per rotation.  ``create(B)``
+    call's own rotations: two timed batches each after the warm-up, in the
+    backend's launch mode (graph replay or eager), HIP events on the launch
+    stream; keep the fastest per rotation.  ``create(B)``
     returns a plan handle for batch B.  Results are cached in :data:`TUNED`
     (``batch`` and ``tiles``)."""
     t = backend._torch
     if key in TUNED:
         return TUNED[key]["batch"]
-    scores = t.empty(state_shape, dtype=t.float32, device=rots.device)
-    rot = t.empty(state_shape, dtype=t.int32, device=rots.device)
+    graph = bool(getattr(backend, "scan_graph", True))
+    scores = t.zeros(state_shape, dtype=t.float32, device=rots.device)
+    rot = t.full(state_shape, -1, dtype=t.int32, device=rots.device)
     n = int(rots.shape[0])
     timings = {}
     for B in candidates:
@@ -101,7 +133,7 @@ def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, can
             continue
         handle = create(B)
         try:
-            timings[B] = _time_plan(t, handle, stream, rots, idx_dev, B, scores, rot)
+            timings[B] = _time_plan(t, handle, stream, rots, idx_dev, B, scores, rot, graph)
         finally:
             _lib.call("tm_scan_plan_destroy", handle)
     best = min(timings, key=timings.get) if timings else min(candidates)
@@ -115,7 +147,7 @@ def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, can
                     _lib.call("tm_scan_plan_set_tiles", handle, *combo)
                 except ValueError:
                     continue  # not a tiling of this plan
-                tiles[combo] = _time_plan(t, handle, stream, rots, idx_dev, best, scores, rot)
+                tiles[combo] = _time_plan(t, handle, stream, rots, idx_dev, best, scores, rot, graph)
             finally:
                 _lib.call("tm_scan_plan_destroy", handle)
     best_tiles = min(tiles, key=tiles.get) if tiles else TILE_CANDIDATES[0]
@@ -237,11 +269,17 @@ def run(backend, score: str, **kw):
     max_scores, max_rot = job["max_scores"], job["max_rot"]
     n = int(rots.shape[0])
 
+    keep = (templ, mask, filt, ft_target, ft2, inv_den)
+    base = (
+        t.cuda.current_device(), mode, fast_shape, tshape, normalize, mask_sum,
+        tuple(0 if x is None else x.data_ptr() for x in keep), bool(getattr(backend, "scan_graph", True)),
+    )
+    if _PLAN["base"] != base:
+        _free_plan()  # before measuring the free memory the new plan may use
     free = backend.get_available_memory()
     auto = backend.rotation_batch == "auto"
     B = choose_batch(fast_shape, tshape, mode, filt is not None, free, None if auto else backend.rotation_batch, n)
 
-    keep = (templ, mask, filt, ft_target, ft2, inv_den)
     ptr = _ptr
     stream = backend._stream()
 
@@ -256,10 +294,7 @@ def run(backend, score: str, **kw):
         if tune_key not in TUNED:
             _free_plan()  # the candidates' workspaces need the memory
         B = autotune_batch(backend, tune_key, create, stream, rots, idx_dev, fast_shape, candidates)
-    key = (
-        t.cuda.current_device(), mode, fast_shape, tshape, B, normalize, mask_sum,
-        tuple(0 if x is None else x.data_ptr() for x in keep), bool(getattr(backend, "scan_graph", True)),
-    )
+    key = base + (B,)
     # tile shapes: the backend's, the autotuner's, or the kernels' defaults
     tiles = getattr(backend, "scan_tiles", None)
     if tiles is None and auto:
@@ -267,14 +302,14 @@ def run(backend, score: str, **kw):
     tiles = (0, 0) if tiles is None else tiles
     if _PLAN["key"] != key:
         _free_plan()
-        _PLAN.update(key=key, handle=create(B), keep=keep)
+        _PLAN.update(key=key, base=base, handle=create(B), keep=keep)
         _lib.call("tm_scan_plan_use_graph", _PLAN["handle"], int(getattr(backend, "scan_graph", True)))
-        _lib.call("tm_scan_plan_set_tiles", _PLAN["handle"], *tiles)
-    else:
-        _lib.call("tm_scan_plan_set_tiles", _PLAN["handle"], *tiles)
-        # the plan holds a row-padded copy of the target spectra: refresh it,
-        # the caller's arrays may hold new values at the same addresses
+    elif _PLAN["seen"] != _spectra_state(ft_target, ft2):
+        # the plan holds a row-padded copy of the target spectra: refresh it
+        # when the caller's arrays may hold new values (a torch write bumps
+        # their version counters; a library call may have written them)
         _lib.call("tm_scan_plan_set_target", _PLAN["handle"], ptr(ft_target), ptr(ft2), stream)
+    _lib.call("tm_scan_plan_set_tiles", _PLAN["handle"], *tiles)
 
     for start in range(0, n, B):
         nb = min(B, n - start)
@@ -282,6 +317,7 @@ def run(backend, score: str, **kw):
             "tm_scan_batch", _PLAN["handle"], rots[start].data_ptr(), idx_dev[start].data_ptr(), nb,
             max_scores.data_ptr(), max_rot.data_ptr(), stream,
         )
+    _PLAN["seen"] = _spectra_state(ft_target, ft2)
     return kw["callback"]
 
 

@@ -14,7 +14,15 @@ from scipy.ndimage import affine_transform, binary_dilation
 from .matching_utils import elliptical_mask
 from .rotations import uniform_rotations
 
-__all__ = ["blob_template", "threshold_mask", "spherical_mask", "embed_copies", "make_workload"]
+__all__ = [
+    "blob_template",
+    "threshold_mask",
+    "spherical_mask",
+    "add_copies",
+    "embed_copies",
+    "make_workload",
+    "make_multi_workload",
+]
 
 
 def blob_template(size: int, n_blobs: int = 20, seed: int = 0) -> np.ndarray:
@@ -54,6 +62,37 @@ def _rotate(arr: np.ndarray, rot: np.ndarray) -> np.ndarray:
     return affine_transform(arr, rinv, offset=c - rinv @ c, order=1, mode="constant")
 
 
+def add_copies(
+    target: np.ndarray,
+    template: np.ndarray,
+    mask: np.ndarray,
+    n_copies: int,
+    snr: float,
+    rng: np.random.Generator,
+    rotation_seed: int,
+) -> Tuple[np.ndarray, np.ndarray]:
+    """Add ``n_copies`` rotated copies of ``template`` to ``target`` in place.
+
+    Copies are scaled so that var(signal inside the mask) = ``snr`` (the
+    noise has unit variance); positions are at least T/2 from the border.
+    Returns the copy centres (n, 3) and rotations (n, 3, 3).
+    """
+    t = template.astype(np.float64)
+    sel = mask > 0
+    scale = np.sqrt(snr / max(t[sel].var(), 1e-12))
+    rots = uniform_rotations(n_copies, seed=rotation_seed)
+    T = np.asarray(template.shape)
+    lo = T // 2
+    hi = np.asarray(target.shape) - (T - T // 2)
+    centres = np.stack([rng.integers(lo[k], hi[k] + 1, size=n_copies) for k in range(3)], axis=1)
+    for i in range(n_copies):
+        cp = (_rotate(t, rots[i]) * scale).astype(np.float32)
+        start = centres[i] - T // 2
+        box = tuple(slice(int(s), int(s) + int(n)) for s, n in zip(start, T))
+        target[box] += cp
+    return centres, rots
+
+
 def embed_copies(
     shape: Tuple[int],
     template: np.ndarray,
@@ -70,19 +109,7 @@ def embed_copies(
     """
     rng = np.random.default_rng(seed)
     target = rng.standard_normal(shape, dtype=np.float32)
-    t = template.astype(np.float64)
-    sel = mask > 0
-    scale = np.sqrt(snr / max(t[sel].var(), 1e-12))
-    rots = uniform_rotations(n_copies, seed=seed + 17)
-    T = np.asarray(template.shape)
-    lo = T // 2
-    hi = np.asarray(shape) - (T - T // 2)
-    centres = np.stack([rng.integers(lo[k], hi[k] + 1, size=n_copies) for k in range(3)], axis=1)
-    for i in range(n_copies):
-        cp = (_rotate(t, rots[i]) * scale).astype(np.float32)
-        start = centres[i] - T // 2
-        box = tuple(slice(int(s), int(s) + int(n)) for s, n in zip(start, T))
-        target[box] += cp
+    centres, rots = add_copies(target, template, mask, n_copies, snr, rng, seed + 17)
     return target, centres, rots
 
 
@@ -131,18 +158,69 @@ def make_workload(name: str, shape=None, copies=None, n_rotations=None):
     rotations = uniform_rotations(spec["rotations"], seed=spec["seed"] + 1, include_identity=True)
     filters = (None, None)
     if spec["filters"]:
-        from .filters import Compose, CTFReconstructed, WedgeReconstructed
-
-        lo, hi, step = spec["tilt"]
-        angles = np.arange(lo, hi + step / 2, step, dtype=np.float64)
-        wedge = WedgeReconstructed(angles=angles, weight_wedge=False, create_continuous_wedge=False,
-                                   opening_axis=2, tilt_axis=0)
-        ctf = CTFReconstructed(defocus_x=spec["defocus"], sampling_rate=spec["sampling"],
-                               acceleration_voltage=spec["voltage"], spherical_aberration=spec["cs"],
-                               amplitude_contrast=spec["amplitude_contrast"], flip_phase=True)
-        target_wedge = WedgeReconstructed(angles=(lo, hi), create_continuous_wedge=True, weight_wedge=False,
-                                          opening_axis=2, tilt_axis=0)
-        filters = (Compose((wedge, ctf)), Compose((target_wedge,)))
+        filters = _cryoet_filters(spec, spec["defocus"])
     spec["centres"] = centres
     spec["copy_rotations"] = rots
     return target, template, mask, rotations, filters, spec
+
+
+MULTI_WORKLOADS: Dict[str, Dict] = {
+    # one 512x512x200 tomogram, four templates of uneven size (blob
+    # densities), 50 embedded copies of each at SNR 0.1, each template with
+    # its own thresholded mask and CTF defocus, the step wedge of the
+    # single-template workload; 5,000 rotations per template
+    "flc_multi_512x512x200": dict(
+        shape=(512, 512, 200), templates=((24, 20000.0), (32, 25000.0), (48, 30000.0), (64, 35000.0)), copies=50,
+        snr=0.1, score="FLC", rotations=5000, tilt=(-60, 60, 3), sampling=6.8, voltage=300e3, cs=2.7e7,
+        amplitude_contrast=0.07, seed=1002,
+    ),
+}
+
+
+def _cryoet_filters(spec, defocus):
+    from .filters import Compose, CTFReconstructed, WedgeReconstructed
+
+    lo, hi, step = spec["tilt"]
+    angles = np.arange(lo, hi + step / 2, step, dtype=np.float64)
+    wedge = WedgeReconstructed(angles=angles, weight_wedge=False, create_continuous_wedge=False,
+                               opening_axis=2, tilt_axis=0)
+    ctf = CTFReconstructed(defocus_x=defocus, sampling_rate=spec["sampling"], acceleration_voltage=spec["voltage"],
+                           spherical_aberration=spec["cs"], amplitude_contrast=spec["amplitude_contrast"],
+                           flip_phase=True)
+    target_wedge = WedgeReconstructed(angles=(lo, hi), create_continuous_wedge=True, weight_wedge=False,
+                                      opening_axis=2, tilt_axis=0)
+    return Compose((wedge, ctf)), Compose((target_wedge,))
+
+
+def make_multi_workload(name: str = "flc_multi_512x512x200", shape=None, copies=None, n_rotations=None):
+    """Build (target, entries, spec) for a multi-template workload.
+
+    ``entries`` holds one dict per template with ``template``,
+    ``template_mask``, ``rotations``, ``template_filter`` and
+    ``target_filter`` (the form ``scan_templates`` takes) plus the embedded
+    ``centres`` and ``copy_rotations``.  ``shape``/``copies``/``n_rotations``
+    override the defaults.
+    """
+    spec = dict(MULTI_WORKLOADS[name])
+    if shape is not None:
+        spec["shape"] = tuple(shape)
+    if copies is not None:
+        spec["copies"] = copies
+    if n_rotations is not None:
+        spec["rotations"] = n_rotations
+    rng = np.random.default_rng(spec["seed"])
+    target = rng.standard_normal(spec["shape"], dtype=np.float32)
+    entries = []
+    for k, (T, defocus) in enumerate(spec["templates"]):
+        template = blob_template(T, seed=spec["seed"] + 10 * (k + 1))
+        mask = threshold_mask(template)
+        centres, rots = add_copies(target, template, mask, spec["copies"], spec["snr"], rng,
+                                   spec["seed"] + 10 * (k + 1) + 7)
+        tfilt, gfilt = _cryoet_filters(spec, defocus)
+        entries.append(dict(
+            template=template, template_mask=mask, template_filter=tfilt, target_filter=gfilt,
+            rotations=uniform_rotations(spec["rotations"], seed=spec["seed"] + 10 * (k + 1) + 1,
+                                        include_identity=True),
+            centres=centres, copy_rotations=rots, defocus=defocus,
+        ))
+    return target, entries, spec

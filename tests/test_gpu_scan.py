@@ -185,11 +185,14 @@ def test_c1_full_size_vs_fp64_oracle():
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
 
 
+@pytest.mark.parametrize("writer", ["torch", "library"])
 @pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
-def test_kept_plan_sees_new_target_values(score):
+def test_kept_plan_sees_new_target_values(score, writer):
     """The fused plan is kept across calls with the same arrays; it holds a
     row-padded copy of the target spectra, so new values written into the
-    caller's arrays at the same addresses must reach the next call."""
+    caller's arrays at the same addresses must reach the next call, whether
+    torch wrote them (version counters) or a library kernel did (the
+    backend's spectral multiply by ones)."""
     import torch
 
     from pytme_amd.analyzer import MaxScoreOverRotations
@@ -228,11 +231,20 @@ def test_kept_plan_sees_new_target_values(score):
         if key.startswith("ft_target") or key == "inv_denominator":
             dst, src = be.from_sharedarr(setup_a[key]), be.from_sharedarr(setup_b[key])
             assert torch.is_tensor(dst) and dst.shape == src.shape
-            dst.copy_(src)
+            if writer == "torch" or not torch.is_complex(dst):
+                dst.copy_(src)
+            else:
+                version = dst._version
+                be.multiply(src, torch.ones_like(src), out=dst)
+                assert dst._version == version  # invisible to torch
             copied += 1
     assert copied == 2
     torch.cuda.synchronize()
+    from pytme_amd.backends import hip_fused
+
+    handle = hip_fused._PLAN["handle"]
     s_a2, r_a2 = run(setup_a, cb_a, chunk_a)  # same arrays and plan, target b's values
+    assert hip_fused._PLAN["handle"] == handle
     assert np.abs(s_a2 - s_b).max() < 1e-6
     assert (r_a2 == r_b).mean() > 0.9999
 
@@ -271,6 +283,91 @@ def test_c4_fused_matches_plain_full_size():
     assert s0.shape == (1024, 1024, 256)
     assert np.abs(s0 - s1).max() < 1e-5
     assert (r0 == r1).mean() > 0.9999
+
+
+def _vs_oracle(target, entry, result, threshold, workers=1):
+    """One template's scan result against the FP64 oracle (template filter
+    at template size, target filter at the fast shape)."""
+    from pytme_amd.matching_data import MatchingData
+
+    template = entry["template"]
+    tfilt = gfilt = None
+    if entry.get("template_filter") is not None:
+        tfilt = entry["template_filter"](shape=template.shape, return_real_fourier=True)["data"]
+    if entry.get("target_filter") is not None:
+        fast = MatchingData._fourier_padding(target.shape, template.shape)[1]
+        gfilt = entry["target_filter"](shape=fast, return_real_fourier=True)["data"]
+    _, _, raws, oc = orc.scan("FLC", target, template, entry["rotations"], template_mask=entry["template_mask"],
+                              score_threshold=threshold, return_raw=True, template_filter=tfilt,
+                              target_filter=gfilt, workers=workers)
+    n_diff, err = assert_scan_parity(result[0], result[2], raws, oc, threshold)
+    print(f"  template {template.shape[0]}^3: max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+def test_c2_full_size_vs_fp64_oracle():
+    """The multi-template workload at its own size: one 512x512x200
+    tomogram with 50 embedded copies of each of four templates (24^3, 32^3,
+    48^3, 64^3 blob densities, own thresholded masks, wedge x CTF at 2.0 /
+    2.5 / 3.0 / 3.5 um defocus, continuous-wedge target filter), all
+    templates' batches in one replayed hipGraph schedule, 2 rotations per
+    template; every template's scores within 1e-4 of the FP64 oracle and
+    identical rotation indices except at ties."""
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.backends import backend as be
+    from pytme_amd.matching_exhaustive import scan_templates
+    from pytme_amd.synthetic import make_multi_workload
+
+    target, entries, spec = make_multi_workload(n_rotations=2)
+    be.change_backend("hip", device="cuda", fused=True)
+    print("C2 full size, 4 templates x 2 rotations, one-graph schedule", flush=True)
+    kw = dict(callback_class=MaxScoreOverRotations, callback_class_args={"score_threshold": -1.0},
+              interpolation_order=1, pad_template_filter=False)
+    fields = ("template", "template_mask", "template_filter", "target_filter", "rotations")
+    res = scan_templates(target, [{k: e[k] for k in fields} for e in entries], score="FLC", schedule="graph", **kw)
+    for entry, result in zip(entries, res):
+        assert result[0].shape == target.shape
+        _vs_oracle(target, entry, result, -1.0, workers=8)
+
+
+@pytest.mark.parametrize("shape", [(1024, 48, 256), (128, 1024, 256)])
+@pytest.mark.parametrize("fused", [False, True])
+def test_1024_point_axes_vs_oracle(shape, fused):
+    """The 1024-point kernel instantiations of the large-tomogram scan
+    against the FP64 oracle: pass A's z transform at z = 1024 (1024, 48,
+    256) and pass B's y transform at y = 1024 (128, 1024, 256), both with
+    the split rows of the 256-point x axis; 32^3 template with a threshold
+    mask, 2 rotations."""
+    from pytme_amd.synthetic import blob_template, embed_copies, threshold_mask
+    from pytme_amd.rotations import uniform_rotations
+
+    template = blob_template(32, n_blobs=12, seed=91)
+    mask = threshold_mask(template)
+    target, _, _ = embed_copies(shape, template, mask, 3, 0.5, seed=92)
+    rots = uniform_rotations(2, seed=93, include_identity=True)
+    print(f"1024-point axes {shape} fused={fused}", flush=True)
+    s, off, r, _ = _run("FLC", target, template, mask, rots, fused, threshold=-1.0)
+    _, _, raws, oc = orc.scan("FLC", target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True, workers=8)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+def test_scan_releases_the_fused_plan():
+    """A scan's batch workspace is released when the scan ends (the
+    backend's free_cache), so the next scan's batch choice and other HBM
+    users see the same free memory as before."""
+    import torch
+
+    from pytme_amd.backends import hip_fused
+
+    target, template, mask, rots = _workload(seed=61, n_rot=6)
+    _run("FLC", target, template, mask, rots, True, threshold=-1.0)  # settles torch's caches
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    s, off, r, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0)
+    torch.cuda.synchronize()
+    assert hip_fused._PLAN["handle"] is None
+    assert abs(torch.cuda.mem_get_info()[0] - free0) < (256 << 20)
 
 
 def test_multi_template_schedule():
@@ -315,6 +412,7 @@ def test_multi_template_schedule():
         single = _run("FLC", target, entry["template"], entry["template_mask"], entry["rotations"], True,
                       threshold=0.0, template_filter=entry["template_filter"])
         assert np.array_equal(multi[k][0], single[0]) and np.array_equal(multi[k][2], single[2])
+        _vs_oracle(target, entry, multi[k], 0.0)
         s = multi[k][0]
         for ctr in centres[k]:
             win = tuple(slice(int(x) - 2, int(x) + 3) for x in ctr)
@@ -355,6 +453,7 @@ def test_multi_template_one_graph_schedule(counts):
         assert np.array_equal(graph[k][2], seq[k][2]), k
         assert np.array_equal(graph[k][1], seq[k][1]), k
         assert sorted(graph[k][3]) == sorted(seq[k][3]), k
+        _vs_oracle(target, entries[k], graph[k], -1.0)
     with pytest.raises(ValueError):
         scan_templates(target, entries[:1], score="FLC", schedule="graph", target_splits={0: 2}, **kw)
 
