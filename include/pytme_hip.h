@@ -254,6 +254,23 @@ int tm_peak_caller_fold(const int64_t* cand, int64_t n_cand, const float* scores
                         int64_t min_boundary_distance, int has_min, float min_score, int has_max, float max_score,
                         int64_t* out_pos, float* out_rot, float* out_score, float* out_det, int64_t* counts,
                         void* work, int64_t work_bytes, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tm_peak_caller_fold with the same arguments, results
+ * and errors but no workspace argument; its workspace is a static scratch
+ * array of the library, one copy per GPU (tm_peak_caller_fold_noalloc_max_bytes
+ * = 8 MiB: with num_peaks 1000 and a full state about 1.8 million
+ * candidates), held until the process ends.  Steps whose
+ * tm_peak_caller_fold_workspace_bytes exceeds it return TM_ERR_VALUE before
+ * any launch.  It calls no allocator.  Calls that overlap on one GPU race on
+ * the shared scratch. */
+int64_t tm_peak_caller_fold_noalloc_max_bytes(void);
+int tm_peak_caller_fold_noalloc(const int64_t* cand, int64_t n_cand, const float* scores, const int64_t* shape,
+                                const int64_t* st_pos, const float* st_rot, const float* st_score,
+                                const float* st_det, int64_t n_state, const float* rotation, int64_t num_peaks,
+                                int64_t min_distance, int64_t min_boundary_distance, int has_min, float min_score,
+                                int has_max, float max_score, int64_t* out_pos, float* out_rot, float* out_score,
+                                float* out_det, int64_t* counts, void* stream);
 
 #ifdef __cplusplus
 }

@@ -117,6 +117,15 @@ SIGNATURES = {
         c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]
     ),
     "tm_peak_caller_fold_workspace_bytes": (c_int64, [c_int64, c_int64, c_int64]),
+    "tm_peak_caller_fold_noalloc_max_bytes": (c_int64, []),
+    "tm_peak_caller_fold_noalloc": (
+        c_int,
+        [
+            c_void_p, c_int64, c_void_p, c_int64_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+            c_int64, c_int64, c_int64, c_int, c_float, c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+            c_void_p, c_void_p,
+        ],
+    ),
     "tm_peak_caller_fold": (
         c_int,
         [

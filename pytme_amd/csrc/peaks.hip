@@ -19,6 +19,7 @@
 // one single-block exclusive scan and a ballot-ranked write, so every output
 // is deterministic.
 #include <algorithm>
+#include <mutex>
 
 #include "tm_common.h"
 
@@ -1160,6 +1161,70 @@ int tm_peak_caller_fold(const int64_t* cand, int64_t n_cand, const float* scores
                      n_state, rotation, new_pos, new_score, out_pos, out_rot, out_score, out_det, counts);
   TM_CHECK_LAUNCH();
   return TM_OK;
+}
+
+}  // extern "C"
+
+// Scratch of tm_peak_caller_fold_noalloc: a file-scope device array, one
+// copy per GPU (each at its own address), held until the process ends.
+// Every call on a GPU shares that GPU's copy and leaves its contents behind;
+// the fold's own kernels clear what must start at zero (counts, conflict
+// lists, the top-k selection state), so no call depends on an earlier one.
+// Calls that overlap on one GPU race on it.
+#define PK_NOALLOC_SCRATCH_BYTES (8ll << 20)
+#define PK_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) char scratch[PK_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[PK_NOALLOC_MAX_DEVICES];
+
+// Device address of this GPU's scratch copy, looked up once per device (with
+// that device current): host code uses this address, never the array's name.
+static int pk_scratch_address(void** out) {
+  int dev = -1;
+  TM_CHECK_HIP(hipGetDevice(&dev));
+  TM_REQUIRE(dev >= 0 && dev < PK_NOALLOC_MAX_DEVICES, "device %d beyond the %d devices the scratch covers", dev,
+             PK_NOALLOC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TM_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TM_OK;
+}
+
+extern "C" {
+
+int64_t tm_peak_caller_fold_noalloc_max_bytes(void) { return PK_NOALLOC_SCRATCH_BYTES; }
+
+// tm_peak_caller_fold without a workspace argument and without any device
+// allocation: the workspace is the library's static per-device scratch of
+// PK_NOALLOC_SCRATCH_BYTES.  Arguments, results and errors are those of
+// tm_peak_caller_fold; a step whose workspace
+// (tm_peak_caller_fold_workspace_bytes) exceeds the scratch returns
+// TM_ERR_VALUE before anything is launched.
+int tm_peak_caller_fold_noalloc(const int64_t* cand, int64_t n_cand, const float* scores, const int64_t* shape,
+                                const int64_t* st_pos, const float* st_rot, const float* st_score,
+                                const float* st_det, int64_t n_state, const float* rotation, int64_t num_peaks,
+                                int64_t min_distance, int64_t min_boundary_distance, int has_min, float min_score,
+                                int has_max, float max_score, int64_t* out_pos, float* out_rot, float* out_score,
+                                float* out_det, int64_t* counts, void* stream) {
+  TM_REQUIRE(n_cand >= 1 && n_state >= 0 && num_peaks >= 1, "tm_peak_caller_fold_noalloc: bad sizes");
+  const int64_t need = tm_peak_caller_fold_workspace_bytes(n_cand, n_state, num_peaks);
+  TM_REQUIRE(need <= PK_NOALLOC_SCRATCH_BYTES,
+             "tm_peak_caller_fold_noalloc: %lld candidates, %lld state rows and %lld peaks need %lld bytes of "
+             "scratch, more than its %lld; use tm_peak_caller_fold with a workspace",
+             (long long)n_cand, (long long)n_state, (long long)num_peaks, (long long)need,
+             (long long)PK_NOALLOC_SCRATCH_BYTES);
+  void* work = nullptr;
+  const int rc = pk_scratch_address(&work);
+  if (rc != TM_OK) return rc;
+  return tm_peak_caller_fold(cand, n_cand, scores, shape, st_pos, st_rot, st_score, st_det, n_state, rotation,
+                             num_peaks, min_distance, min_boundary_distance, has_min, min_score, has_max, max_score,
+                             out_pos, out_rot, out_score, out_det, counts, work, PK_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 }  // extern "C"
