@@ -636,7 +636,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if not args.no_peaks and args.path == "fused":
-        out["peak_calling"] = bench_peaks(proxy._state[0], args.num_peaks, args.min_distance, 5,
+        out["peak_calling"] = bench_peaks(proxy.state[0], args.num_peaks, args.min_distance, 5,
                                           not args.no_cpu_baseline)
     if not args.no_refinement and args.workload.startswith("flc_512"):
         out["refinement"] = bench_refinement(target, template, mask, spec, args.refine_peaks, args.refine_steps,

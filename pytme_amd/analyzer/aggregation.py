@@ -10,7 +10,7 @@ from ..backends import backend as be
 from ..matching_utils import apply_convolution_mode
 from .base import AbstractAnalyzer
 
-__all__ = ["MaxScoreOverRotations"]
+__all__ = ["MaxScoreOverRotations", "assign_rotation_indices"]
 
 
 def _to_memmap(arr: np.ndarray) -> np.ndarray:
@@ -20,6 +20,25 @@ def _to_memmap(arr: np.ndarray) -> np.ndarray:
     mm[:] = arr
     mm.flush()
     return np.memmap(name, mode="r", shape=arr.shape, dtype=arr.dtype)
+
+
+def assign_rotation_indices(mapping: Dict, rotation_matrices, inversion_mapping: bool) -> List[int]:
+    """Rotation indices of a batch of rotations as successive analyzer calls
+    assign them, updating ``mapping`` the same way
+    (tme/analyzer/aggregation.py:170-189): with ``inversion_mapping`` the
+    index is the call number and the mapping stores index -> matrix;
+    otherwise matrices are keyed by their float bytes and a repeated matrix
+    reuses its index."""
+    out = []
+    for i in range(rotation_matrices.shape[0]):
+        rot = be.astype(rotation_matrices[i], be._float_dtype)
+        idx = len(mapping)
+        if inversion_mapping:
+            mapping[idx] = rot
+        else:
+            idx = mapping.setdefault(be.tobytes(rot), idx)
+        out.append(idx)
+    return out
 
 
 class MaxScoreOverRotations(AbstractAnalyzer):
@@ -67,17 +86,7 @@ class MaxScoreOverRotations(AbstractAnalyzer):
 
     def rotation_indices(self, state, rotation_matrices) -> List[int]:
         """Assign indices to a batch of rotations exactly as successive calls would."""
-        _, _, mapping = state
-        out = []
-        for i in range(rotation_matrices.shape[0]):
-            rot = be.astype(rotation_matrices[i], be._float_dtype)
-            idx = len(mapping)
-            if self._inversion_mapping:
-                mapping[idx] = rot
-            else:
-                idx = mapping.setdefault(be.tobytes(rot), idx)
-            out.append(idx)
-        return out
+        return assign_rotation_indices(state[2], rotation_matrices, self._inversion_mapping)
 
     def __call__(self, state, scores, rotation_matrix, **kwargs):
         prev_scores, rotations, mapping = state

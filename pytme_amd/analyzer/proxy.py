@@ -1,70 +1,95 @@
-"""Shared analyzer proxies (mirrors ``tme/analyzer/proxy.py:21-123``).
+"""Analyzer proxies: the objects the scan hands to its scoring functions as
+``callback`` (public interface of ``tme/analyzer/proxy.py:21-123``).
 
-The proxy owns an analyzer instance and its state.  When a shared-memory
-manager is given (CPU process pools) the state arrays move to shared memory
-and every update takes a lock; device backends run one rotation job per
-analyzer and never share.
+A proxy owns one analyzer.  In the usual case (one rotation job per
+analyzer, always on a GPU device) the analyzer's state tuple stays local and
+each call replaces it with the analyzer's return value.  When the scan
+gives a shared-memory handler (CPU process pools), the state is published
+once through :class:`_SharedState`: its arrays go through the backend's
+``to_sharedarr``, its mappings into a manager dict, and every update runs
+under one manager lock on the attached arrays, which the analyzer updates
+in place.
 """
 
 from multiprocessing import Manager
-from multiprocessing.shared_memory import SharedMemory
-from typing import Tuple
+from typing import Any, Dict, Tuple
 
 from ..backends import backend as be
 
 __all__ = ["StatelessSharedAnalyzerProxy", "SharedAnalyzerProxy"]
 
 
+def _is_backend_array(item) -> bool:
+    # backend arrays have one concrete class (tme/matching_exhaustive.py:46)
+    return isinstance(item, type(be.zeros((1,), dtype=be._float_dtype)))
+
+
+class _SharedState:
+    """Publishes an analyzer state across processes and attaches to it.
+
+    Remembers which slots of the state tuple it turned into backend shared
+    arrays, so that :meth:`attach` maps exactly those back (the other slots,
+    manager dicts and plain values, are usable as they are).  The lock and
+    the dict proxies pickle into worker processes; the manager process lives
+    as long as a proxy to it does.
+    """
+
+    def __init__(self, state: Tuple, shm_handler):
+        manager = Manager()
+        self.lock = manager.Lock()
+        self.array_slots = frozenset(i for i, item in enumerate(state) if _is_backend_array(item))
+        published = []
+        for i, item in enumerate(state):
+            if i in self.array_slots:
+                item = be.to_sharedarr(item, shm_handler)
+            elif isinstance(item, dict):
+                item = manager.dict(item)
+            published.append(item)
+        self.state = tuple(published)
+
+    def attach(self, state: Tuple) -> Tuple:
+        return tuple(be.from_sharedarr(item) if i in self.array_slots else item for i, item in enumerate(state))
+
+
 class StatelessSharedAnalyzerProxy:
-    """Wrap a functional analyzer for (optionally locked) shared updates."""
+    """Proxy of a functional analyzer; the caller carries the state.
 
-    def __init__(self, analyzer_class: type, analyzer_params: dict):
-        self._shared = False
-        self._process = self._direct_call
+    ``init_state(shm_handler)`` returns a fresh state (published when a
+    handler is given); ``proxy(state, ...)`` applies one update and returns
+    the analyzer's new state; ``result`` and ``merge`` forward to the
+    analyzer.
+    """
+
+    def __init__(self, analyzer_class: type, analyzer_params: Dict[str, Any]):
         self._analyzer = analyzer_class(**analyzer_params)
+        self._sharing = None  # _SharedState once the state is published
 
-    def __call__(self, state, *args, **kwargs):
-        return self._process(state, *args, **kwargs)
+    @property
+    def analyzer(self):
+        return self._analyzer
+
+    @property
+    def is_shared(self) -> bool:
+        """True when the state lives in shared memory behind a lock."""
+        return self._sharing is not None
 
     def init_state(self, shm_handler=None, *args, **kwargs) -> Tuple:
         state = self._analyzer.init_state()
-        if shm_handler is not None:
-            self._shared = True
-            state = self._to_shared(state, shm_handler)
-            self._lock = Manager().Lock()
-            self._process = self._thread_safe_call
-        return state
+        if shm_handler is None:
+            self._sharing = None
+            return state
+        self._sharing = _SharedState(state, shm_handler)
+        return self._sharing.state
 
-    def _to_shared(self, state: Tuple, shm_handler):
-        backend_arr = type(be.zeros((1), dtype=be._float_dtype))
-        out = []
-        for v in state:
-            if isinstance(v, backend_arr):
-                v = be.to_sharedarr(v, shm_handler)
-            elif isinstance(v, dict):
-                v = Manager().dict(**v)
-            out.append(v)
-        return tuple(out)
-
-    def _shared_to_object(self, shared):
-        if not self._shared:
-            return shared
-        if isinstance(shared, tuple) and len(shared):
-            if isinstance(shared[0], SharedMemory) or len(shared) == 1:
-                return be.from_sharedarr(shared)
-        return shared
-
-    def _thread_safe_call(self, state, *args, **kwargs):
-        with self._lock:
-            state = tuple(self._shared_to_object(x) for x in state)
-            return self._direct_call(state, *args, **kwargs)
-
-    def _direct_call(self, state, *args, **kwargs):
-        return self._analyzer(state, *args, **kwargs)
+    def __call__(self, state, *args, **kwargs):
+        if self._sharing is None:
+            return self._analyzer(state, *args, **kwargs)
+        with self._sharing.lock:
+            return self._analyzer(self._sharing.attach(state), *args, **kwargs)
 
     def result(self, state, **kwargs):
-        if self._shared:
-            state = tuple(self._shared_to_object(x) for x in state)
+        if self._sharing is not None:
+            state = self._sharing.attach(state)
         return self._analyzer.result(state, **kwargs)
 
     def merge(self, *args, **kwargs):
@@ -72,29 +97,25 @@ class StatelessSharedAnalyzerProxy:
 
 
 class SharedAnalyzerProxy(StatelessSharedAnalyzerProxy):
-    """Stateful proxy: keeps the analyzer state and updates it on every call."""
+    """Proxy that keeps the state: ``proxy(...)`` updates it, ``result()``
+    reads it.  A handler is used only for analyzers that declare their state
+    ``shareable``."""
 
-    def __init__(self, analyzer_class: type, analyzer_params: dict, shm_handler: type = None, **kwargs):
-        super().__init__(analyzer_class=analyzer_class, analyzer_params=analyzer_params)
-        if not self._analyzer.shareable:
-            shm_handler = None
-        self.init_state(shm_handler)
+    def __init__(self, analyzer_class: type, analyzer_params: Dict[str, Any], shm_handler=None, **kwargs):
+        super().__init__(analyzer_class, analyzer_params)
+        self.init_state(shm_handler if self._analyzer.shareable else None)
 
     def init_state(self, shm_handler=None, *args, **kwargs) -> None:
-        self._state = super().init_state(shm_handler, *args, **kwargs)
+        self._state = super().init_state(shm_handler)
 
     @property
-    def analyzer(self):
-        return self._analyzer
-
-    @property
-    def state(self):
+    def state(self) -> Tuple:
         return self._state
 
-    def __call__(self, *args, **kwargs):
-        state = super().__call__(self._state, *args, **kwargs)
-        if not self._shared:
-            self._state = state
+    def __call__(self, *args, **kwargs) -> None:
+        updated = super().__call__(self._state, *args, **kwargs)
+        if not self.is_shared:  # shared arrays were updated in place
+            self._state = updated
 
     def result(self, **kwargs):
         return super().result(self._state, **kwargs)

@@ -161,15 +161,40 @@ def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, can
 
 
 def _proxy_state(callback):
+    """(analyzer, state) of an unshared ``MaxScoreOverRotations`` behind the
+    scan's callback proxy, or None.
+
+    Two proxies qualify: this package's ``SharedAnalyzerProxy``, and pyTME's
+    own (tme/analyzer/proxy.py:72-123 around tme/analyzer/aggregation.py:
+    31-431, when an installed pyTME runs on this backend): the latter is
+    recognised by its attributes ``_analyzer``, ``_state`` and ``_shared``
+    and an analyzer class named ``MaxScoreOverRotations`` whose state is the
+    same (scores, rotations, mapping) tuple of backend arrays.
+    """
     from ..analyzer.aggregation import MaxScoreOverRotations
     from ..analyzer.proxy import SharedAnalyzerProxy
 
-    if not isinstance(callback, SharedAnalyzerProxy):
+    if isinstance(callback, SharedAnalyzerProxy):
+        analyzer, state, shared = callback.analyzer, callback.state, callback.is_shared
+        if type(analyzer) is not MaxScoreOverRotations:
+            return None
+    elif all(hasattr(callback, a) for a in ("_analyzer", "_state", "_shared")):
+        analyzer, state, shared = callback._analyzer, callback._state, callback._shared
+        if type(analyzer).__name__ != "MaxScoreOverRotations" or not hasattr(analyzer, "_inversion_mapping"):
+            return None
+        if not (isinstance(state, tuple) and len(state) == 3 and isinstance(state[2], dict)):
+            return None
+    else:
         return None
-    analyzer = callback._analyzer
-    if type(analyzer) is not MaxScoreOverRotations or callback._shared or analyzer._jax_mode:
+    if shared or getattr(analyzer, "_jax_mode", False):
         return None
-    return analyzer, callback._state
+    return analyzer, state
+
+
+def _rotation_indices(job):
+    from ..analyzer.aggregation import assign_rotation_indices
+
+    return assign_rotation_indices(job["state"][2], job["rots"], bool(job["analyzer"]._inversion_mapping))
 
 
 def _job(backend, score: str, kw):
@@ -286,7 +311,7 @@ def run(backend, score: str, **kw):
     def create(batch):
         return _create_plan(job, batch)
 
-    idx = job["analyzer"].rotation_indices(job["state"], rots)
+    idx = _rotation_indices(job)
     idx_dev = t.tensor(np.asarray(idx, dtype=np.int32), device=rots.device)
     if auto:
         candidates = tuple(b for b in (B, B // 2, B // 4) if b >= 1)
@@ -337,8 +362,7 @@ def run_group(backend, kws, stats=None):
         job = _job(backend, score, kw)
         if job is NotImplemented:
             raise ValueError(f"template {i}: the fused scan does not apply to these arguments")
-        job["idx_dev"] = t.tensor(np.asarray(job["analyzer"].rotation_indices(job["state"], job["rots"]),
-                                             dtype=np.int32), device=job["rots"].device)
+        job["idx_dev"] = t.tensor(np.asarray(_rotation_indices(job), dtype=np.int32), device=job["rots"].device)
         jobs.append(job)
     if not jobs:
         return

@@ -19,12 +19,7 @@ import numpy as np
 from scipy.ndimage import laplace
 
 from .backends import backend as be
-from .matching_utils import (
-    _normalize_template_overflow_safe,
-    conditional_execute,
-    identity,
-    normalize_template,
-)
+from .matching_utils import _normalize_template_overflow_safe, normalize_template
 
 __all__ = [
     "cc_setup",
@@ -37,8 +32,62 @@ __all__ = [
 ]
 
 
-def _shape_match(shape1: Tuple[int], shape2: Tuple[int]) -> bool:
-    return len(shape1) == len(shape2) and tuple(int(x) for x in shape1) == tuple(int(x) for x in shape2)
+def _shape(x) -> Tuple[int, ...]:
+    return tuple(int(v) for v in x)
+
+
+def _half_spectrum(shape) -> Tuple[int, ...]:
+    shape = _shape(shape)
+    return shape[:-1] + (shape[-1] // 2 + 1,)
+
+
+class _TemplateFilter:
+    """Fourier filter of the rotated template (tme/matching_scores.py:44-150).
+
+    The filter array is a real half spectrum.  Its shape picks the box it
+    acts on: the template's own half spectrum filters the template-sized
+    top-left box of the array (``pad_template_filter=False``, the CLI
+    default), the half spectrum of the array filters the whole array.  A
+    filter of size 1 is the identity.  The box is transformed, multiplied
+    and transformed back in place: out = irfftn(rfftn(box) * filter).
+    """
+
+    def __init__(self, arr_filter, template_shape, arr_shape, real_dtype, cmpl_dtype, plans=None):
+        self.identity = be.size(arr_filter) == 1
+        if self.identity:
+            return
+        fshape = _shape(arr_filter.shape)
+        if fshape == _half_spectrum(arr_shape):
+            box = _shape(arr_shape)
+        elif fshape == _half_spectrum(template_shape):
+            box = _shape(template_shape)
+        else:
+            raise NotImplementedError(
+                f"template filter of shape {fshape} fits neither the template {_shape(template_shape)} "
+                f"nor the array {_shape(arr_shape)} (batch-broadcast filters are not supported)"
+            )
+        self.whole = box == _shape(arr_shape)
+        self.box = tuple(slice(0, n) for n in box)
+        if plans is not None and self.whole:
+            self.fwd, self.inv = plans
+        else:
+            self.fwd, self.inv = be.build_fft(fwd_shape=box, inv_shape=fshape, real_dtype=real_dtype,
+                                              cmpl_dtype=cmpl_dtype, inv_output_shape=box)
+        self.real = None if self.whole else be.zeros(box, real_dtype)
+        self.spec = be.zeros(fshape, cmpl_dtype)
+
+    def __call__(self, arr, arr_filter):
+        if self.identity:
+            return arr
+        src = arr if self.whole else self.real
+        if not self.whole:
+            src[...] = arr[self.box]
+        spec = be.multiply(self.fwd(src, self.spec), arr_filter, out=self.spec)
+        src = self.inv(spec, src)
+        if self.whole:
+            return src
+        arr[self.box] = src
+        return arr
 
 
 def _create_filter_func(
@@ -55,58 +104,18 @@ def _create_filter_func(
     rfftn: Callable = None,
     irfftn: Callable = None,
 ) -> Callable:
-    """Build ``f(arr, ft_temp, filter) -> arr`` applying a Fourier filter.
-
-    A filter of size 1 is the identity.  A filter shaped like the template's
-    half spectrum is applied at template size to the top-left template box
-    of ``arr`` (reference default ``pad_template_filter=False`` path,
-    tme/matching_scores.py:44-150).
-    """
-    if be.size(arr_filter) == 1:
-        return conditional_execute(identity, identity, False)
-
-    filter_shape = tuple(int(x) for x in arr_filter.shape)
-    try:
-        product_ft_shape = tuple(int(x) for x in np.broadcast_shapes(tuple(arr_ft_shape), filter_shape))
-    except ValueError:
-        product_ft_shape, inv_output_shape = filter_shape, tuple(arr_shape)
-
-    valid = _shape_match(arr_shape, fwd_shape) and _shape_match(product_ft_shape, inv_shape)
-    valid = valid and rfftn is not None and irfftn is not None
-    if not valid:
-        _fwd_shape = tuple(arr_shape)
-        if all(x > y for x, y in zip(arr_shape, product_ft_shape)):
-            _fwd_shape = tuple(fwd_shape)
-        rfftn, irfftn = be.build_fft(
-            fwd_shape=_fwd_shape,
-            inv_shape=product_ft_shape,
-            real_dtype=real_dtype,
-            cmpl_dtype=cmpl_dtype,
-            inv_output_shape=inv_output_shape,
-            fwd_axes=fwd_axes,
-            inv_axes=inv_axes,
-        )
-
-    def _apply_filter(template, ft_temp, template_filter):
-        ft_temp = rfftn(template, ft_temp)
-        ft_temp = be.multiply(ft_temp, template_filter, out=ft_temp)
-        return irfftn(ft_temp, template)
-
-    if not _shape_match(arr_ft_shape, filter_shape):
-        if any(x > y and y == 1 for x, y in zip(filter_shape, arr_ft_shape)):
-            raise NotImplementedError("filters broadcast over a batch dimension are not supported")
-        real_subset = tuple(slice(0, int(x)) for x in arr_shape)
-        _template = be.zeros(tuple(arr_shape), be._float_dtype)
-        _ft_temp = be.zeros(product_ft_shape, be._complex_dtype)
-
-        def _apply_filter_subset(template, ft_temp, template_filter):
-            _template[:] = template[real_subset]
-            template[real_subset] = _apply_filter(_template, _ft_temp, template_filter)
-            return template
-
-        return _apply_filter_subset
-
-    return _apply_filter
+    """``f(arr, ft_temp, filter) -> arr`` applying the template filter
+    (signature of tme/matching_scores.py:44-57).  ``arr_shape`` is the
+    template's shape; the returned function filters the array it is given,
+    either whole (filter shaped like ``arr_ft_shape``, using the given
+    transforms when they fit ``fwd_shape``) or its template-sized top-left
+    box (filter shaped like the template's half spectrum)."""
+    full = tuple(int(x) for x in fwd_shape)
+    if _shape(arr_filter.shape) != _shape(arr_ft_shape) and be.size(arr_filter) != 1:
+        full = _shape(inv_output_shape)
+    plans = (rfftn, irfftn) if rfftn is not None and irfftn is not None else None
+    flt = _TemplateFilter(arr_filter, arr_shape, full, real_dtype, cmpl_dtype, plans=plans)
+    return lambda arr, ft_temp, template_filter: flt(arr, template_filter)
 
 
 # --------------------------------------------------------------------- setups
@@ -191,6 +200,45 @@ def _fused(score: str, **kwargs):
     return hook(score=score, **kwargs)
 
 
+class _PlainWorkspace:
+    """Buffers and transforms of the per-rotation loop on ``fast_shape``.
+
+    ``forward(real)`` leaves the half spectrum of a zero-padded array in the
+    workspace's spectrum buffer; ``correlate(spec, target_spec, out)`` is
+    the circular correlation irfftn(spec * target_spec) written to ``out``
+    (the template and mask are stored reversed, so a plain product of
+    spectra correlates, tme/matching_data.py:698-715).  The inverse
+    transform consumes a separate product buffer, so one template spectrum
+    can be correlated with several target spectra.
+    """
+
+    def __init__(self, fast_shape, fast_ft_shape):
+        self.real_dtype, self.cmpl_dtype = be._float_dtype, be._complex_dtype
+        self.fast_shape = _shape(fast_shape)
+        self.arr = be.zeros(self.fast_shape, self.real_dtype)
+        self.spec = be.zeros(_shape(fast_ft_shape), self.cmpl_dtype)
+        self.prod = be.zeros(_shape(fast_ft_shape), self.cmpl_dtype)
+        self.fwd, self.inv = be.build_fft(
+            fwd_shape=self.fast_shape, inv_shape=_shape(fast_ft_shape), real_dtype=self.real_dtype,
+            cmpl_dtype=self.cmpl_dtype, inv_output_shape=self.fast_shape, temp_fwd=self.arr, temp_inv=self.spec,
+        )
+
+    def volume(self):
+        return be.zeros(self.fast_shape, self.real_dtype)
+
+    def template_filter(self, arr_filter, template_shape):
+        return _TemplateFilter(arr_filter, template_shape, self.fast_shape, self.real_dtype, self.cmpl_dtype,
+                               plans=(self.fwd, self.inv))
+
+    def forward(self, real):
+        self.spec = self.fwd(real, self.spec)
+        return self.spec
+
+    def correlate(self, spec, target_spec, out):
+        self.prod = be.multiply(target_spec, spec, out=self.prod)
+        return self.inv(self.prod, out)
+
+
 def corr_scoring(
     template,
     template_filter,
@@ -209,7 +257,9 @@ def corr_scoring(
     t_R is the rotated (about the template centre), filtered and - when a
     mask is given - mask-normalised template; CC is the circular correlation
     on ``fast_shape`` computed as irfftn(rfftn(pad(t_R)) * ft_target)
-    (tme/matching_scores.py:491-626).
+    (tme/matching_scores.py:491-626).  ``numerator`` / ``inv_denominator``
+    take part only when they are ``fast_shape`` volumes (the setups pass
+    size-1 placeholders otherwise).
     """
     ret = _fused(
         "corr", template=template, template_filter=template_filter, ft_target=ft_target,
@@ -220,67 +270,43 @@ def corr_scoring(
     if ret is not NotImplemented:
         return ret
 
-    template = be.from_sharedarr(template)
-    ft_target = be.from_sharedarr(ft_target)
-    inv_denominator = be.from_sharedarr(inv_denominator)
-    numerator = be.from_sharedarr(numerator)
-    template_filter = be.from_sharedarr(template_filter)
+    template, ft_target, template_filter = (be.from_sharedarr(x) for x in (template, ft_target, template_filter))
+    numerator, inv_denominator = be.from_sharedarr(numerator), be.from_sharedarr(inv_denominator)
+    work = _PlainWorkspace(fast_shape, fast_ft_shape)
+    filt = work.template_filter(template_filter, template.shape)
+    offset = numerator if _shape(numerator.shape) == work.fast_shape else None
+    scale = inv_denominator if _shape(inv_denominator.shape) == work.fast_shape else None
 
-    norm_func, norm_template, mask_sum = normalize_template, False, 1
+    normalise = None
     if template_mask is not None:
         template_mask = be.from_sharedarr(template_mask)
-        norm_template, mask_sum = True, be.sum(template_mask)
-        if be.datatype_bytes(template_mask.dtype) == 2:
-            norm_func = _normalize_template_overflow_safe
-            mask_sum = be.sum(be.astype(template_mask, be._overflow_safe_dtype))
+        if be.datatype_bytes(template_mask.dtype) == 2:  # 2-byte masks: sums in the overflow-safe type
+            n_obs = be.sum(be.astype(template_mask, be._overflow_safe_dtype))
+            normalise = lambda t: _normalize_template_overflow_safe(t, template_mask, n_obs)  # noqa: E731
+        else:
+            n_obs = be.sum(template_mask)
+            normalise = lambda t: normalize_template(t, template_mask, n_obs)  # noqa: E731
 
-    callback_func = conditional_execute(callback, callback is not None)
-    norm_template = conditional_execute(norm_func, norm_template)
-    norm_numerator = conditional_execute(be.subtract, identity, _shape_match(numerator.shape, fast_shape))
-    norm_denominator = conditional_execute(be.multiply, identity, _shape_match(inv_denominator.shape, fast_shape))
-
-    arr = be.zeros(fast_shape, be._float_dtype)
-    ft_temp = be.zeros(fast_ft_shape, be._complex_dtype)
-    fftargs = {
-        "real_dtype": be._float_dtype,
-        "cmpl_dtype": be._complex_dtype,
-        "inv_output_shape": fast_shape,
-        "fwd_axes": None,
-        "inv_axes": None,
-        "inv_shape": fast_ft_shape,
-        "fwd_shape": tuple(arr.shape),
-    }
-    rfftn, irfftn = be.build_fft(temp_fwd=arr, temp_inv=ft_temp, **fftargs)
-    template_filter_func = _create_filter_func(
-        arr_shape=template.shape,
-        arr_ft_shape=fast_ft_shape,
-        arr_filter=template_filter,
-        rfftn=rfftn,
-        irfftn=irfftn,
-        **fftargs,
-    )
-
-    center = [(int(x) - 1) / 2 for x in template.shape]
-    unpadded = tuple(slice(0, int(x)) for x in template.shape)
-    template_rot = be.zeros(template.shape, be._float_dtype)
-    for index in range(rotations.shape[0]):
-        rotation = rotations[index]
-        matrix = be._rigid_transform_matrix(rotation_matrix=rotation, center=center)
-        template_rot, _ = be.rigid_transform(
-            arr=template, rotation_matrix=matrix, out=template_rot, order=interpolation_order, cache=True
-        )
-        template_rot = template_filter_func(template_rot, ft_temp, template_filter)
-        norm_template(template_rot, template_mask, mask_sum)
-
-        arr = be.fill(arr, 0)
-        arr[unpadded] = template_rot
-        ft_temp = rfftn(arr, ft_temp)
-        ft_temp = be.multiply(ft_target, ft_temp, out=ft_temp)
-        arr = irfftn(ft_temp, arr)
-
-        arr = norm_numerator(arr, numerator, out=arr)
-        arr = norm_denominator(arr, inv_denominator, out=arr)
-        callback_func(arr, rotation_matrix=rotation)
+    centre = [(int(x) - 1) / 2 for x in template.shape]
+    box = tuple(slice(0, int(x)) for x in template.shape)
+    rotated = be.zeros(template.shape, be._float_dtype)
+    scores = work.arr
+    for rotation in rotations:
+        pose = be._rigid_transform_matrix(rotation_matrix=rotation, center=centre)
+        rotated, _ = be.rigid_transform(arr=template, rotation_matrix=pose, out=rotated, order=interpolation_order,
+                                        cache=True)
+        rotated = filt(rotated, template_filter)
+        if normalise is not None:
+            normalise(rotated)
+        scores = be.fill(scores, 0)
+        scores[box] = rotated
+        scores = work.correlate(work.forward(scores), ft_target, out=scores)
+        if offset is not None:
+            scores = be.subtract(scores, offset, out=scores)
+        if scale is not None:
+            scores = be.multiply(scores, scale, out=scores)
+        if callback is not None:
+            callback(scores, rotation_matrix=rotation)
     return callback
 
 
@@ -312,70 +338,30 @@ def flc_scoring(
     if ret is not NotImplemented:
         return ret
 
-    float_dtype, complex_dtype = be._float_dtype, be._complex_dtype
-    template = be.from_sharedarr(template)
-    template_mask = be.from_sharedarr(template_mask)
-    ft_target = be.from_sharedarr(ft_target)
-    ft_target2 = be.from_sharedarr(ft_target2)
-    template_filter = be.from_sharedarr(template_filter)
-
-    arr = be.zeros(fast_shape, float_dtype)
-    temp = be.zeros(fast_shape, float_dtype)
-    temp2 = be.zeros(fast_shape, float_dtype)
-    ft_temp = be.zeros(fast_ft_shape, complex_dtype)
-    ft_denom = be.zeros(fast_ft_shape, complex_dtype)
-
-    fftargs = {
-        "real_dtype": float_dtype,
-        "cmpl_dtype": complex_dtype,
-        "inv_output_shape": fast_shape,
-        "fwd_axes": None,
-        "inv_axes": None,
-        "inv_shape": fast_ft_shape,
-        "fwd_shape": tuple(arr.shape),
-    }
-    rfftn, irfftn = be.build_fft(temp_fwd=arr, temp_inv=ft_temp, **fftargs)
-    template_filter_func = _create_filter_func(
-        arr_shape=template.shape,
-        arr_ft_shape=fast_ft_shape,
-        arr_filter=template_filter,
-        rfftn=rfftn,
-        irfftn=irfftn,
-        **fftargs,
+    template, template_mask, template_filter = (
+        be.from_sharedarr(x) for x in (template, template_mask, template_filter)
     )
-
-    eps = be.eps(float_dtype)
-    callback_func = conditional_execute(callback, callback is not None)
-    for index in range(rotations.shape[0]):
-        rotation = rotations[index]
-        arr = be.fill(arr, 0)
-        temp = be.fill(temp, 0)
-        arr, temp = be.rigid_transform(
-            arr=template,
-            arr_mask=template_mask,
-            rotation_matrix=rotation,
-            out=arr,
-            out_mask=temp,
-            use_geometric_center=True,
-            order=interpolation_order,
-            cache=True,
+    ft_target, ft_target2 = be.from_sharedarr(ft_target), be.from_sharedarr(ft_target2)
+    work = _PlainWorkspace(fast_shape, fast_ft_shape)
+    filt = work.template_filter(template_filter, template.shape)
+    t_rot, m_rot, local_sq = work.arr, work.volume(), work.volume()
+    eps = be.eps(be._float_dtype)
+    for rotation in rotations:
+        t_rot, m_rot = be.fill(t_rot, 0), be.fill(m_rot, 0)
+        t_rot, m_rot = be.rigid_transform(
+            arr=template, arr_mask=template_mask, rotation_matrix=rotation, out=t_rot, out_mask=m_rot,
+            use_geometric_center=True, order=interpolation_order, cache=True,
         )
-        n_obs = be.sum(temp)
-        arr = template_filter_func(arr, ft_temp, template_filter)
-        arr = normalize_template(arr, temp, n_obs, axis=None)
-
-        ft_temp = rfftn(temp, ft_temp)
-        ft_denom = be.multiply(ft_target, ft_temp, out=ft_denom)
-        temp = irfftn(ft_denom, temp)
-        ft_denom = be.multiply(ft_target2, ft_temp, out=ft_denom)
-        temp2 = irfftn(ft_denom, temp2)
-
-        ft_temp = rfftn(arr, ft_temp)
-        ft_temp = be.multiply(ft_target, ft_temp, out=ft_temp)
-        arr = irfftn(ft_temp, arr)
-
-        arr = be.norm_scores(arr, temp2, temp, n_obs, eps, arr)
-        callback_func(arr, rotation_matrix=rotation)
+        n_obs = be.sum(m_rot)
+        t_rot = normalize_template(filt(t_rot, template_filter), m_rot, n_obs, axis=None)
+        # local sums of f and f^2 under the rotated mask, then the overlap
+        mask_spec = work.forward(m_rot)
+        local_sq = work.correlate(mask_spec, ft_target2, out=local_sq)
+        m_rot = work.correlate(mask_spec, ft_target, out=m_rot)
+        t_rot = work.correlate(work.forward(t_rot), ft_target, out=t_rot)
+        t_rot = be.norm_scores(t_rot, local_sq, m_rot, n_obs, eps, t_rot)
+        if callback is not None:
+            callback(t_rot, rotation_matrix=rotation)
     return callback
 
 

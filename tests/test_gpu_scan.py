@@ -567,3 +567,62 @@ def test_graph_replays_match_eager_batches():
     finally:
         for plan in plans:
             _lib.call("tm_scan_plan_destroy", plan)
+
+
+class _ReferenceShapedProxy:
+    """The attribute layout of pyTME's SharedAnalyzerProxy
+    (tme/analyzer/proxy.py:72-123: _analyzer, _state, _shared) around an
+    analyzer named MaxScoreOverRotations."""
+
+    def __init__(self, analyzer_class, params):
+        self._analyzer = analyzer_class(**params)
+        self._shared = False
+        self._state = self._analyzer.init_state()
+
+    def __call__(self, *args, **kwargs):
+        self._state = self._analyzer(self._state, *args, **kwargs)
+
+    def result(self, **kwargs):
+        return self._analyzer.result(self._state, **kwargs)
+
+
+def test_installed_pytme_reaches_the_fused_scan(monkeypatch):
+    """integration.install on a pyTME-shaped module tree: its registered
+    flc_scoring (a stand-in that fails if its loop runs) called the way
+    pyTME's scan calls it, with a pyTME-shaped proxy, runs tm_scan_batch and
+    matches the FP64 oracle."""
+    from pytme_amd import _lib
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.backends import backend as be
+    from pytme_amd.integration import install
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+    from tests.test_host import _stand_in_pytme
+
+    tme, calls = _stand_in_pytme()
+
+    def flc_scoring(**kw):
+        raise AssertionError("the reference's per-rotation loop ran")
+
+    tme.matching_scores.MATCHING_EXHAUSTIVE_REGISTER["FLC"] = ("flc_setup", flc_scoring)
+    install(tme)
+    tme.backends.backend.change_backend("hip", device="cuda")
+    be.change_backend("hip", device="cuda")
+    target, template, mask, rots = _workload(seed=71, n_rot=10)
+    md = MatchingData(target.copy(), template.copy(), template_mask=mask.copy(), rotations=rots)
+    setup, cb_args, chunks = prepare_scan(md, MATCHING_EXHAUSTIVE_REGISTER["FLC"][0], n_jobs=1,
+                                          callback_class_args={"score_threshold": -1.0}, pad_target=False,
+                                          pad_template_filter=False, interpolation_order=1)
+    proxy = _ReferenceShapedProxy(MaxScoreOverRotations, cb_args)
+    seen = []
+    real_call = _lib.call
+    monkeypatch.setattr(_lib, "call", lambda name, *a: (seen.append(name), real_call(name, *a))[1])
+    print("pyTME-shaped registry -> fused scan", flush=True)
+    tme.matching_scores.MATCHING_EXHAUSTIVE_REGISTER["FLC"][1](rotations=chunks[0], callback=proxy, **setup)
+    assert "tm_scan_batch" in seen
+    s, off, r, mapping = MaxScoreOverRotations.merge([proxy.result(**cb_args)], **cb_args)
+    _, _, raws, oc = orc.scan("FLC", target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+    be.free_cache()

@@ -253,3 +253,226 @@ class TestCABI:
         assert lib.tm_fft_factor(200, r, 16, ctypes.byref(k)) == 0
         assert list(r)[: k.value] == [8, 5, 5]
         assert int(np.prod(list(r)[: k.value])) == 200
+
+
+class _CountingAnalyzer:
+    """Stand-in analyzer: state (counts array, mapping dict, call number);
+    each call adds ``value`` to the counts in place and records it."""
+
+    shareable = True
+
+    def init_state(self):
+        from pytme_amd.backends import backend as be
+
+        return be.zeros((4,), dtype=be._float_dtype), {}, 0
+
+    def __call__(self, state, value):
+        counts, mapping, n = state
+        counts += value
+        mapping[n] = value
+        return counts, mapping, n + 1
+
+    def result(self, state, **kwargs):
+        counts, mapping, n = state
+        return counts.clone(), dict(mapping), n
+
+    @classmethod
+    def merge(cls, results, **kwargs):
+        return len(results)
+
+
+class TestAnalyzerProxy:
+    def test_local_state_follows_the_analyzer(self, cpu_backend):
+        from pytme_amd.analyzer.proxy import SharedAnalyzerProxy, StatelessSharedAnalyzerProxy
+
+        proxy = SharedAnalyzerProxy(_CountingAnalyzer, {})
+        assert not proxy.is_shared
+        proxy(1.0)
+        proxy(2.0)
+        counts, mapping, n = proxy.result()
+        assert n == 2 and mapping == {0: 1.0, 1: 2.0} and float(counts[0]) == 3.0
+        assert proxy.state[2] == 2 and isinstance(proxy.analyzer, _CountingAnalyzer)
+        assert proxy.merge([1, 2, 3]) == 3
+        stateless = StatelessSharedAnalyzerProxy(_CountingAnalyzer, {})
+        st = stateless.init_state()
+        st = stateless(st, 5.0)
+        assert stateless.result(st)[2] == 1
+
+    def test_shared_state_is_published_and_updated_in_place(self, cpu_backend):
+        from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+
+        proxy = SharedAnalyzerProxy(_CountingAnalyzer, {}, shm_handler=object())
+        assert proxy.is_shared
+        published = proxy.state
+        assert isinstance(published[0], tuple)  # the backend's shared-array form
+        proxy(1.5)
+        proxy(2.5)
+        counts, mapping, n = proxy.result()
+        # arrays and the manager dict are updated in place; the plain call
+        # counter is not part of the shared state, so both calls wrote key 0
+        assert float(counts[1]) == 4.0 and mapping == {0: 2.5} and n == 0
+        assert proxy.state is published
+
+
+class TestRotationSets:
+    def _write_sets(self, path):
+        import yaml
+        from scipy.spatial.transform import Rotation
+
+        rng = np.random.default_rng(5)
+        sets = {"c48u9.npy": (9, 36.5, 2.9), "c48u27.npy": (27, 20.8, 1.6)}
+        for name, (n, angle, cov) in sets.items():
+            q = rng.standard_normal((n, 4))
+            q /= np.linalg.norm(q, axis=1, keepdims=True)
+            np.save(os.path.join(path, name), np.concatenate([q, np.full((n, 1), 1.0 / n)], axis=1))
+        with open(os.path.join(path, "metadata.yaml"), "w") as fh:
+            yaml.safe_dump({k: list(v) for k, v in sets.items()}, fh)
+        return Rotation
+
+    def test_nearest_set_scalar_last(self, tmp_path):
+        from pytme_amd.rotations import get_rotation_matrices, load_quaternions_by_angle
+
+        Rotation = self._write_sets(str(tmp_path))
+        q, w, n = load_quaternions_by_angle(30.0, data_dir=str(tmp_path))
+        assert q.shape == (9, 4) and n == 9 and np.allclose(w, 1 / 9)
+        assert load_quaternions_by_angle(15.0, data_dir=str(tmp_path))[0].shape == (27, 4)
+        mats = get_rotation_matrices(15.0, data_dir=str(tmp_path))
+        want = Rotation.from_quat(np.load(os.path.join(str(tmp_path), "c48u27.npy"))[:, :4]).as_matrix()
+        assert np.array_equal(mats, want)
+        assert np.allclose(np.einsum("nij,nkj->nik", mats, mats), np.eye(3), atol=1e-12)
+
+    def test_missing_sets_raise(self, tmp_path, monkeypatch):
+        from pytme_amd.rotations import get_rotation_matrices
+
+        monkeypatch.delenv("PYTME_ROTATION_SETS", raising=False)
+        monkeypatch.setattr("importlib.util.find_spec", lambda name: None)
+        with pytest.raises(FileNotFoundError):
+            get_rotation_matrices(10.0, data_dir=str(tmp_path))
+        monkeypatch.setenv("PYTME_ROTATION_SETS", str(tmp_path))
+        self._write_sets(str(tmp_path))
+        assert get_rotation_matrices(40.0).shape == (9, 3, 3)
+
+    def test_random_sets(self):
+        from pytme_amd.rotations import get_rotation_matrices
+
+        mats = get_rotation_matrices(60.0, dim=3, use_optimized_set=False, seed=1)
+        assert mats.shape == (216, 3, 3) and np.array_equal(mats[0], np.eye(3))
+        assert np.allclose(np.linalg.det(mats), 1.0)
+        two = get_rotation_matrices(30.0, dim=2, seed=2)
+        assert two.shape == (12, 2, 2) and np.allclose(np.linalg.det(two), 1.0)
+
+
+class TestCoordinateMaps:
+    @pytest.mark.parametrize("pad", [False, True])
+    @pytest.mark.parametrize("tshape", [(7, 6, 5), (8, 8, 8)])
+    def test_cart_to_score_indexes_the_raw_volume(self, pad, tshape):
+        # the result volume is the raw volume rolled by fourier_shift and
+        # cropped by the convolution mode (tme/analyzer/aggregation.py:247-261);
+        # cart_to_score(p) must name the raw voxel that lands at p
+        from oracle import scan as orc
+        from pytme_amd.analyzer._utils import cart_to_score, score_to_cart
+
+        target_shape = (20, 17, 15)
+        padded = tuple(n + (t - 1 if pad else 0) for n, t in zip(target_shape, tshape))
+        conv, fast, _, shift = orc.fourier_padding(padded, tshape, pad_target=pad)
+        raw = np.random.default_rng(3).standard_normal(fast)
+        mode = "valid" if pad else "same"
+        result = orc.apply_convolution_mode(np.roll(raw, shift, axis=(0, 1, 2)), mode, padded, tshape, conv)
+        pos = np.argwhere(np.ones(result.shape, bool))
+        kw = dict(fast_shape=fast, targetshape=padded, templateshape=tshape, convolution_shape=conv,
+                  fourier_shift=shift, convolution_mode=mode)
+        raw_pos, valid = cart_to_score(pos, **kw)
+        assert valid.all()
+        assert np.array_equal(raw[tuple(raw_pos.T)], result[tuple(pos.T)])
+        back, ok = score_to_cart(raw_pos, **kw)
+        assert ok.all() and np.array_equal(back, pos)
+        # positions outside the result are flagged
+        outside = np.array([[-1, 0, 0], [result.shape[0], 0, 0]])
+        assert not cart_to_score(outside, **kw)[1].any()
+
+
+def _stand_in_pytme():
+    """A module tree with pyTME's registration surface: a BackendManager
+    whose add_backend checks the package's own MatchingBackend base
+    (tme/backends/__init__.py:82-166) and a score registry whose FLC /
+    correlation entries are functions named flc_scoring / corr_scoring
+    (tme/matching_scores.py:1183-1193)."""
+    import abc
+    import types
+
+    class MatchingBackend(abc.ABC):
+        @abc.abstractmethod
+        def multiply(self, x1, x2, out=None): ...
+
+        @abc.abstractmethod
+        def some_method_the_scan_never_calls(self): ...
+
+    class BackendManager:
+        def __init__(self):
+            self._BACKEND_REGISTRY, self._backend, self._backend_name = {}, None, None
+
+        def add_backend(self, backend_name, backend_class):
+            if not issubclass(backend_class, MatchingBackend):
+                raise ValueError("backend_class needs to inherit from MatchingBackend.")
+            self._BACKEND_REGISTRY[backend_name] = backend_class
+
+        def change_backend(self, backend_name, **kw):
+            self._backend = self._BACKEND_REGISTRY[backend_name](**kw)
+            self._backend_name = backend_name
+
+        def available_backends(self):
+            return ["numpyfftw"]
+
+    calls = []
+
+    def flc_scoring(**kw):
+        calls.append(("flc", kw))
+        return "reference loop"
+
+    def corr_scoring(**kw):
+        calls.append(("corr", kw))
+        return "reference loop"
+
+    def mcc_scoring(**kw):
+        return "reference loop"
+
+    tme = types.ModuleType("tme")
+    tme.backends = types.ModuleType("tme.backends")
+    tme.backends.MatchingBackend = MatchingBackend
+    tme.backends.backend = BackendManager()
+    tme.matching_scores = types.ModuleType("tme.matching_scores")
+    tme.matching_scores.MATCHING_EXHAUSTIVE_REGISTER = {
+        "CC": ("cc_setup", corr_scoring), "FLC": ("flc_setup", flc_scoring), "MCC": ("mcc_setup", mcc_scoring),
+    }
+    return tme, calls
+
+
+class TestIntegration:
+    def test_install_registers_backend_and_routes_scores(self, cpu_backend):
+        from pytme_amd.backends.hip_backend import HipBackend
+        from pytme_amd.integration import install
+
+        tme, calls = _stand_in_pytme()
+        info = install(tme)
+        manager = tme.backends.backend
+        assert "hip" in manager.available_backends() and "numpyfftw" in manager.available_backends()
+        manager.change_backend("hip", device="cpu")
+        assert isinstance(manager._backend, HipBackend)
+        assert isinstance(manager._backend, tme.backends.MatchingBackend)
+        with pytest.raises(NotImplementedError):
+            manager._backend.some_method_the_scan_never_calls()
+        reg = tme.matching_scores.MATCHING_EXHAUSTIVE_REGISTER
+        assert sorted(info["fused_scores"]) == ["CC", "FLC"] and reg["MCC"][1].__name__ == "mcc_scoring"
+        assert reg["FLC"][1].__name__ == "flc_scoring"  # wraps keep the name
+        # the hip backend on the CPU device declines the fused scan: the
+        # reference's own loop runs with the same arguments
+        assert reg["FLC"][1](rotations=None, callback=None) == "reference loop"
+        assert calls[-1][0] == "flc" and set(calls[-1][1]) == {"rotations", "callback"}
+        # a backend whose hook takes the call never reaches the loop
+        manager._backend.fused_scoring = lambda score, **kw: ("fused", score)
+        n = len(calls)
+        assert reg["CC"][1](rotations=None) == ("fused", "corr") and len(calls) == n
+        # installing twice wraps nothing twice
+        before = dict(reg)
+        install(tme)
+        assert reg == before and manager.available_backends().count("hip") == 1
