@@ -218,6 +218,7 @@ def build_scan(workload, target, template, mask, filters, rotations, path, batch
         batch = tuned["batch"]
         be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch, scan_tiles=tuned["tiles"])
     torch.cuda.synchronize()
+    build_scan.callback_args = cb_args
     return scoring, setup, proxy, rot_dev, batch, tuned
 
 
@@ -323,6 +324,45 @@ def bench_refinement(target, template, mask, spec, n_peaks, steps, cpu_rate=None
         "kernels": kern,
         "start_poses": "embedded copies, N(0,4 deg) per zyz angle, N(0,1.5 voxel) per axis",
         "cpu_baseline": cpu,
+    }
+
+
+def bench_called_refinement(proxy, cb_args, target, template, mask, n_peaks, steps, min_distance):
+    """The device chain of the peak-calling and local-optimisation steps of
+    scripts/postprocess.py:482-613 on the scan's own state: roll and crop
+    the max-score and rotation-index volumes, PeakCallerMaximumFilter
+    (n_peaks, min_distance, boundary distance T/2), each peak's rotation
+    from the rotation-index volume and the analyzer's mapping, zyz angles,
+    and the batched gradient refinement (50 Adam steps), all on the device;
+    refined peaks/s over the whole chain (the rotation table and the
+    refinement plan are built before the timed region)."""
+    import torch
+
+    from pytme_amd.postprocess import call_and_refine, rotation_table
+    from pytme_amd.refine import LocalFLC
+
+    table = rotation_table(proxy.state[2], device="cuda")
+    target_dev = torch.as_tensor(target, device="cuda")
+    local = LocalFLC(target_dev, template, mask, max_poses=n_peaks)
+    kw = dict(num_peaks=n_peaks, min_distance=min_distance, steps=steps, local=local)
+    call_and_refine(proxy.state, cb_args, table, target_dev, template, mask, **dict(kw, steps=1))  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, a, s, info = call_and_refine(proxy.state, cb_args, table, target_dev, template, mask, **kw)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n = int(len(s))
+    return {
+        "metric": "refined peaks/s from called peaks (scan state -> peak calling -> gradient refinement, on the device)",
+        "value": round(n / elapsed, 1),
+        "unit": "refined peaks/s",
+        "peaks": n,
+        "steps": steps,
+        "ms_chain": round(elapsed * 1e3, 2),
+        "score_start_mean": round(float(np.mean(info.get("scan_scores", [0.0]))), 4),
+        "local_score_final_mean": round(float(np.mean(s)) if n else 0.0, 4),
+        "note": "candidates are the num_peaks best maxima of the partial max-score volume of the timed steps; "
+                "scan_scores are filtered-template FLC, local scores the unfiltered local FLC",
     }
 
 
@@ -638,6 +678,10 @@ def main():
     if not args.no_peaks and args.path == "fused":
         out["peak_calling"] = bench_peaks(proxy.state[0], args.num_peaks, args.min_distance, 5,
                                           not args.no_cpu_baseline)
+    if not args.no_refinement and args.workload.startswith("flc_512") and args.path == "fused":
+        out["called_peaks_refinement"] = bench_called_refinement(
+            proxy, build_scan.callback_args, target, template, mask, args.refine_peaks, args.refine_steps,
+            args.min_distance)
     if not args.no_refinement and args.workload.startswith("flc_512"):
         out["refinement"] = bench_refinement(target, template, mask, spec, args.refine_peaks, args.refine_steps,
                                              cpu_refine)

@@ -346,10 +346,19 @@ def _refine_fused(local, t0, a0, steps, lr_t, lr_a, bound, bsz, subvoxel):
     return best_t, best_a, best_s
 
 
+def _poses_on_device(x, dev):
+    """(n, 3) float64 device tensor of host or device input (a device tensor
+    stays on the device)."""
+    torch = _torch()
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float64).reshape(-1, 3)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev).reshape(-1, 3)
+
+
 def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_translation: float = 0.1,
                  lr_angle_deg: float = 1.0, bounds_translation: Optional[float] = None,
                  batch: Optional[int] = None, subvoxel: str = "rigid",
-                 fused: bool = True) -> Tuple[np.ndarray, np.ndarray, np.ndarray, dict]:
+                 fused: bool = True, return_device: bool = False) -> Tuple[np.ndarray, np.ndarray, np.ndarray, dict]:
     """Gradient ascent (Adam) on the local FLC of every pose at once.
 
     ``translations`` (n,3) are the template-corner positions the reference's
@@ -362,19 +371,21 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
     gradient ``tm_flc_local_fwd_jac`` and ``tm_refine_adam``); otherwise the
     step goes through the autograd function and ``torch.optim.Adam``.
     Returns the best translations, angles (degrees), scores and evaluation
-    counts.
+    counts; host arrays, or device tensors with ``return_device`` (the
+    poses may be passed as device tensors too: nothing crosses to the host).
     """
     torch = _torch()
     dev = local.device
-    t0 = torch.as_tensor(np.asarray(translations, dtype=np.float64), device=dev).reshape(-1, 3)
-    a0 = torch.deg2rad(torch.as_tensor(np.asarray(angles_deg, dtype=np.float64), device=dev).reshape(-1, 3))
+    t0 = _poses_on_device(translations, dev)
+    a0 = torch.deg2rad(_poses_on_device(angles_deg, dev))
+    out = (lambda x: x) if return_device else (lambda x: x.cpu().numpy())  # noqa: E731
     n = t0.shape[0]
     bsz = min(batch or local.max_poses, local.max_poses)
     if fused:
         bt, ba, bs = _refine_fused(local, t0, a0, steps, lr_translation, math.radians(lr_angle_deg),
                                    bounds_translation, bsz, subvoxel)
         info = {"score_evaluations_per_peak": steps + 1, "gradient_evaluations_per_peak": steps}
-        return bt.cpu().numpy(), np.rad2deg(ba.cpu().numpy()), bs.cpu().numpy(), info
+        return out(bt), out(torch.rad2deg(ba)), out(bs), info
     best_t, best_a = t0.clone(), a0.clone()
     best_s = torch.full((n,), -math.inf, dtype=torch.float64, device=dev)
     t = t0.clone().requires_grad_(True)
@@ -409,4 +420,4 @@ def refine_peaks(local: LocalFLC, translations, angles_deg, steps: int = 50, lr_
             with torch.no_grad():
                 t.copy_(torch.clamp(t, t0 - bounds_translation, t0 + bounds_translation))
     info = {"score_evaluations_per_peak": evals // max(n, 1), "gradient_evaluations_per_peak": steps}
-    return (best_t.cpu().numpy(), np.rad2deg(best_a.cpu().numpy()), best_s.cpu().numpy(), info)
+    return out(best_t), out(torch.rad2deg(best_a)), out(best_s), info

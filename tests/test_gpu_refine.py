@@ -341,3 +341,74 @@ def test_refinement_at_the_large_tomogram_scale():
     assert info["score_evaluations_per_peak"] == 51
     assert np.all(s1 >= s0 - 1e-6)
     assert s1.mean() >= 0.95 * s_true.mean()
+
+
+def test_device_chain_scan_peaks_refinement():
+    """scan -> PeakCallerMaximumFilter -> refinement with the score and
+    rotation volumes, the peaks and the start poses on the device
+    (postprocess.call_and_refine) equals the host flow of
+    scripts/postprocess.py:482-613: peaks of the host result volume by the
+    oracle's caller, each peak's rotation from the rotation array and
+    mapping as scipy zyz angles, then the same refinement fed host arrays."""
+    import torch
+    from scipy.spatial.transform import Rotation
+
+    from oracle import peaks as op
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+    from pytme_amd.backends import backend as be
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+    from pytme_amd.postprocess import call_and_refine
+    from pytme_amd.refine import LocalFLC, refine_peaks
+    from pytme_amd.synthetic import make_workload
+
+    be.change_backend("hip", device="cuda", fused=True)
+    target, template, mask, rots, filters, spec = make_workload("flc_512x512x200_t48", shape=(160, 144, 128),
+                                                                copies=12, n_rotations=48)
+    md = MatchingData(target, template, template_mask=mask, rotations=rots)
+    md.template_filter, md.target_filter = filters
+    setup_fn, scoring = MATCHING_EXHAUSTIVE_REGISTER["FLC"]
+    setup, cb_args, chunks = prepare_scan(md, setup_fn, n_jobs=1, callback_class_args={"score_threshold": 0.0},
+                                          pad_target=False, pad_template_filter=False, interpolation_order=1)
+    proxy = SharedAnalyzerProxy(MaxScoreOverRotations, cb_args)
+    print("device chain: scan 48 rotations of 160x144x128 / 48^3", flush=True)
+    scoring(rotations=chunks[0], callback=proxy, **setup)
+    kw = dict(num_peaks=64, min_distance=6, min_boundary_distance=24)
+    target_dev = torch.as_tensor(target, device="cuda")
+    local = LocalFLC(target_dev, template, mask, max_poses=64)
+    print("  call_and_refine (device)", flush=True)
+    from pytme_amd.postprocess import rotation_table
+
+    table = rotation_table(proxy.state[2], device="cuda")
+    assert torch.equal(table, chunks[0].to(torch.float32))  # one call: indices in call order
+    t_dev, a_dev, s_dev, info = call_and_refine(proxy.state, cb_args, table, target_dev, template, mask,
+                                                steps=10, local=local, **kw)
+    print("  host flow", flush=True)
+    scores, _, rot_array, mapping = MaxScoreOverRotations.merge([proxy.result(**cb_args)], **cb_args)
+    ref = op.PeakCallerOracle("maximum_filter", shape=scores.shape, **kw)
+    pos, _, peak_scores, _ = ref.result(ref(ref.init_state(), np.ascontiguousarray(scores, dtype=np.float32)))
+    keep = (pos >= 0).all(axis=1)
+    pos, peak_scores = pos[keep], peak_scores[keep]
+    idx = rot_array[tuple(pos.T)]
+    pos, peak_scores, idx = pos[idx >= 0], peak_scores[idx >= 0], idx[idx >= 0]
+    assert np.array_equal(info["positions"], pos) and np.array_equal(info["scan_scores"], peak_scores)
+    angles = Rotation.from_matrix(np.stack([np.asarray(mapping[int(i)], dtype=np.float64) for i in idx])
+                                  ).as_euler("zyz", degrees=True)
+    # the scanned matrices are float32, orthogonal to ~1e-7: scipy projects
+    # them onto the nearest rotation first, the device reads their entries
+    assert np.abs(np.angle(np.exp(1j * np.radians(info["start_angles"] - angles)))).max() < 1e-6
+    t0 = pos - np.divide(template.shape, 2)
+    assert np.array_equal(info["start_translations"], t0)
+    # the same start poses fed as host arrays refine to the same poses (Adam
+    # amplifies start differences, so the chain is compared from its own
+    # start angles)
+    t_h, a_h, s_h, _ = refine_peaks(local, t0, info["start_angles"], steps=10, bounds_translation=5.0)
+    assert np.abs(s_dev - s_h).max() < 1e-12 and np.abs(t_dev - t_h).max() < 1e-9
+    # the best pose is tracked from the start pose on, so no peak ends below it
+    from pytme_amd.refine import score_poses
+
+    s_start = score_poses(local, t0, info["start_angles"]).cpu().numpy()
+    assert np.all(s_dev >= s_start - 1e-12) and s_dev.mean() > s_start.mean()
+    print(f"  {len(s_dev)} peaks, mean local score {s_start.mean():.3f} -> {s_dev.mean():.3f}", flush=True)
+    be.free_cache()

@@ -476,3 +476,60 @@ class TestIntegration:
         before = dict(reg)
         install(tme)
         assert reg == before and manager.available_backends().count("hip") == 1
+
+
+class TestPostprocess:
+    def test_euler_zyz_matches_scipy(self):
+        import warnings
+
+        import torch
+        from scipy.spatial.transform import Rotation
+
+        from pytme_amd.postprocess import rotation_matrix_to_euler_zyz
+        from pytme_amd.refine import euler_zyz_matrix
+
+        rots = Rotation.random(500, random_state=4)
+        got = rotation_matrix_to_euler_zyz(torch.as_tensor(rots.as_matrix())).numpy()
+        assert np.abs(np.angle(np.exp(1j * (got - rots.as_euler("zyz"))))).max() < 1e-12
+        # gimbal lock: scipy sets the third angle to 0 and the matrix is kept
+        locked = Rotation.from_euler("zyz", [[10, 0, 20], [30, 180, 0], [10, 180, 40], [-170, 180, 60]], degrees=True)
+        got = rotation_matrix_to_euler_zyz(torch.as_tensor(locked.as_matrix()))
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            want = locked.as_euler("zyz")
+        assert np.abs(np.angle(np.exp(1j * (got.numpy() - want)))).max() < 1e-12
+        assert np.abs(euler_zyz_matrix(got).numpy() - locked.as_matrix()).max() < 1e-12
+
+    @pytest.mark.parametrize("pad", [False, True])
+    def test_device_result_is_the_analyzer_roll_and_crop(self, pad):
+        import torch
+
+        from oracle import scan as orc
+        from pytme_amd.postprocess import device_result
+
+        target_shape, tshape = (20, 17, 15), (7, 6, 5)
+        padded = tuple(n + (t - 1 if pad else 0) for n, t in zip(target_shape, tshape))
+        conv, fast, _, shift = orc.fourier_padding(padded, tshape, pad_target=pad)
+        raw = np.random.default_rng(2).standard_normal(fast).astype(np.float32)
+        rot = np.arange(raw.size, dtype=np.int32).reshape(fast)
+        mode = "valid" if pad else "same"
+        kw = dict(fourier_shift=shift, convolution_mode=mode, targetshape=padded, templateshape=tshape,
+                  convolution_shape=conv)
+        s, r = device_result((torch.as_tensor(raw), torch.as_tensor(rot), {}), **kw)
+        axes = (0, 1, 2)
+        want_s = orc.apply_convolution_mode(np.roll(raw, shift, axes), mode, padded, tshape, conv)
+        want_r = orc.apply_convolution_mode(np.roll(rot, shift, axes), mode, padded, tshape, conv)
+        assert np.array_equal(s.numpy(), want_s) and np.array_equal(r.numpy(), want_r)
+
+
+def test_rotation_table_of_both_mapping_kinds():
+    import torch
+
+    from pytme_amd.postprocess import rotation_table
+
+    mats = torch.as_tensor(np.random.default_rng(0).standard_normal((5, 3, 3)).astype(np.float32))
+    by_index = {i: mats[i] for i in range(5)}
+    assert torch.equal(rotation_table(by_index), mats)
+    by_bytes = {mats[i].numpy().tobytes(): i for i in (3, 1, 4, 0, 2)}
+    assert torch.equal(rotation_table(by_bytes), mats)
+    assert rotation_table({}).shape == (0, 3, 3)
