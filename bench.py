@@ -216,7 +216,8 @@ def build_scan(workload, target, template, mask, filters, rotations, path, batch
                                        and k[3] == tshape][-1]
         tuned = hip_fused.TUNED[key]
         batch = tuned["batch"]
-        be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch, scan_tiles=tuned["tiles"])
+        be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch, scan_tiles=tuned["tiles"],
+                          scan_radix=tuned["radix"])
     torch.cuda.synchronize()
     build_scan.callback_args = cb_args
     return scoring, setup, proxy, rot_dev, batch, tuned
@@ -412,6 +413,7 @@ def bench_multi_template(n_rot, steps):
             "defocus_A": entry["defocus"],
             "autotune_ms_per_rotation": None if tuned is None else tuned["ms_per_rotation"],
             "tiles": None if tuned is None else list(tuned["tiles"]),
+            "radix_plans": None if tuned is None else tuned["radix_plans"],
             "setup_s": round(setup_s, 3),
         }
         total_rot += steps * batch
@@ -639,7 +641,8 @@ def main():
                                "fused_over_plain": round(value / plain, 3)}
         extra["plain_path_rotations_per_s"] = round(plain, 2)
         be.change_backend("hip", device="cuda", fused=True, rotation_batch=batch,
-                          scan_tiles=None if tuned is None else tuned["tiles"])
+                          scan_tiles=None if tuned is None else tuned["tiles"],
+                          scan_radix=None if tuned is None else tuned["radix"])
 
     out = {
         "metric": "FLC rotations/s, 512x512x200 tomogram vs 48^3 template" if args.workload.startswith("flc_512")
@@ -665,6 +668,9 @@ def main():
             "autotune_ms_per_rotation": None if tuned is None else tuned["ms_per_rotation"],
             "autotune_tiles": None if tuned is None else list(tuned["tiles"]),
             "autotune_tiles_ms_per_rotation": None if tuned is None else tuned["tiles_ms_per_rotation"],
+            "autotune_radix": None if tuned is None else tuned["radix"],
+            "radix_plans": None if tuned is None else tuned["radix_plans"],
+            "autotune_radix_ms_per_rotation": None if tuned is None else tuned["radix_ms_per_rotation"],
             "path": args.path,
             "filters": "wedge x CTF template filter, continuous-wedge target filter" if spec["filters"] else None,
         },

@@ -139,6 +139,16 @@ int tm_scan_plan_use_graph(tm_scan_plan* plan, int enable);
  * of 200); unsupported values return an error.  Results do not depend on
  * them. */
 int tm_scan_plan_set_tiles(tm_scan_plan* plan, int pass_a_columns, int pass_c_rows);
+/* No counterpart in the reference (pyFFTW plans with FFTW_MEASURE,
+ * tme/backends/npfftw_backend.py:243-306, choose their own algorithm): the
+ * radix plan variant of the batch passes' compile-time FFTs, chosen by the
+ * backend's autotuner.  Variant 0 takes radix-8 stages first (512 = 8*8*8,
+ * 100 = 4*5*5), variant 1 the reverse order, or radix-4 stages for lengths
+ * of radix-8 stages only (512 = 4*4*4*8).  Results agree to rounding.
+ * tm_fft_radix_plan reports the forward radices of a length in a variant
+ * (count 0: no compile-time plan for n). */
+int tm_scan_plan_set_radix(tm_scan_plan* plan, int variant);
+int tm_fft_radix_plan(int n, int variant, int* radices, int max_out, int* count);
 /* A group of scan plans (several templates of uneven size against one
  * target, each with its own state arrays) stepped one batch per plan per
  * call: tm_scan_group_batch runs, in plan order, one batch of every plan

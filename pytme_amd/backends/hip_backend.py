@@ -100,6 +100,10 @@ class HipBackend(MatchingBackend):
         ``(pass_a_columns, pass_c_rows)`` of the fused passes
         (``tm_scan_plan_set_tiles``); by default the kernels' own choice, or
         with ``rotation_batch="auto"`` the autotuner's.
+    scan_radix : int, optional
+        Radix plan variant (0 or 1, ``tm_scan_plan_set_radix``) of the fused
+        passes' compile-time FFTs; by default 0, or with
+        ``rotation_batch="auto"`` the autotuner's.
     """
 
     #: spline order the rotation kernel implements (reference CLI default for
@@ -117,6 +121,7 @@ class HipBackend(MatchingBackend):
         rotation_batch: int = None,
         scan_graph: bool = True,
         scan_tiles=None,
+        scan_radix=None,
         **kwargs,
     ):
         import torch
@@ -135,8 +140,9 @@ class HipBackend(MatchingBackend):
         self.rotation_batch = rotation_batch
         self.scan_graph = bool(scan_graph)
         self.scan_tiles = None if scan_tiles is None else tuple(int(x) for x in scan_tiles)
+        self.scan_radix = None if scan_radix is None else int(scan_radix)
         self._init_kwargs = dict(device=device, fused=fused, rotation_batch=rotation_batch, scan_graph=scan_graph,
-                                 scan_tiles=scan_tiles, **kwargs)
+                                 scan_tiles=scan_tiles, scan_radix=scan_radix, **kwargs)
 
     # ------------------------------------------------------------------ infra
     def _require_gpu(self, what: str):

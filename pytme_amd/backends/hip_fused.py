@@ -151,13 +151,52 @@ def autotune_batch(backend, key, create, stream, rots, idx_dev, state_shape, can
             finally:
                 _lib.call("tm_scan_plan_destroy", handle)
     best_tiles = min(tiles, key=tiles.get) if tiles else TILE_CANDIDATES[0]
+    # radix plan variants of the compile-time FFTs at the chosen batch and
+    # tiles (their results agree to rounding, checked against the oracle)
+    radix = {}
+    if best in timings:
+        radix[0] = tiles.get(best_tiles, timings[best])
+        for variant in RADIX_CANDIDATES[1:]:
+            handle = create(best)
+            try:
+                _lib.call("tm_scan_plan_set_tiles", handle, *best_tiles)
+                _lib.call("tm_scan_plan_set_radix", handle, variant)
+                radix[variant] = _time_plan(t, handle, stream, rots, idx_dev, best, scores, rot, graph)
+            finally:
+                _lib.call("tm_scan_plan_destroy", handle)
+    best_radix = min(radix, key=radix.get) if radix else 0
     TUNED[key] = {
         "batch": int(best),
         "tiles": tuple(int(x) for x in best_tiles),
+        "radix": int(best_radix),
+        "radix_plans": radix_plans(key[2], int(best_radix)),
         "ms_per_rotation": {int(b): round(v, 5) for b, v in timings.items()},
         "tiles_ms_per_rotation": {f"{a}x{c}": round(v, 5) for (a, c), v in tiles.items()},
+        "radix_ms_per_rotation": {int(v): round(ms, 5) for v, ms in radix.items()},
     }
     return int(best)
+
+
+#: radix plan variants the autotuner times (tm_scan_plan_set_radix)
+RADIX_CANDIDATES = (0, 1)
+
+
+def radix_plans(fast_shape, variant: int):
+    """{"z": "8x8x8", "y": ..., "x": ...}: the forward radices of the batch
+    passes' compile-time FFTs in a radix plan variant (the x pass transforms
+    half the x length); axes without a compile-time plan are omitted."""
+    lib = _lib.lib()
+    out = {}
+    lengths = {"z": int(fast_shape[0]), "y": int(fast_shape[1]), "x": int(fast_shape[2]) // 2}
+    compiled = {"z": (128, 256, 512, 1024), "y": (128, 256, 512, 1024), "x": (64, 100, 128)}
+    for axis, n in lengths.items():
+        if n not in compiled[axis]:
+            continue
+        buf, cnt = (ctypes.c_int * 16)(), ctypes.c_int()
+        _lib.check(lib.tm_fft_radix_plan(n, int(variant), buf, 16, ctypes.byref(cnt)), "tm_fft_radix_plan")
+        if cnt.value:
+            out[axis] = "x".join(str(buf[i]) for i in range(cnt.value))
+    return out
 
 
 def _proxy_state(callback):
@@ -325,6 +364,9 @@ def run(backend, score: str, **kw):
     if tiles is None and auto:
         tiles = TUNED.get(tune_key, {}).get("tiles")
     tiles = (0, 0) if tiles is None else tiles
+    radix = getattr(backend, "scan_radix", None)
+    if radix is None:
+        radix = TUNED.get(tune_key, {}).get("radix", 0) if auto else 0
     if _PLAN["key"] != key:
         _free_plan()
         _PLAN.update(key=key, base=base, handle=create(B), keep=keep)
@@ -335,6 +377,7 @@ def run(backend, score: str, **kw):
         # their version counters; a library call may have written them)
         _lib.call("tm_scan_plan_set_target", _PLAN["handle"], ptr(ft_target), ptr(ft2), stream)
     _lib.call("tm_scan_plan_set_tiles", _PLAN["handle"], *tiles)
+    _lib.call("tm_scan_plan_set_radix", _PLAN["handle"], int(radix))
 
     for start in range(0, n, B):
         nb = min(B, n - start)
@@ -379,6 +422,7 @@ def run_group(backend, kws, stats=None):
                                     fixed, int(job["rots"].shape[0]))
             plans.append(_create_plan(job, job["B"]))
             _lib.call("tm_scan_plan_set_tiles", plans[-1], *tiles)
+            _lib.call("tm_scan_plan_set_radix", plans[-1], int(getattr(backend, "scan_radix", None) or 0))
         n = len(plans)
         handles = (ctypes.c_void_p * n)(*plans)
         gh = ctypes.c_void_p()

@@ -63,11 +63,11 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
 // `inner` are padding that may be transformed), so no load or store is
 // predicated; the compiler can then wait for the prefetched tile without
 // draining the previous tile's stores.
-template <int N, int L, int NT, int DIR, bool FULL>
+template <int N, int L, int NT, int DIR, bool FULL, int V = 0>
 __global__ void __launch_bounds__(NT)
 k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner, int ld,
          int64_t tiles_per_outer, int64_t ntiles, float scale) {
-  using PL = CtPlan<N, L, NT, true, false>;
+  using PL = CtPlan<N, L, NT, true, false, V>;
   constexpr int EI = PL::E_IN, EO = PL::E_OUT, EW = EI > EO ? EI : EO;
   constexpr bool WIDE = L * sizeof(float2) >= 128;  // whole-line row segments: streaming hints
   extern __shared__ float2 lds[];
@@ -139,7 +139,7 @@ static int device_cus() {
   return cus[dev];
 }
 
-template <int N, int L, int NT>
+template <int N, int L, int NT, int V = 0>
 static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
                          int64_t ld, int dir, float scale, hipStream_t s, const char* name) {
   const int64_t tpo = tm_ceil_div(inner, L);
@@ -150,7 +150,7 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   static int per_cu = 0;
   if (per_cu == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_ct<N, L, NT, 1, true>, NT, shm) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_ct<N, L, NT, 1, true, V>, NT, shm) !=
             hipSuccess ||
         nb < 1)
       nb = 1;
@@ -159,12 +159,12 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   int64_t blocks = (int64_t)device_cus() * per_cu;
   if (blocks > ntiles) blocks = ntiles;
   const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
-  using PL = CtPlan<N, L, NT, true, false>;
+  using PL = CtPlan<N, L, NT, true, false, V>;
   // unpredicated form: all slots valid, no pruning, tiles inside the stride
   const bool full = (N / PL::R_IN * L) % NT == 0 && (N / PL::R_OUT * L) % NT == 0 && lin == N && tpo * L <= ld;
 #define TM_C2C_LAUNCH(D, F)                                                                                      \
   TM_TIMED(name, bytes, s,                                                                                      \
-           hipLaunchKernelGGL((k_c2c_ct<N, L, NT, D, F>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw, \
+           hipLaunchKernelGGL((k_c2c_ct<N, L, NT, D, F, V>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw, \
                               lin, inner, (int)ld, tpo, ntiles, scale))
   if (dir < 0) {
     if (full) TM_C2C_LAUNCH(-1, true);
@@ -286,22 +286,33 @@ static int tile_lines(int n, int64_t avail) {
   return t;
 }
 
+template <int V>
+static int launch_c2c_ct_len(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
+                             int64_t ld, int dir, float scale, hipStream_t s, const char* name, bool& done) {
+  done = true;
+  switch (lp.n) {
+    case 128: return launch_c2c_ct<128, 16, 256, V>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+    case 256: return launch_c2c_ct<256, 16, 256, V>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+    case 512:
+      // unpredicated 8-line tiles when 16-line tiles would not fit the stride
+      if (lin == 512 && tm_ceil_div(inner, 16) * 16 > ld && tm_ceil_div(inner, 8) * 8 <= ld)
+        return launch_c2c_ct<512, 8, 512, V>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+      return launch_c2c_ct<512, 16, 512, V>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+    case 1024: return launch_c2c_ct<1024, 8, 512, V>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
+    default: done = false; return TM_OK;
+  }
+}
+
 int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
-                  int64_t ld, int dir, float scale, hipStream_t s, const char* name) {
+                  int64_t ld, int dir, float scale, hipStream_t s, const char* name, int variant) {
   TM_REQUIRE(inner >= 1 && inner <= ld, "C2C: %lld inner lines at stride %lld", (long long)inner, (long long)ld);
+  TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
   // the specialised kernels address a tile with 32-bit offsets
   if (inner >= 12 && (int64_t)lp.n * ld < (1LL << 31)) {
-    switch (lp.n) {
-      case 128: return launch_c2c_ct<128, 16, 256>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
-      case 256: return launch_c2c_ct<256, 16, 256>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
-      case 512:
-        // unpredicated 8-line tiles when 16-line tiles would not fit the stride
-        if (lin == 512 && tm_ceil_div(inner, 16) * 16 > ld && tm_ceil_div(inner, 8) * 8 <= ld)
-          return launch_c2c_ct<512, 8, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
-        return launch_c2c_ct<512, 16, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
-      case 1024: return launch_c2c_ct<1024, 8, 512>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name);
-      default: break;
-    }
+    bool done = false;
+    const int rc = variant ? launch_c2c_ct_len<1>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name, done)
+                           : launch_c2c_ct_len<0>(lp, in, lin, out, outer, inner, ld, dir, scale, s, name, done);
+    if (done) return rc;
   }
   const int TI = tile_lines(lp.n, inner);
   const int64_t tpo = tm_ceil_div(inner, TI);

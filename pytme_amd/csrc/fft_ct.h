@@ -55,25 +55,61 @@ constexpr int ct_factor(int n, int idx) {
   return 0;
 }
 
-constexpr int ct_nstages(int n) {
+constexpr int ct_nstages0(int n) {
   int s = 0;
   while (ct_factor(n, s) != 0) ++s;
   return s;
 }
 
-constexpr bool ct_supported(int n) {
+// Radix plan variants of a length.  Variant 0 is ct_factor's order (radix 8
+// stages first, then 4 or 2, then the odd radices).  Variant 1 takes the
+// stages in the reverse order; where that order is the same (all stages of
+// one radix: 64 = 8*8, 512 = 8*8*8) it splits every pair of radix-8 stages
+// into three radix-4 stages instead (512 = 4*4*4*8).  Both variants have
+// the same radices per thread (8 elements for radix 8 and 4), so a kernel
+// instantiated for either keeps its register ownership.
+constexpr bool ct_palindrome(int n) {
+  const int ns = ct_nstages0(n);
+  for (int s = 0; s < ns; ++s)
+    if (ct_factor(n, s) != ct_factor(n, ns - 1 - s)) return false;
+  return true;
+}
+
+constexpr int ct_eights(int n) {
+  int c = 0;
+  for (int s = 0; s < ct_nstages0(n); ++s) c += ct_factor(n, s) == 8;
+  return c;
+}
+
+constexpr int ct_factor_v(int n, int idx, int variant) {
+  const int ns = ct_nstages0(n);
+  if (variant == 0) return ct_factor(n, idx);
+  if (!ct_palindrome(n) || ct_eights(n) < 2) return idx < ns ? ct_factor(n, ns - 1 - idx) : 0;
+  // pairs of radix-8 stages become three radix-4 stages; the rest follows
+  const int pairs = ct_eights(n) / 2;
+  if (idx < 3 * pairs) return 4;
+  return ct_factor(n, 2 * pairs + (idx - 3 * pairs));
+}
+
+constexpr int ct_nstages(int n, int variant = 0) {
+  int s = 0;
+  while (ct_factor_v(n, s, variant) != 0) ++s;
+  return s;
+}
+
+constexpr bool ct_supported(int n, int variant = 0) {
   int p = 1;
-  for (int s = 0; s < ct_nstages(n); ++s) p *= ct_factor(n, s);
+  for (int s = 0; s < ct_nstages(n, variant); ++s) p *= ct_factor_v(n, s, variant);
   return p == n;
 }
 
-template <int N_, int L_, int NT_, bool LFAST_, bool REV_>
+template <int N_, int L_, int NT_, bool LFAST_, bool REV_, int V_ = 0>
 struct CtPlan {
-  static constexpr int N = N_, L = L_, NT = NT_;
+  static constexpr int N = N_, L = L_, NT = NT_, V = V_;
   static constexpr bool LFAST = LFAST_, REV = REV_;
-  static constexpr int NST = ct_nstages(N_);
-  static_assert(ct_supported(N_), "length not supported by the compile-time FFT");
-  static constexpr int radix(int s) { return REV ? ct_factor(N_, NST - 1 - s) : ct_factor(N_, s); }
+  static constexpr int NST = ct_nstages(N_, V_);
+  static_assert(ct_supported(N_, V_), "length not supported by the compile-time FFT");
+  static constexpr int radix(int s) { return REV ? ct_factor_v(N_, NST - 1 - s, V_) : ct_factor_v(N_, s, V_); }
   static constexpr int ns(int s) {
     int v = 1;
     for (int i = 0; i < s; ++i) v *= radix(i);

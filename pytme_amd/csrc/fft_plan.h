@@ -23,8 +23,10 @@ struct tm_fft_plan {
 // C2C along the middle axis of [outer][lin][ld] -> [outer][n][ld] for the
 // first `inner` <= ld inner indices (lin <= n zero-padded; dir -1 forward,
 // +1 inverse unnormalised).  Inner indices inner..ld-1 are not touched.
+// `variant` picks the radix plan of the compile-time kernels (fft_ct.h:
+// ct_factor_v); the generic path ignores it.
 int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
-                  int64_t ld, int dir, float scale, hipStream_t s, const char* name);
+                  int64_t ld, int dir, float scale, hipStream_t s, const char* name, int variant = 0);
 // R2C of nrows contiguous rows of lin <= n reals (zero-padded to n) into
 // rows of n/2+1 bins at a row stride of ldo >= n/2+1 complex.
 int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,

@@ -33,6 +33,8 @@
 #include "fft_core.h"
 #include "fft_ct.h"
 #include "fft_plan.h"
+#include <type_traits>
+
 #include "tm_common.h"
 
 #define TM_FU_THREADS 256
@@ -330,13 +332,13 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
 // the tail tiles are their own launch (kx_base = ldo) whose stores are
 // predicated to the valid columns, so the main launch stays unpredicated; a
 // tail tile packs L / ntl rows (ky) of ntl columns each.
-template <int N0, int L, int NT, int KIND, bool PR, bool TAIL = false>
+template <int N0, int L, int NT, int KIND, bool PR, bool TAIL = false, int V = 0>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((KIND == 1 && PR && N0 <= 512) ? 4 : 1, 8)))
 k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
             float2* __restrict__ Q, float2* __restrict__ Qt, const float2* __restrict__ tw, int T0, int N1, int N2s,
             int ldo, int B, int tiles_per_row, int kx_base, int ntl, int q_in, int o_x) {
-  using PF = CtPlan<N0, L, NT, true, false>;
-  using PI = CtPlan<N0, L, NT, true, true>;
+  using PF = CtPlan<N0, L, NT, true, false, V>;
+  using PI = CtPlan<N0, L, NT, true, true, V>;
   static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
   static_assert((N0 / PF::R_IN * L) % NT == 0 && (N0 / PF::R_OUT * L) % NT == 0 &&
                     (N0 / PI::R_OUT * L) % NT == 0,
@@ -492,13 +494,13 @@ constexpr int pass_c_ld(int M) { return M + 1; }
 //
 // SPLIT: rows hold ldq main bins in Q and the remaining N2c - ldq bins in
 // the tail array Qt[o][b][y][z][ntl] (row r = z * N1 + y).
-template <int M, int RL, int NT, int NQ, int PD, bool SPLIT = false>
+template <int M, int RL, int NT, int NQ, int PD, bool SPLIT = false, int V = 0>
 __global__ void __launch_bounds__(NT)
 k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const float2* __restrict__ tw2,
             int64_t nrows, int ldq, int N0, int N1, int ntl, int B,
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
             float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
-  using PI = CtPlan<M, RL, NT, false, true>;
+  using PI = CtPlan<M, RL, NT, false, true, V>;
   constexpr int N2 = 2 * M, N2c = M + 1, LD = pass_c_ld(M);
   constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
   constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread and tile
@@ -679,7 +681,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
 
 // ntl > 0: split rows (main rows of ldo bins in Q, ntl tail bins in Qt); the
 // tail tile (kx_base = ldo) is a second launch of each kind.
-template <int N0, int L, int NT, int NT0, bool PR>
+template <int V, int N0, int L, int NT, int NT0, bool PR>
 static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
                                float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int nm, int ntl,
                                int tpr, int B, double bytes, hipStream_t s) {
@@ -691,24 +693,24 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   const unsigned tail_split = ntl > 0 ? (unsigned)std::min<int64_t>(B, std::max<int64_t>(1, tm_ceil_div(512, tail_blocks))) : 1;
   if (flc) {
     TM_TIMED("fused_pass_a", bytes * 0.6 * fm, s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, false, V>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
                                 Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, 0, 0));
     TM_CHECK_LAUNCH();
     if (ntl > 0) {
       TM_TIMED("fused_pass_a_tail", bytes * 0.6 * (1.0 - fm), s,
-               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true>), dim3(tail_blocks, tail_split), dim3(NT0), shm, s, P,
+               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT0), shm, s, P,
                                   ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 0));
       TM_CHECK_LAUNCH();
     }
   }
   TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", (flc ? bytes * 0.4 : bytes) * fm, s,
-           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
+           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, false, V>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
                               (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, flc ? 1 : 0,
                               flc ? 2 : 0));
   TM_CHECK_LAUNCH();
   if (ntl > 0) {
     TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", (flc ? bytes * 0.4 : bytes) * (1.0 - fm), s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true>), dim3(tail_blocks, tail_split), dim3(NT), shm, s, P, ft,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT), shm, s, P, ft,
                                 (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
                                 flc ? 2 : 0));
     TM_CHECK_LAUNCH();
@@ -717,7 +719,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
 }
 
 // NT: threads of the X kernel, NT0: threads of the FLC mask kernel
-template <int N0, int L, int NT, int NT0 = NT>
+template <int V, int N0, int L, int NT, int NT0 = NT>
 static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q, float2* Qt,
                             const float2* tw, int T0, int N1, int N2c, int N2s, int ldo, int ntl, int B, double bytes,
                             hipStream_t s) {
@@ -727,15 +729,15 @@ static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const f
   TM_REQUIRE((int64_t)tpr * L <= N2s, "pass A: spectra rows not padded to whole tiles");
   TM_REQUIRE(ntl == 0 || (nm % L == 0 && nm <= ldo && ntl <= L && nm + L <= N2s),
              "pass A: split rows need main rows of whole tiles and one tail tile inside the padded input rows");
-  using PF = CtPlan<N0, L, NT, true, false>;
+  using PF = CtPlan<N0, L, NT, true, false, V>;
   if (T0 <= N0 / PF::R_IN)
-    return launch_pass_a_ct_pr<N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr, B,
-                                                     bytes, s);
-  return launch_pass_a_ct_pr<N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr, B,
-                                                    bytes, s);
+    return launch_pass_a_ct_pr<V, N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr, B,
+                                                        bytes, s);
+  return launch_pass_a_ct_pr<V, N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr,
+                                                       B, bytes, s);
 }
 
-template <int M, int RL, int NT, int PD3, int PD1, bool SPLIT = false>
+template <int V, int M, int RL, int NT, int PD3, int PD1, bool SPLIT = false>
 static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const float2* tw2, int64_t nrows, int ldq,
                             int N0, int N1, int ntl, int B, const double* n_obs, const float* inv_den,
                             const int* rot_idx, float scale, float eps, float* max_scores, int* max_rot, double bytes,
@@ -751,17 +753,17 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const f
   if (flc) {
     static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3, SPLIT>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
                                 Qt, tw2, nrows, ldq, N0, N1, ntl, B, n_obs, (const float*)nullptr, rot_idx, scale, eps,
                                 max_scores, max_rot));
   } else if (B % PD1 == 0) {
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1, SPLIT>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
                                 Qt, tw2, nrows, ldq, N0, N1, ntl, B, (const double*)nullptr, inv_den, rot_idx, scale,
                                 eps, max_scores, max_rot));
   } else {
     TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1, SPLIT>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
                                 Qt, tw2, nrows, ldq, N0, N1, ntl, B, (const double*)nullptr, inv_den, rot_idx, scale,
                                 eps, max_scores, max_rot));
   }
@@ -797,6 +799,7 @@ struct tm_scan_plan {
   // graph replay of the batch sequence (kernels only) on a plan stream
   int a_cols;  // pass A column tile: 16 (default where the rows allow it) or 8
   int c_rows;  // pass C rows per block for N2 = 200: 5 (default, 128 threads) or 10 (256 threads)
+  int radix;   // radix plan variant of the compile-time passes (fft_ct.h ct_factor_v): 0 or 1
   int use_graph;
   hipStream_t gs;
   hipEvent_t ev_in, ev_out;
@@ -1121,7 +1124,7 @@ int tm_scan_group_create(tm_scan_plan* const* plans, int n, tm_scan_group** out)
   g->k_nb = (int*)calloc(n, sizeof(int));
   g->k_ms = (float**)calloc(n, sizeof(float*));
   g->k_mr = (int**)calloc(n, sizeof(int*));
-  g->k_tiles = (int*)calloc(2 * n, sizeof(int));
+  g->k_tiles = (int*)calloc(3 * n, sizeof(int));
   if (!g->plans || !g->k_nb || !g->k_ms || !g->k_mr || !g->k_tiles) {
     tm_scan_group_destroy(g);
     tm_set_error("plan group: out of host memory");
@@ -1204,14 +1207,15 @@ int tm_scan_group_batch(tm_scan_group* g, const float* const* rots_dev, const in
     }
     float* ms = nbatch[i] ? max_scores[i] : nullptr;
     int* mr = nbatch[i] ? max_rot[i] : nullptr;
-    if (g->k_nb[i] != nbatch[i] || g->k_ms[i] != ms || g->k_mr[i] != mr || g->k_tiles[2 * i] != p->a_cols ||
-        g->k_tiles[2 * i + 1] != p->c_rows)
+    if (g->k_nb[i] != nbatch[i] || g->k_ms[i] != ms || g->k_mr[i] != mr || g->k_tiles[3 * i] != p->a_cols ||
+        g->k_tiles[3 * i + 1] != p->c_rows || g->k_tiles[3 * i + 2] != p->radix)
       same = false;
     g->k_nb[i] = nbatch[i];
     g->k_ms[i] = ms;
     g->k_mr[i] = mr;
-    g->k_tiles[2 * i] = p->a_cols;
-    g->k_tiles[2 * i + 1] = p->c_rows;
+    g->k_tiles[3 * i] = p->a_cols;
+    g->k_tiles[3 * i + 1] = p->c_rows;
+    g->k_tiles[3 * i + 2] = p->radix;
   }
   auto launch_all = [&]() -> int {
     for (int i = 0; i < g->n; ++i) {
@@ -1290,6 +1294,40 @@ int tm_scan_plan_set_tiles(tm_scan_plan* p, int pass_a_columns, int pass_c_rows)
   return TM_OK;
 }
 
+// Radix plan variant (0 or 1, fft_ct.h ct_factor_v) of the compile-time z,
+// y and x passes of the batch.
+int tm_scan_plan_set_radix(tm_scan_plan* p, int variant) {
+  TM_REQUIRE(p, "null plan");
+  TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
+  if (p->radix != variant) {
+    p->radix = variant;
+    if (p->gexec) {
+      TM_CHECK_HIP(hipGraphExecDestroy(p->gexec));
+      p->gexec = nullptr;
+    }
+    p->g_warm = 0;
+  }
+  return TM_OK;
+}
+
+// Radices (in stage order) of the forward compile-time plan of length n in
+// a variant; *count = 0 when n has no compile-time plan.
+int tm_fft_radix_plan(int n, int variant, int* radices, int max_out, int* count) {
+  TM_REQUIRE(radices && count && max_out >= 1, "null argument");
+  TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
+  *count = 0;
+  if (n < 2 || n > 4096) return TM_OK;
+  int prod = 1, k = 0;
+  while (k < max_out) {
+    const int r = ct_factor_v(n, k, variant);
+    if (!r) break;
+    radices[k++] = r;
+    prod *= r;
+  }
+  if (prod == n) *count = k;
+  return TM_OK;
+}
+
 }  // extern "C"
 
 static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch,
@@ -1345,7 +1383,8 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   rc = tm_launch_r2c_rows(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, N2s, s,
                           "fused_x_pass");
   if (rc != TM_OK) return rc;
-  rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, N2s, -1, 1.0f, s, "fused_y_pass");
+  rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, N2s, -1, 1.0f, s, "fused_y_pass",
+                     p->radix);
   if (rc != TM_OK) return rc;
 
   // 5. pass A: z-axis, spectral products
@@ -1355,17 +1394,21 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   bool done_a = true;
   // 16-bin column tiles when they fit the padded rows, else 8-bin tiles
   const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s && p->a_cols != 8;
+  auto pass_a = [&](auto vtag) -> int {
+    constexpr int V = decltype(vtag)::value;
 #define TM_PASS_A(N0_, L_, ...) \
-  launch_pass_a_ct<N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, p->Qt, lz.tw, T0, N1, N2c, N2s, p->N2m, \
-                                         p->ntl, B, bytes_a, s)
-  switch (N0) {
-    case 128: rc = l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128); break;
-    case 256: rc = l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256); break;
-    case 512: rc = l16 ? TM_PASS_A(512, 16, 512, 1024) : TM_PASS_A(512, 8, 512); break;
-    case 1024: rc = TM_PASS_A(1024, 8, 512); break;
+  launch_pass_a_ct<V, N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, p->Qt, lz.tw, T0, N1, N2c, N2s, p->N2m, \
+                                            p->ntl, B, bytes_a, s)
+    switch (N0) {
+      case 128: return l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128);
+      case 256: return l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256);
+      case 512: return l16 ? TM_PASS_A(512, 16, 512, 1024) : TM_PASS_A(512, 8, 512);
+      case 1024: return TM_PASS_A(1024, 8, 512);
 #undef TM_PASS_A
-    default: done_a = false; break;
-  }
+      default: done_a = false; return TM_OK;
+    }
+  };
+  rc = p->radix ? pass_a(std::integral_constant<int, 1>{}) : pass_a(std::integral_constant<int, 0>{});
   if (done_a && rc != TM_OK) return rc;
   if (!done_a) {
     int TI = 4096 / N0;
@@ -1395,12 +1438,13 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     // split rows: the main rows (whole 16-bin tiles), then the tail array
     // (lines (z, c) of N0 * ntl, whole 128-byte segments)
     rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c - p->ntl, p->N2m, 1, 1.0f, s,
-                       "fused_pass_b");
+                       "fused_pass_b", p->radix);
     if (rc != TM_OK) return rc;
     rc = tm_launch_c2c(ly, p->Qt, N1, p->Qt, (int64_t)p->nq_out * B, (int64_t)N0 * p->ntl, (int64_t)N0 * p->ntl, 1,
-                       1.0f, s, "fused_pass_b_tail");
+                       1.0f, s, "fused_pass_b_tail", p->radix);
   } else {
-    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, N2s, 1, 1.0f, s, "fused_pass_b");
+    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c, N2s, 1, 1.0f, s, "fused_pass_b",
+                       p->radix);
   }
   if (rc != TM_OK) return rc;
 
@@ -1413,28 +1457,31 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     const float scale = (float)(1.0 / F);
     const float eps = 1.1920928955078125e-07f;
     done_c = true;
+    auto pass_c = [&](auto vtag) -> int {
+    constexpr int V = decltype(vtag)::value;
     switch (N2) {
-      case 128: rc = launch_pass_c_ct<64, 32, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s); break;
+      case 128: return launch_pass_c_ct<V, 64, 32, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
       case 200:
         if (p->ntl) {  // split rows: main bins in Q (row stride N2m), the tail bins in Qt
           if (p->c_rows == 10)
-            rc = launch_pass_c_ct<100, 10, 256, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+            return launch_pass_c_ct<V, 100, 10, 256, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
           else
-            rc = launch_pass_c_ct<100, 5, 128, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+            return launch_pass_c_ct<V, 100, 5, 128, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         } else if (p->c_rows == 10) {
-          rc = launch_pass_c_ct<100, 10, 256, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 100, 10, 256, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         } else {
-          rc = launch_pass_c_ct<100, 5, 128, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 100, 5, 128, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         }
-        break;
       case 256:
         if (p->ntl)  // split rows: 128 bins in Q, the Nyquist bin in Qt
-          rc = launch_pass_c_ct<128, 16, 256, 1, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 128, 16, 256, 1, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         else
-          rc = launch_pass_c_ct<128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
-        break;
-      default: done_c = false; break;
+          return launch_pass_c_ct<V, 128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+      default: done_c = false;
     }
+    return TM_OK;
+    };
+    rc = p->radix ? pass_c(std::integral_constant<int, 1>{}) : pass_c(std::integral_constant<int, 0>{});
     if (done_c && rc != TM_OK) return rc;
   }
   if (!done_c) {

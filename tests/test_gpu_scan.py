@@ -626,3 +626,61 @@ def test_installed_pytme_reaches_the_fused_scan(monkeypatch):
     n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
     be.free_cache()
+
+
+@pytest.mark.parametrize("shape,T", [((256, 128, 200), 24), ((512, 64, 256), 32), ((128, 256, 128), 24),
+                                     ((1024, 48, 256), 32)])
+def test_radix_variant_vs_oracle(shape, T):
+    """The fused scan with the second radix plan of the compile-time FFTs
+    (tm_scan_plan_set_radix 1: 256 = 4*8*8, 512 = 4*4*4*8, 1024 = 4*4*8*8,
+    128 = 4*4*8 for z / y, 100 = 5*5*4, 128 = 4*4*8, 64 = 4*4*4 for the x
+    pass) against the FP64 oracle; the two plans round differently, so each
+    is checked against the oracle rather than against the other."""
+    from pytme_amd.rotations import uniform_rotations
+    from pytme_amd.synthetic import blob_template, embed_copies, threshold_mask
+
+    template = blob_template(T, n_blobs=10, seed=95)
+    mask = threshold_mask(template)
+    target, _, _ = embed_copies(shape, template, mask, 3, 0.5, seed=96)
+    rots = uniform_rotations(3, seed=97, include_identity=True)
+    _, _, raws, oc = orc.scan("FLC", target, template, rots, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True, workers=8)
+    for radix in (0, 1):
+        print(f"radix plan {radix}, {shape} / {T}^3", flush=True)
+        s, off, r, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, scan_radix=radix)
+        n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+        print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+def test_c1_radix_variant_vs_fp64_oracle():
+    """The benchmark configuration with the second radix plan (512 = 4*4*4*8
+    for z and y, 100 = 5*5*4 for x) against the FP64 oracle, 2 rotations."""
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.synthetic import make_workload
+
+    target, template, mask, rotations, filters, spec = make_workload("flc_512x512x200_t48", n_rotations=2)
+    print("C1 full size, radix plan 1, fused vs FP64 oracle", flush=True)
+    s, off, r, _ = _run("FLC", target, template, mask, rotations, True, threshold=-1.0, scan_radix=1,
+                        template_filter=filters[0], target_filter=filters[1])
+    conv, fast, ft, shift = MatchingData._fourier_padding(target.shape, template.shape)
+    tfilt = filters[0](shape=template.shape, return_real_fourier=True)["data"]
+    gfilt = filters[1](shape=fast, return_real_fourier=True)["data"]
+    _, _, raws, oc = orc.scan("FLC", target, template, rotations, template_mask=mask, score_threshold=-1.0,
+                              return_raw=True, template_filter=tfilt, target_filter=gfilt, workers=8)
+    n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
+    print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
+
+
+def test_autotuner_reports_the_radix_plan():
+    """rotation_batch="auto" times both radix plans at the chosen batch and
+    tiles and records the choice with the plans' radices."""
+    from pytme_amd.backends import hip_fused
+
+    target, template, mask, rots = _workload(seed=81, shape=(128, 128, 100), T=16, n_rot=16)
+    before = set(hip_fused.TUNED)
+    _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch="auto")
+    added = [k for k in hip_fused.TUNED if k not in before]
+    assert added
+    rec = hip_fused.TUNED[added[-1]]
+    assert rec["radix"] in (0, 1) and set(rec["radix_ms_per_rotation"]) == {0, 1}
+    assert rec["radix_plans"]["z"] in ("8x4x4", "4x4x8") and rec["radix_plans"]["y"] in ("8x4x4", "4x4x8")

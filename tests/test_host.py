@@ -533,3 +533,26 @@ def test_rotation_table_of_both_mapping_kinds():
     by_bytes = {mats[i].numpy().tobytes(): i for i in (3, 1, 4, 0, 2)}
     assert torch.equal(rotation_table(by_bytes), mats)
     assert rotation_table({}).shape == (0, 3, 3)
+
+
+def test_fft_radix_plans():
+    import ctypes
+
+    from pytme_amd import _lib
+
+    lib = _lib.lib()
+
+    def plan(n, v):
+        buf, cnt = (ctypes.c_int * 16)(), ctypes.c_int()
+        assert lib.tm_fft_radix_plan(n, v, buf, 16, ctypes.byref(cnt)) == 0
+        return [buf[i] for i in range(cnt.value)]
+
+    assert plan(512, 0) == [8, 8, 8] and plan(512, 1) == [4, 4, 4, 8]
+    assert plan(100, 0) == [4, 5, 5] and plan(100, 1) == [5, 5, 4]
+    assert plan(1024, 0) == [8, 8, 4, 4] and plan(1024, 1) == [4, 4, 8, 8]
+    assert plan(256, 1) == [4, 8, 8] and plan(128, 1) == [4, 4, 8] and plan(64, 1) == [4, 4, 4]
+    for n in (48, 96, 100, 128, 200, 256, 384, 512, 640, 1024):
+        for v in (0, 1):
+            assert int(np.prod(plan(n, v))) == n, (n, v)
+    assert plan(11, 0) == [] and plan(22, 1) == []
+    assert lib.tm_fft_radix_plan(512, 2, (ctypes.c_int * 4)(), 4, ctypes.byref(ctypes.c_int())) != 0
