@@ -131,8 +131,14 @@ int tm_scan_batch(tm_scan_plan* plan, const float* rotations, const int* rotatio
 /* No counterpart in the reference: replay the batch's kernel sequence as a
  * hipGraph on a plan-owned stream (event-ordered with the caller's stream);
  * the first batch of a (batch size, state arrays) configuration runs eagerly,
- * the second is captured.  Off by default; per-kernel timing forces eager. */
+ * the second is captured, later ones replay.  Off by default; per-kernel
+ * timing forces eager. */
 int tm_scan_plan_use_graph(tm_scan_plan* plan, int enable);
+/* No counterpart in the reference: how many of the plan's graph-mode
+ * batches were captured, replayed and run eagerly.  The plan keeps the
+ * captured sequences of two configurations (a scan's full batches and its
+ * remainder batch), so alternating full and partial batches keep replaying. */
+int tm_scan_plan_graph_stats(const tm_scan_plan* plan, int64_t* captures, int64_t* replays, int64_t* eager);
 /* No counterpart in the reference: tile shapes of the batch passes, chosen
  * by the backend's autotuner.  pass_a_columns 16 or 8 (0: 16 where the rows
  * are padded to 16 bins), pass_c_rows 5 or 10 (0: 5; 10 only for an x length

@@ -74,6 +74,7 @@ SIGNATURES = {
     "tm_scan_plan_destroy": (c_int, [c_void_p]),
     "tm_scan_plan_set_target": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "tm_scan_plan_use_graph": (c_int, [c_void_p, c_int]),
+    "tm_scan_plan_graph_stats": (c_int, [c_void_p, c_int64_p, c_int64_p, c_int64_p]),
     "tm_scan_plan_set_tiles": (c_int, [c_void_p, c_int, c_int]),
     "tm_scan_plan_set_radix": (c_int, [c_void_p, c_int]),
     "tm_fft_radix_plan": (c_int, [c_int, c_int, c_int_p, c_int, c_int_p]),

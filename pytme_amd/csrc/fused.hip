@@ -821,10 +821,19 @@ struct tm_scan_plan {
   hipEvent_t ev_in, ev_out;
   float* rbuf;  // rotations and analyzer indices of the batch, read by the graph
   int* ibuf;
-  hipGraphExec_t gexec;
-  int g_nb, g_warm;
-  float* g_ms;
-  int* g_mr;
+  // captured batch sequences: one per (batch size, state arrays), two kept
+  // (the full batch and a call's remainder batch), least recently used
+  // replaced; a configuration runs eagerly once (warm), is captured on its
+  // second occurrence and replayed after that
+  struct GraphSlot {
+    hipGraphExec_t exec;
+    int nb, warm;
+    float* ms;
+    int* mr;
+    uint64_t used;
+  } gslot[2];
+  uint64_t g_clock;
+  int64_t g_captures, g_replays, g_eager;
 };
 
 extern "C" int tm_fft_plan_create(int ndim, const int64_t* shape, int64_t batch, tm_fft_plan** out);
@@ -836,7 +845,8 @@ static void scan_plan_free(tm_scan_plan* p) {
                   p->rbuf, p->ibuf};
   for (void* b : bufs)
     if (b) hipFree(b);
-  if (p->gexec) hipGraphExecDestroy(p->gexec);
+  for (auto& g : p->gslot)
+    if (g.exec) hipGraphExecDestroy(g.exec);
   if (p->ev_in) hipEventDestroy(p->ev_in);
   if (p->ev_out) hipEventDestroy(p->ev_out);
   if (p->gs) hipStreamDestroy(p->gs);
@@ -1031,48 +1041,65 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
 // the rotations and indices from plan buffers, so only the state arrays and
 // the batch size key it; the first batch of a configuration runs eagerly
 // (it also settles every cached launch attribute), the second captures.
+// Forget every captured batch sequence (the kernel sequence changed).
+static int scan_graph_reset(tm_scan_plan* p) {
+  for (auto& g : p->gslot) {
+    if (g.exec) TM_CHECK_HIP(hipGraphExecDestroy(g.exec));
+    g.exec = nullptr;
+    g.warm = 0;
+    g.used = 0;
+  }
+  return TM_OK;
+}
+
 static int scan_batch_graph(tm_scan_plan* p, const float* rots_dev, const int* rot_index_dev, int nbatch,
                             float* max_scores, int* max_rot, hipStream_t s) {
   TM_CHECK_HIP(hipEventRecord(p->ev_in, s));
   TM_CHECK_HIP(hipStreamWaitEvent(p->gs, p->ev_in, 0));
   TM_CHECK_HIP(hipMemcpyAsync(p->rbuf, rots_dev, sizeof(float) * 9 * nbatch, hipMemcpyDeviceToDevice, p->gs));
   TM_CHECK_HIP(hipMemcpyAsync(p->ibuf, rot_index_dev, sizeof(int) * nbatch, hipMemcpyDeviceToDevice, p->gs));
-  const bool same = p->gexec && p->g_nb == nbatch && p->g_ms == max_scores && p->g_mr == max_rot;
+  tm_scan_plan::GraphSlot* slot = nullptr;
+  for (auto& g : p->gslot)
+    if (g.warm && g.nb == nbatch && g.ms == max_scores && g.mr == max_rot) slot = &g;
   int rc = TM_OK;
-  if (!same) {
-    if (p->gexec) {
-      TM_CHECK_HIP(hipGraphExecDestroy(p->gexec));
-      p->gexec = nullptr;
+  if (!slot) {
+    // a new configuration: replace the least recently used slot, run eagerly
+    slot = p->gslot[0].used <= p->gslot[1].used ? &p->gslot[0] : &p->gslot[1];
+    if (slot->exec) {
+      TM_CHECK_HIP(hipGraphExecDestroy(slot->exec));
+      slot->exec = nullptr;
     }
-    if (!p->g_warm || p->g_nb != nbatch || p->g_ms != max_scores || p->g_mr != max_rot) {
-      p->g_warm = 1;
-      p->g_nb = nbatch;
-      p->g_ms = max_scores;
-      p->g_mr = max_rot;
-      rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch, max_scores, max_rot, p->gs);
-    } else {
-      hipGraph_t g = nullptr;
-      TM_CHECK_HIP(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
-      rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch, max_scores, max_rot, p->gs);
-      const hipError_t e = hipStreamEndCapture(p->gs, &g);
-      if (rc == TM_OK && e != hipSuccess) {
-        tm_set_error("capturing the batch graph: %s", hipGetErrorString(e));
+    slot->warm = 1;
+    slot->nb = nbatch;
+    slot->ms = max_scores;
+    slot->mr = max_rot;
+    ++p->g_eager;
+    rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch, max_scores, max_rot, p->gs);
+  } else if (!slot->exec) {
+    hipGraph_t g = nullptr;
+    TM_CHECK_HIP(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
+    rc = scan_batch_launch(p, p->rbuf, p->ibuf, nbatch, max_scores, max_rot, p->gs);
+    const hipError_t e = hipStreamEndCapture(p->gs, &g);
+    if (rc == TM_OK && e != hipSuccess) {
+      tm_set_error("capturing the batch graph: %s", hipGetErrorString(e));
+      rc = TM_ERR_HIP;
+    }
+    if (rc == TM_OK) {
+      const hipError_t ie = hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        tm_set_error("instantiating the batch graph: %s", hipGetErrorString(ie));
+        slot->exec = nullptr;
         rc = TM_ERR_HIP;
       }
-      if (rc == TM_OK) {
-        const hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-          tm_set_error("instantiating the batch graph: %s", hipGetErrorString(ie));
-          p->gexec = nullptr;
-          rc = TM_ERR_HIP;
-        }
-      }
-      if (g) hipGraphDestroy(g);
-      if (rc == TM_OK) rc = hipGraphLaunch(p->gexec, p->gs) == hipSuccess ? TM_OK : TM_ERR_HIP;
     }
+    if (g) hipGraphDestroy(g);
+    if (rc == TM_OK) rc = hipGraphLaunch(slot->exec, p->gs) == hipSuccess ? TM_OK : TM_ERR_HIP;
+    ++p->g_captures;
   } else {
-    TM_CHECK_HIP(hipGraphLaunch(p->gexec, p->gs));
+    TM_CHECK_HIP(hipGraphLaunch(slot->exec, p->gs));
+    ++p->g_replays;
   }
+  slot->used = ++p->g_clock;
   if (rc != TM_OK) return rc;
   TM_CHECK_HIP(hipEventRecord(p->ev_out, p->gs));
   TM_CHECK_HIP(hipStreamWaitEvent(s, p->ev_out, 0));
@@ -1301,12 +1328,19 @@ int tm_scan_plan_set_tiles(tm_scan_plan* p, int pass_a_columns, int pass_c_rows)
   if (p->a_cols != pass_a_columns || p->c_rows != pass_c_rows) {
     p->a_cols = pass_a_columns;
     p->c_rows = pass_c_rows;
-    if (p->gexec) {
-      TM_CHECK_HIP(hipGraphExecDestroy(p->gexec));
-      p->gexec = nullptr;
-    }
-    p->g_warm = 0;
+    const int rc = scan_graph_reset(p);
+    if (rc != TM_OK) return rc;
   }
+  return TM_OK;
+}
+
+// Counts of the plan's graph-mode batches: captured, replayed and run
+// eagerly (the first occurrence of a configuration).
+int tm_scan_plan_graph_stats(const tm_scan_plan* p, int64_t* captures, int64_t* replays, int64_t* eager) {
+  TM_REQUIRE(p && captures && replays && eager, "null argument");
+  *captures = p->g_captures;
+  *replays = p->g_replays;
+  *eager = p->g_eager;
   return TM_OK;
 }
 
@@ -1317,11 +1351,8 @@ int tm_scan_plan_set_radix(tm_scan_plan* p, int variant) {
   TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
   if (p->radix != variant) {
     p->radix = variant;
-    if (p->gexec) {
-      TM_CHECK_HIP(hipGraphExecDestroy(p->gexec));
-      p->gexec = nullptr;
-    }
-    p->g_warm = 0;
+    const int rc = scan_graph_reset(p);
+    if (rc != TM_OK) return rc;
   }
   return TM_OK;
 }

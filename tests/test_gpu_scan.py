@@ -684,3 +684,45 @@ def test_autotuner_reports_the_radix_plan():
     rec = hip_fused.TUNED[added[-1]]
     assert rec["radix"] in (0, 1) and set(rec["radix_ms_per_rotation"]) == {0, 1}
     assert rec["radix_plans"]["z"] in ("8x4x4", "4x4x8") and rec["radix_plans"]["y"] in ("8x4x4", "4x4x8")
+
+
+def test_full_and_partial_batches_keep_replaying():
+    """Scans whose rotation count is not a multiple of the batch (20 rotations
+    in batches of 8: 8, 8, 4) called repeatedly on a kept plan and the same
+    state arrays: after the first call's eager runs and captures, the full
+    and the remainder batch both replay (tm_scan_plan_graph_stats), and every
+    call equals the same scan on an eager plan."""
+    import ctypes
+
+    from pytme_amd import _lib
+    from pytme_amd.analyzer import MaxScoreOverRotations
+    from pytme_amd.analyzer.proxy import SharedAnalyzerProxy
+    from pytme_amd.backends import backend as be
+    from pytme_amd.backends import hip_fused
+    from pytme_amd.matching_data import MatchingData
+    from pytme_amd.matching_exhaustive import MATCHING_EXHAUSTIVE_REGISTER, prepare_scan
+
+    target, template, mask, rots = _workload(seed=63, shape=(40, 36, 30), n_rot=20)
+    eager = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=8, scan_graph=False)
+    be.change_backend("hip", device="cuda", fused=True, rotation_batch=8, scan_graph=True)
+    setup_fn, scoring = MATCHING_EXHAUSTIVE_REGISTER["FLC"]
+    md = MatchingData(target.copy(), template.copy(), template_mask=mask.copy(), rotations=rots)
+    setup, cb_args, chunks = prepare_scan(md, setup_fn, n_jobs=1, callback_class_args={"score_threshold": -1.0},
+                                          pad_target=False, pad_template_filter=False, interpolation_order=1)
+    proxy = SharedAnalyzerProxy(MaxScoreOverRotations, cb_args)
+    scores, rot_idx, mapping = proxy.state
+    for call in range(3):
+        print(f"partial batches, call {call}", flush=True)
+        scores.fill_(-1.0)
+        rot_idx.fill_(-1)
+        mapping.clear()
+        scoring(rotations=chunks[0], callback=proxy, **setup)
+        s, off, r, _ = MaxScoreOverRotations.merge([proxy.result(**cb_args)], **cb_args)
+        assert np.array_equal(s, eager[0]) and np.array_equal(r, eager[2])
+    cap, rep, eag = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _lib.call("tm_scan_plan_graph_stats", hip_fused._PLAN["handle"], ctypes.byref(cap), ctypes.byref(rep),
+              ctypes.byref(eag))
+    # call 0: 8 eager, 8 captured, 4 eager; call 1: 8 and 8 replayed, 4
+    # captured; call 2: all three replayed
+    assert (eag.value, cap.value, rep.value) == (2, 2, 5), (eag.value, cap.value, rep.value)
+    be.free_cache()
