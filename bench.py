@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP32_PEAK_GFLOPS = 157300.0  # MI355X FP32 (the kernels issue non-packed FP32: half of it is their ceiling)
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the default workload (scaled by
 # a 1 GiB copy); its traffic_bytes_per_launch line feeds roofline.traffic
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48_b64.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fused_scan_flc_512x512x200_t48.md")
 
 
 #: kernels whose per-launch traffic is proportional to the batch (their
