@@ -696,32 +696,37 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
 template <int V, int N0, int L, int NT, int NT0, bool PR>
 static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q,
                                float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int nm, int ntl,
-                               int tpr, int B, double bytes, hipStream_t s) {
+                               int tpr, int B, hipStream_t s) {
   const int64_t blocks = (int64_t)N1 * tpr;
   const size_t shm = (size_t)N0 * L * sizeof(float2);
-  const double fm = ntl > 0 ? (double)nm / (nm + ntl) : 1.0;  // the main launch's share of the bytes
+  // algorithmic bytes of a launch over `bins` bins of every row: the target
+  // spectra it multiplies by (nin), the T0 pruned planes of one template-side
+  // input per rotation and its nout outputs per rotation
+  auto bytes_of = [&](int bins, int nin, int nout) {
+    return 8.0 * bins * N1 * ((double)N0 * nin + (double)T0 * B + (double)N0 * nout * B);
+  };
   const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
   // the tail blocks are fewer than the CUs: split their rotations over gridDim.y
   const unsigned tail_split = ntl > 0 ? (unsigned)std::min<int64_t>(B, std::max<int64_t>(1, tm_ceil_div(512, tail_blocks))) : 1;
   if (flc) {
-    TM_TIMED("fused_pass_a", bytes * 0.6 * fm, s,
+    TM_TIMED("fused_pass_a", bytes_of(nm, 2, 2), s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, false, V>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
                                 Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, 0, 0));
     TM_CHECK_LAUNCH();
     if (ntl > 0) {
-      TM_TIMED("fused_pass_a_tail", bytes * 0.6 * (1.0 - fm), s,
+      TM_TIMED("fused_pass_a_tail", bytes_of(ntl, 2, 2), s,
                hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT0), shm, s, P,
                                   ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 0));
       TM_CHECK_LAUNCH();
     }
   }
-  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", (flc ? bytes * 0.4 : bytes) * fm, s,
+  TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", bytes_of(nm, 1, 1), s,
            hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, false, V>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
                               (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, flc ? 1 : 0,
                               flc ? 2 : 0));
   TM_CHECK_LAUNCH();
   if (ntl > 0) {
-    TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", (flc ? bytes * 0.4 : bytes) * (1.0 - fm), s,
+    TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", bytes_of(ntl, 1, 1), s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT), shm, s, P, ft,
                                 (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
                                 flc ? 2 : 0));
@@ -733,7 +738,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
 // NT: threads of the X kernel, NT0: threads of the FLC mask kernel
 template <int V, int N0, int L, int NT, int NT0 = NT>
 static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const float2* ft2, float2* Q, float2* Qt,
-                            const float2* tw, int T0, int N1, int N2c, int N2s, int ldo, int ntl, int B, double bytes,
+                            const float2* tw, int T0, int N1, int N2c, int N2s, int ldo, int ntl, int B,
                             hipStream_t s) {
   TM_REQUIRE((int64_t)N0 * N1 * N2s < (1LL << 31), "pass A: z column stride too large");
   const int nm = N2c - ntl;  // main bins (split rows) or N2c
@@ -744,9 +749,9 @@ static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const f
   using PF = CtPlan<N0, L, NT, true, false, V>;
   if (T0 <= N0 / PF::R_IN)
     return launch_pass_a_ct_pr<V, N0, L, NT, NT0, true>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr, B,
-                                                        bytes, s);
+                                                        s);
   return launch_pass_a_ct_pr<V, N0, L, NT, NT0, false>(flc, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, nm, ntl, tpr,
-                                                       B, bytes, s);
+                                                       B, s);
 }
 
 template <int V, int M, int RL, int NT, int PD3, int PD1, bool SPLIT = false>
@@ -1435,9 +1440,6 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   if (rc != TM_OK) return rc;
 
   // 5. pass A: z-axis, spectral products
-  const double C_ = (double)N0 * N1 * N2c;
-  const double bytes_a = 8.0 * C_ * (flc ? 2 : 1) + 8.0 * p->nq_in * B * (double)T0 * N1 * N2c +
-                         8.0 * C_ * p->nq_out * B;
   bool done_a = true;
   // 16-bin column tiles when they fit the padded rows, else 8-bin tiles
   const bool l16 = tm_ceil_div(N2c, 16) * 16 <= N2s && p->a_cols != 8;
@@ -1445,7 +1447,7 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     constexpr int V = decltype(vtag)::value;
 #define TM_PASS_A(N0_, L_, ...) \
   launch_pass_a_ct<V, N0_, L_, __VA_ARGS__>(flc, p->P, p->ft, p->ft2, p->Q, p->Qt, lz.tw, T0, N1, N2c, N2s, p->N2m, \
-                                            p->ntl, B, bytes_a, s)
+                                            p->ntl, B, s)
     switch (N0) {
       case 128: return l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128);
       case 256: return l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256);
