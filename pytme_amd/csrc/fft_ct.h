@@ -16,6 +16,8 @@
 // same per-thread ownership: spectra can be multiplied in registers between
 // a forward and an inverse transform.
 #pragma once
+#include <type_traits>
+
 #include "fft_core.h"
 
 constexpr int ct_factor(int n, int idx) {
@@ -238,9 +240,16 @@ __device__ __forceinline__ void ct_twiddle(float2* v, const float2* __restrict__
   ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, base));
 }
 
-// middle stage s (reads LDS, writes LDS)
-template <class P, int DIR, int LD, int TWS, int S>
-__device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid) {
+// No work between stages (the default hook of ct_fft).
+struct ct_no_hook {
+  template <class K>
+  __device__ __forceinline__ void operator()(K) const {}
+};
+
+// middle stage s (reads LDS, writes LDS); hook(integral_constant<S>) runs
+// after the stage's LDS writes, before the barrier
+template <class P, int DIR, int LD, int TWS, int S, class H>
+__device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid, H& hook) {
   if constexpr (S < P::NST - 1) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
     // the twiddle is read with the data, before the barrier: the table is
@@ -272,18 +281,22 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
         for (int r = 0; r < R; ++r) lds[ct_lds<P, LD>(l, base + r * NS)] = v[i][r];
       }
     }
+    hook(std::integral_constant<int, S>{});
     lds_barrier();
-    ct_mid<P, DIR, LD, TWS, S + 1>(lds, tw, tid);
+    ct_mid<P, DIR, LD, TWS, S + 1>(lds, tw, tid, hook);
   }
 }
 
 // Transform: in[E_IN] (first-stage ownership) -> out[E_OUT] (last-stage
 // ownership).  All threads of the block must call it.  `lds` holds at least
 // the L lines at stride LD.  TWS: the table tw has TWS*N entries of
-// exp(-2 pi i k / (TWS*N)).
-template <class P, int DIR, int LD, int TWS = 1>
+// exp(-2 pi i k / (TWS*N)).  hook(integral_constant<s>) is called after the
+// LDS writes of stage s = 0 .. NST-2, before the barrier that ends it: a
+// caller issues independent work there (stores of an earlier result) so
+// that it spreads over the transform instead of coming in one burst.
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
 __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ tw, float2* in, float2* out,
-                                       int tid) {
+                                       int tid, H hook = H{}) {
   constexpr int R0 = P::radix(0);
   constexpr int B0 = P::bpt(R0);
   if constexpr (P::NST == 1) {
@@ -305,8 +318,9 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
         for (int r = 0; r < R0; ++r) lds[ct_lds<P, LD>(l, j * R0 + r)] = in[i * R0 + r];
       }
     }
+    hook(std::integral_constant<int, 0>{});
     lds_barrier();
-    ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid);
+    ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook);
     constexpr int S = P::NST - 1;
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
 #pragma unroll

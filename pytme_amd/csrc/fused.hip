@@ -323,16 +323,16 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
 // read).  PR: T0 <= N0 / R_IN, so only the first element of each first-stage
 // butterfly can be non-zero and the others are compile-time zeros.  The next
 // rotation's input columns are loaded before this rotation's transforms.
-// The X kernel (KIND 1) is held to 128 VGPRs for the 512-point case so two
-// blocks share a CU (a couple of spilled registers are cheaper than half the
-// occupancy).
+// The X kernel's 512-thread blocks (KIND 1) are held to 128 VGPRs for
+// lengths up to 512 so two blocks share a CU (a couple of spilled registers
+// are cheaper than half the occupancy).
 //
 // Split rows (TAIL): the output rows hold ldo bins (a multiple of L) and the
 // last ntl bins of every row go to the dense tail array Qt[o][b][ky][z][ntl];
 // the tail tiles are their own launch (kx_base = ldo) whose stores are
 // predicated to the valid columns, so the main launch stays unpredicated; a
 // tail tile packs L / ntl rows (ky) of ntl columns each.
-template <int N0, int L, int NT, int KIND, bool PR, bool TAIL = false, int V = 0>
+template <int N0, int L, int NT, int KIND, bool PR, bool TAIL = false, int V = 0, bool DEFER = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu((KIND == 1 && PR && N0 <= 512) ? 4 : 1, 8)))
 k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const float2* __restrict__ Ft2,
             float2* __restrict__ Q, float2* __restrict__ Qt, const float2* __restrict__ tw, int T0, int N1, int N2s,
@@ -340,6 +340,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   using PF = CtPlan<N0, L, NT, true, false, V>;
   using PI = CtPlan<N0, L, NT, true, true, V>;
   static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
+  static_assert(!DEFER || (!TAIL && PF::NST >= 2 && PI::NST == PF::NST), "deferred stores need staged transforms");
   static_assert((N0 / PF::R_IN * L) % NT == 0 && (N0 / PF::R_OUT * L) % NT == 0 &&
                     (N0 / PI::R_OUT * L) % NT == 0,
                 "every butterfly slot must be valid");
@@ -437,11 +438,35 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   // rotations of this block: all of them, or a share of them where the
   // launch splits the batch over gridDim.y (the few tail blocks)
   const int b0 = (int)((int64_t)blockIdx.y * B / gridDim.y), b1 = (int)((int64_t)(blockIdx.y + 1) * B / gridDim.y);
-  load_in(b0, x);
-  ct_ready<EO>(F);
-  if constexpr (KIND == 0) ct_ready<EO>(F2);
-  ct_ready<NX>(x);
-  for (int b = b0; b < b1; ++b) {
+
+  // Deferred stores (main launches, DEFER): the last output of rotation b
+  // (S2, or X) is held in registers and stored in pieces after the LDS
+  // stages of rotation b + 1's forward and first inverse transform, S1 after
+  // those of S2's inverse.  Stored in one burst after its own transform, the
+  // outputs of the CU's waves leave together (the block's waves meet at every
+  // stage barrier) and the next transforms wait behind them; spread out, the
+  // stores overlap the transforms (512x512x200, 64 rotations: S1/S2 kernel
+  // 7.95 -> 6.53 ms, X kernel 4.15 -> 4.05 ms in 1024-thread blocks).
+  constexpr int NE = PI::E_OUT, NH = PF::NST - 1;  // outputs per thread, hook points per transform
+  float2 held[DEFER ? NE : 1];
+  float2* held_dst = nullptr;
+  auto out_ptr = [&](int o, int b) { return Q + ((int64_t)(o * B + b) * N0) * (int64_t)plane_o + col_o; };
+  // piece K of N of a thread's outputs v
+  auto store_piece = [&](float2* dst, const float2* v, auto k, auto n) {
+    constexpr int K = decltype(k)::value, N = decltype(n)::value;
+#pragma unroll
+    for (int q = K * NE / N; q < (K + 1) * NE / N; ++q) {
+      int l, e;
+      PI::out_pos(tid, q / PI::R_OUT, q % PI::R_OUT, l, e);
+      dst[e * plane_o + l] = v[q];
+    }
+  };
+  using CN2 = std::integral_constant<int, 2 * NH>;
+  using CN1 = std::integral_constant<int, NH>;
+
+  // one rotation; FIRST: no held output to store yet
+  auto rotation = [&](int b, auto first) {
+    constexpr bool FIRST = decltype(first)::value;
     load_in(b + 1 < b1 ? b + 1 : b, xn);  // the last rotation reloads itself, keeping the loop unpredicated
     float2 w[EW];
 #pragma unroll
@@ -456,24 +481,69 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
         w[k] = v;
       }
     }
-    ct_fft<PF, -1, L>(lds, tws, w, w, tid);
-    if constexpr (KIND == 1) {
+    if constexpr (DEFER) {
+      // the held output: first half over the forward's stages, second half
+      // over the first inverse's
+      auto fwd_hook = [&](auto st) {
+        if constexpr (!FIRST) store_piece(held_dst, held, st, CN2{});
+      };
+      auto inv_hook = [&](auto st) {
+        if constexpr (!FIRST) store_piece(held_dst, held, std::integral_constant<int, NH + decltype(st)::value>{}, CN2{});
+      };
+      ct_fft<PF, -1, L>(lds, tws, w, w, tid, fwd_hook);
+      if constexpr (KIND == 1) {
 #pragma unroll
-      for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
-      inv_store(o_x, b, w);
-    } else {
-      float2 t[EO];
+        for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
+        ct_fft<PI, 1, L>(lds, tws, w, w, tid, inv_hook);
 #pragma unroll
-      for (int s = 0; s < EO; ++s) {
-        t[s] = c_mul(F2[s], w[s]);
-        w[s] = c_mul(F[s], w[s]);
+        for (int q = 0; q < NE; ++q) held[q] = w[q];
+        held_dst = out_ptr(o_x, b);
+      } else {
+        float2 t[EO];
+#pragma unroll
+        for (int s = 0; s < EO; ++s) {
+          t[s] = c_mul(F2[s], w[s]);
+          w[s] = c_mul(F[s], w[s]);
+        }
+        ct_fft<PI, 1, L>(lds, tws, w, w, tid, inv_hook);
+        float2* d1 = out_ptr(0, b);
+        ct_fft<PI, 1, L>(lds, tws, t, t, tid, [&](auto st) { store_piece(d1, w, st, CN1{}); });
+#pragma unroll
+        for (int q = 0; q < NE; ++q) held[q] = t[q];
+        held_dst = out_ptr(1, b);
       }
-      inv_store(0, b, w);
-      inv_store(1, b, t);
+    } else {
+      ct_fft<PF, -1, L>(lds, tws, w, w, tid);
+      if constexpr (KIND == 1) {
+#pragma unroll
+        for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
+        inv_store(o_x, b, w);
+      } else {
+        float2 t[EO];
+#pragma unroll
+        for (int s = 0; s < EO; ++s) {
+          t[s] = c_mul(F2[s], w[s]);
+          w[s] = c_mul(F[s], w[s]);
+        }
+        inv_store(0, b, w);
+        inv_store(1, b, t);
+      }
     }
 #pragma unroll
     for (int k = 0; k < NX; ++k) x[k] = xn[k];
     ct_ready<NX>(x);
+  };
+
+  load_in(b0, x);
+  ct_ready<EO>(F);
+  if constexpr (KIND == 0) ct_ready<EO>(F2);
+  ct_ready<NX>(x);
+  if constexpr (DEFER) {
+    if (b0 < b1) rotation(b0, std::true_type{});
+    for (int b = b0 + 1; b < b1; ++b) rotation(b, std::false_type{});
+    if (b0 < b1) store_piece(held_dst, held, std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
+  } else {
+    for (int b = b0; b < b1; ++b) rotation(b, std::true_type{});
   }
 }
 
@@ -708,9 +778,17 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
   // the tail blocks are fewer than the CUs: split their rotations over gridDim.y
   const unsigned tail_split = ntl > 0 ? (unsigned)std::min<int64_t>(B, std::max<int64_t>(1, tm_ceil_div(512, tail_blocks))) : 1;
+  // pruned main launches in 1024-thread blocks store deferred (8 outputs
+  // per thread; with 16, in the smaller blocks of the 128- and 256-point
+  // kernels or unpruned, the held outputs push the kernels past 128 VGPRs or
+  // into spills); tail launches store directly, the X tail in blocks of at
+  // most 512 threads
+  constexpr bool DS = PR && NT0 >= 1024 && CtPlan<N0, L, NT0, true, false, V>::NST >= 2;
+  constexpr bool DX = PR && NT >= 1024 && CtPlan<N0, L, NT, true, false, V>::NST >= 2;
+  constexpr int NTT = NT > 512 ? 512 : NT;
   if (flc) {
     TM_TIMED("fused_pass_a", bytes_of(nm, 2, 2), s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, false, V>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, false, V, DS>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
                                 Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, 0, 0));
     TM_CHECK_LAUNCH();
     if (ntl > 0) {
@@ -721,13 +799,13 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
     }
   }
   TM_TIMED(flc ? "fused_pass_a_x" : "fused_pass_a", bytes_of(nm, 1, 1), s,
-           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, false, V>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
+           hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, false, V, DX>), dim3((unsigned)blocks), dim3(NT), shm, s, P, ft,
                               (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, flc ? 1 : 0,
                               flc ? 2 : 0));
   TM_CHECK_LAUNCH();
   if (ntl > 0) {
     TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", bytes_of(ntl, 1, 1), s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT, 1, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT), shm, s, P, ft,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NTT, 1, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NTT), shm, s, P, ft,
                                 (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
                                 flc ? 2 : 0));
     TM_CHECK_LAUNCH();
@@ -1451,7 +1529,7 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     switch (N0) {
       case 128: return l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128);
       case 256: return l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256);
-      case 512: return l16 ? TM_PASS_A(512, 16, 512, 1024) : TM_PASS_A(512, 8, 512);
+      case 512: return l16 ? TM_PASS_A(512, 16, 1024, 1024) : TM_PASS_A(512, 8, 512);
       case 1024: return TM_PASS_A(1024, 8, 512);
 #undef TM_PASS_A
       default: done_a = false; return TM_OK;
