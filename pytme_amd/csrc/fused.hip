@@ -218,16 +218,15 @@ __device__ __forceinline__ float fu_flc_norm(float x, float ex2, float ex, float
 }
 
 // The factor r with fu_flc_norm(x * scale, ex2 * scale, ex * scale, n, eps)
-// ~= x * r, from the unscaled inverse-transform outputs and sn = scale / n.
-// The hardware square root and reciprocal (<= 1 ulp each) and the folded
-// scale replace the correctly rounded divisions: a few ulp in the score.
-__device__ __forceinline__ float fu_flc_factor(float ex2, float ex, float sn, float n, float scale, float eps) {
+// ~= x * r, from the unscaled inverse-transform outputs and sn = scale / n:
+// r = sn / sd, or sn where sd = sqrt(v) < eps (v < eps^2).  The hardware
+// reciprocal square root (one quarter-rate instruction instead of a square
+// root and a reciprocal) and the folded scale replace the correctly rounded
+// divisions: a few ulp in the score.
+__device__ __forceinline__ float fu_flc_factor(float ex2, float ex, float sn, float eps2) {
   const float m = ex * sn;
-  float v = ex2 * sn - m * m;
-  v = fmaxf(v, 0.f);
-  float sd = __builtin_amdgcn_sqrtf(v);
-  sd = sd < eps ? 1.f : sd;
-  return scale * __builtin_amdgcn_rcpf(sd * n);
+  const float v = fmaxf(ex2 * sn - m * m, 0.f);  // fmaxf also maps NaN to 0
+  return v < eps2 ? sn : sn * __builtin_amdgcn_rsqf(v);
 }
 
 template <int NQ>
@@ -720,8 +719,8 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     } else {
 #pragma unroll
       for (int s = 0; s < EO; ++s) {
-        d0[s] = fu_flc_factor(z[s].x, d0[s], sn, nf, scale, eps);
-        d1[s] = fu_flc_factor(z[s].y, d1[s], sn, nf, scale, eps);
+        d0[s] = fu_flc_factor(z[s].x, d0[s], sn, eps * eps);
+        d1[s] = fu_flc_factor(z[s].y, d1[s], sn, eps * eps);
       }
     }
   };
