@@ -1529,7 +1529,7 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
       case 128: return l16 ? TM_PASS_A(128, 16, 256) : TM_PASS_A(128, 8, 128);
       case 256: return l16 ? TM_PASS_A(256, 16, 512) : TM_PASS_A(256, 8, 256);
       case 512: return l16 ? TM_PASS_A(512, 16, 1024, 1024) : TM_PASS_A(512, 8, 512);
-      case 1024: return TM_PASS_A(1024, 8, 512);
+      case 1024: return TM_PASS_A(1024, 8, 1024, 512);
 #undef TM_PASS_A
       default: done_a = false; return TM_OK;
     }
