@@ -63,17 +63,36 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
 // `inner` are padding that may be transformed), so no load or store is
 // predicated; the compiler can then wait for the prefetched tile without
 // draining the previous tile's stores.
-template <int N, int L, int NT, int DIR, bool FULL, int V = 0>
+//
+// PH > 0 (inverse transforms of mirror-pair rows, tm_pair_bin with h = PH
+// and tiles of L = 2 PH positions): before the store each output is replaced
+// by the C2R input tm_c2r_pre of its bin k from the pair (X[k], X[M-k]),
+// whose partner sits in lane l ^ PH; twx holds the 2M twiddles of the row
+// transform.  The row pass then runs its M-point inverse directly.
+template <int PH>
+__device__ __forceinline__ float tm_lane_xor(float v) {
+  if constexpr (PH == 8)  // row_ror:8 within 16 lanes is lane ^ 8
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, false));
+  else  // ds_swizzle bit mode: and 0x1f, or 0, xor PH
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (PH << 10)));
+}
+
+template <int N, int L, int NT, int DIR, bool FULL, int V = 0, int PH = 0>
 __global__ void __launch_bounds__(NT)
 k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner, int ld,
-         int64_t tiles_per_outer, int64_t ntiles, float scale) {
+         int64_t tiles_per_outer, int64_t ntiles, float scale, const float2* __restrict__ twx, int M) {
   using PL = CtPlan<N, L, NT, true, false, V>;
   constexpr int EI = PL::E_IN, EO = PL::E_OUT, EW = EI > EO ? EI : EO;
   constexpr bool WIDE = L * sizeof(float2) >= 128;  // whole-line row segments: streaming hints
+  static_assert(PH == 0 || (L == 2 * PH && NT % L == 0 && FULL && (PH == 8 || PH == 4)),
+                "mirror pairs need whole tiles of 2 PH lines and unpredicated tiles");
   extern __shared__ float2 lds[];
   __shared__ float2 tws[N];  // twiddles in LDS: no vector-memory waits inside the transforms
+  __shared__ float2 twxs[PH > 0 ? 256 : 1];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += NT) tws[i] = tw[i];
+  if constexpr (PH > 0)
+    for (int i = tid; i < 2 * M; i += NT) twxs[i] = twx[i];
   // persistent blocks: tile t = blockIdx.x + k * gridDim.x; the next tile's
   // loads are issued before the current tile's transform
   float2 v[EW], nx[EI];
@@ -110,6 +129,18 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
     const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
     const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
     float2* dst = out + o * (int64_t)N * ld + i0;
+    if constexpr (PH > 0) {
+      // every output slot of this thread is on line tid % L of tile jt
+      const int jt = (int)(i0 / L), lt = tid % L;
+      const int k = lt < PH ? PH * jt + lt : M - PH * jt - (lt - PH);
+      const float2 w = tw_load<1>(twxs, k);
+#pragma unroll
+      for (int s2 = 0; s2 < EO; ++s2) {
+        const float2 a = v[s2];
+        const float2 b = make_float2(tm_lane_xor<PH>(a.x), tm_lane_xor<PH>(a.y));
+        v[s2] = tm_c2r_pre(a, b, w, k == 0);
+      }
+    }
 #pragma unroll
     for (int i = 0; i < EO / PL::R_OUT; ++i)
 #pragma unroll
@@ -139,9 +170,10 @@ static int device_cus() {
   return cus[dev];
 }
 
-template <int N, int L, int NT, int V = 0>
+template <int N, int L, int NT, int V = 0, int PH = 0>
 static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
-                         int64_t ld, int dir, float scale, hipStream_t s, const char* name) {
+                         int64_t ld, int dir, float scale, hipStream_t s, const char* name,
+                         const float2* twx = nullptr, int M = 0) {
   const int64_t tpo = tm_ceil_div(inner, L);
   const int64_t ntiles = outer * tpo;
   const size_t shm = (size_t)N * L * sizeof(float2);
@@ -150,7 +182,7 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   static int per_cu = 0;
   if (per_cu == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_ct<N, L, NT, 1, true, V>, NT, shm) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_ct<N, L, NT, 1, true, V, PH>, NT, shm) !=
             hipSuccess ||
         nb < 1)
       nb = 1;
@@ -165,7 +197,16 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
 #define TM_C2C_LAUNCH(D, F)                                                                                      \
   TM_TIMED(name, bytes, s,                                                                                      \
            hipLaunchKernelGGL((k_c2c_ct<N, L, NT, D, F, V>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw, \
-                              lin, inner, (int)ld, tpo, ntiles, scale))
+                              lin, inner, (int)ld, tpo, ntiles, scale, (const float2*)nullptr, 0))
+  if constexpr (PH > 0) {
+    TM_REQUIRE(full && dir > 0 && in == out && twx && 2 * M <= 256 && inner % L == 0,
+               "mirror-pair C2C: in-place inverse over whole unpredicated tiles, rows of at most 129 bins");
+    TM_TIMED(name, bytes, s,
+             hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1, true, V, PH>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out,
+                                lp.tw, lin, inner, (int)ld, tpo, ntiles, scale, twx, M));
+    TM_CHECK_LAUNCH();
+    return TM_OK;
+  }
   if (dir < 0) {
     if (full) TM_C2C_LAUNCH(-1, true);
     else TM_C2C_LAUNCH(-1, false);
@@ -179,10 +220,12 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
 }
 
 // R2C along contiguous rows: in rows of lin <= n reals (stride lin,
-// zero-padded to n), out rows of n/2+1 complex (stride n/2+1).
+// zero-padded to n), out rows of n/2+1 complex (stride ldo).  pair_h > 0:
+// the bins are stored in the mirror-pair order of tm_pair_bin (pair_nm
+// paired positions, pairs in lanes l and l ^ pair_h of a column tile).
 __global__ void __launch_bounds__(TM_FFT_THREADS)
 k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, int lin, int64_t nrows, int RL,
-           int64_t ldo) {
+           int64_t ldo, int pair_nm, int pair_h) {
   extern __shared__ float2 lds[];
   const int n = p.n, nc = n / 2 + 1;
   const int ld = n + 1;
@@ -201,7 +244,7 @@ k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, i
   float2* dst = out + r0 * ldo;
   for (int f = tid; f < nc * rl; f += TM_FFT_THREADS) {
     const int l = f / nc, e = f - l * nc;
-    dst[l * ldo + e] = R[l * ld + e];
+    dst[l * ldo + (pair_h ? tm_pair_pos(e, pair_nm, nc - 1, pair_h) : e)] = R[l * ld + e];
   }
 }
 
@@ -332,6 +375,31 @@ int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, in
   return TM_OK;
 }
 
+int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, float2* buf, int64_t outer,
+                        int64_t inner, int64_t ld, hipStream_t s, const char* name, int variant) {
+  TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
+  TM_REQUIRE(inner >= 1 && inner <= ld && (int64_t)lp.n * ld < (1LL << 31), "mirror-pair C2C: %lld lines at stride %lld",
+             (long long)inner, (long long)ld);
+  auto go = [&](auto vtag) -> int {
+    constexpr int V = decltype(vtag)::value;
+    if (ph == 8 && lp.n == 128)
+      return launch_c2c_ct<128, 16, 256, V, 8>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    if (ph == 8 && lp.n == 256)
+      return launch_c2c_ct<256, 16, 256, V, 8>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    if (ph == 8 && lp.n == 512)
+      return launch_c2c_ct<512, 16, 512, V, 8>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    if (ph == 4 && lp.n == 1024)
+      return launch_c2c_ct<1024, 8, 512, V, 4>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    tm_set_error("mirror-pair C2C: no kernel for length %d with pairs at lane distance %d", lp.n, ph);
+    return TM_ERR_VALUE;
+  };
+  return variant ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, 0>{});
+}
+
+// Lane distance of the mirror pairs for an in-place inverse C2C of length
+// n whose compile-time kernel takes tiles of 2h lines (0: none).
+int tm_c2c_pair_h(int n) { return (n == 128 || n == 256 || n == 512) ? 8 : n == 1024 ? 4 : 0; }
+
 static int rows_per_block(int n, int64_t nrows) {
   int r = 2048 / (n + 1);
   if (r < 1) r = 1;
@@ -341,15 +409,17 @@ static int rows_per_block(int n, int64_t nrows) {
 }
 
 int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
-                       hipStream_t s, const char* name) {
+                       hipStream_t s, const char* name, int pair_nm, int pair_h) {
   TM_REQUIRE(ldo >= lp.n / 2 + 1, "R2C: output row stride %lld below %d", (long long)ldo, lp.n / 2 + 1);
+  TM_REQUIRE(pair_h == 0 || (pair_h > 0 && pair_nm % (2 * pair_h) == 0 && pair_nm <= lp.n / 2 + 1),
+             "R2C: mirror-pair order needs whole tiles of 2 * %d positions", pair_h);
   const int RL = rows_per_block(lp.n, nrows);
   const size_t shm = 2 * (size_t)RL * (lp.n + 1) * sizeof(float2);
   const int64_t blocks = tm_ceil_div(nrows, RL);
   TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
   TM_TIMED(name, (4.0 * lin + 8.0 * (lp.n / 2 + 1)) * (double)nrows, s,
            hipLaunchKernelGGL(k_r2c_rows, dim3((unsigned)blocks), dim3(TM_FFT_THREADS), shm, s, in, out, lp, lin,
-                              nrows, RL, ldo));
+                              nrows, RL, ldo, pair_nm, pair_h));
   TM_CHECK_LAUNCH();
   return TM_OK;
 }

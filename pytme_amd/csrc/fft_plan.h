@@ -27,10 +27,21 @@ struct tm_fft_plan {
 // ct_factor_v); the generic path ignores it.
 int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, int64_t outer, int64_t inner,
                   int64_t ld, int dir, float scale, hipStream_t s, const char* name, int variant = 0);
+// In-place inverse C2C along the middle axis of [outer][n][ld] (first
+// `inner` lines, whole tiles of 2 ph lines) whose lines are mirror-pair
+// rows (tm_pair_bin with h = ph) of a half spectrum of M + 1 bins: each
+// output is stored as the C2R input tm_c2r_pre of its bin (twx: the 2M
+// twiddles of the row transform).  Lengths 128, 256, 512 (ph 8), 1024 (ph 4).
+int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, float2* buf, int64_t outer,
+                        int64_t inner, int64_t ld, hipStream_t s, const char* name, int variant = 0);
+// The ph of tm_launch_c2c_pairs for length n, 0 if it has no kernel.
+int tm_c2c_pair_h(int n);
 // R2C of nrows contiguous rows of lin <= n reals (zero-padded to n) into
-// rows of n/2+1 bins at a row stride of ldo >= n/2+1 complex.
+// rows of n/2+1 bins at a row stride of ldo >= n/2+1 complex; pair_h > 0
+// stores the bins in the mirror-pair order (tm_pair_bin) with pair_nm
+// paired positions.
 int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
-                       hipStream_t s, const char* name);
+                       hipStream_t s, const char* name, int pair_nm = 0, int pair_h = 0);
 // C2R of nrows contiguous half-spectrum rows, times scale.
 int tm_launch_c2r_rows(const LinePlan& lp, const float2* in, float* out, int64_t nrows, float scale, hipStream_t s,
                        const char* name);

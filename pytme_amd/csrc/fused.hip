@@ -563,18 +563,25 @@ constexpr int pass_c_ld(int M) { return M + 1; }
 // re-reads the last tile), so waiting for the oldest one never waits for the
 // younger ones.
 //
-// SPLIT: rows hold ldq main bins in Q and the remaining N2c - ldq bins in
-// the tail array Qt[o][b][y][z][ntl] (row r = z * N1 + y).
+// SPLIT: rows hold NM = 16 * floor(N2c / 16) main positions in Q (row
+// stride ldq) and the remaining NTL bins in the tail array
+// Qt[o][b][y][z][NTL] (row r = z * N1 + y), in the mirror-pair order of
+// tm_pair_bin (h = ph): the main positions already hold the transform
+// inputs Z (pass B computed them from each pair), the tail bins are the
+// middle bins NM/2 .. M - NM/2, paired here.  Only the load slots that
+// meet the tail (the last one or two) carry the pairing; Z is written to
+// LDS at its bin and the transform reads it directly.
 template <int M, int RL, int NT, int NQ, int PD, bool SPLIT = false, int V = 0>
 __global__ void __launch_bounds__(NT)
 k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const float2* __restrict__ tw2,
-            int64_t nrows, int ldq, int N0, int N1, int ntl, int B,
+            int64_t nrows, int ldq, int N0, int N1, int ntl, int ph, int B,
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
             float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
   using PI = CtPlan<M, RL, NT, false, true, V>;
   constexpr int N2 = 2 * M, N2c = M + 1, LD = pass_c_ld(M);
   constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
   constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread and tile
+  constexpr int NM = N2c / 16 * 16, NTL = N2c - NM, NP = NM / 2;  // split rows
   static_assert(PD >= 1 && PD <= 3, "prefetch depth 1 to 3");
   // the tile loop is unrolled by U so that the spectrum index o and the
   // prefetch buffer of every step are compile-time constants
@@ -615,47 +622,78 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     arg0[s] = arg1[s] = -1;
     d0[s] = d1[s] = 0.f;
   }
-  // Each load slot's offset within a slab (o, b) is the same for every
-  // tile: computed once (rows consecutive at stride ldq from row r0; split
-  // rows: >= 0 the main offset from the slab start, < 0 the tail offset -1-t)
-  int offs[NPF];
-  {
+  // Each load slot's offset within a slab (o, b) and its LDS position are
+  // the same for every tile: computed once.  Rows are consecutive at stride
+  // ldq from row r0; split rows: offset >= 0 the main offset from the slab
+  // start, < 0 the tail offset -1-t.  Unpredicated: slots past the tile
+  // (or past the block's rl rows) re-read a valid bin and write the junk
+  // line RL.
+  int offs[NPF], lpos[NPF];
+  int offs2[NPF];      // split rows, slots meeting the tail: the pair partner's offset
+  bool tl[NPF];        // this lane's element of the slot is a tail bin
+  float2 wtw[NPF];     // and W^k of its bin
+  // slots p whose f range [p NT, (p + 1) NT) meets the tail [RL NM, RL N2c)
+  auto tail_slot = [](int p) { return SPLIT && (p + 1) * NT > RL * NM && p * NT < RL * N2c; };
 #pragma unroll
-    for (int p = 0; p < NPF; ++p) {
-      int f = tid + p * NT;
-      f = f < tile_n ? f : tile_n - 1;  // unpredicated: out-of-tile slots re-read a valid bin
-      const int l = f / N2c, e = f - l * N2c;
-      if constexpr (SPLIT) {
-        // row (z, y) = (zg0 + l, yg); its tail bins sit at Qt[y][z][e - nm]
-        const int nm = N2c - ntl;  // main bins; ldq is the row stride (>= nm)
-        offs[p] = e < nm ? (int)row_of(l) * ldq + e : -1 - ((yg * N0 + zg0 + l) * ntl + (e - nm));
+  for (int p = 0; p < NPF; ++p) {
+    const int f = tid + p * NT;
+    tl[p] = false;
+    if constexpr (SPLIT) {
+      if (f < RL * NM) {
+        const int l = f / NM, q = f - l * NM;
+        const int lc = l < rl ? l : rl - 1;
+        offs[p] = (int)row_of(lc) * ldq + q;
+        lpos[p] = l < rl ? l * LD + tm_pair_bin(q, NM, M, ph) : RL * LD;
+        offs2[p] = offs[p];
+      } else if (f < RL * N2c) {
+        // row (z, y) = (zg0 + l, yg): its tail bins sit at Qt[y][z][c]
+        const int t = f - RL * NM, l = t / NTL, c = t - l * NTL;
+        const int lc = l < rl ? l : rl - 1;
+        const int base = (yg * N0 + zg0 + lc) * NTL;
+        offs[p] = -1 - (base + c);
+        offs2[p] = -1 - (base + NTL - 1 - c);
+        lpos[p] = l < rl ? l * LD + NP + c : RL * LD;
+        tl[p] = true;
+        wtw[p] = tw_load<1>(tw2, NP + c);
       } else {
-        offs[p] = l * ldq + e;
+        offs[p] = offs2[p] = (int)row_of(0) * ldq;
+        lpos[p] = RL * LD;
       }
+      if (!tail_slot(p)) wtw[p] = make_float2(0.f, 0.f);
+    } else {
+      const int fc = f < tile_n ? f : tile_n - 1;
+      const int l = fc / N2c, e = fc - l * N2c;
+      offs[p] = l * ldq + e;
+      lpos[p] = f < tile_n ? l * LD + e : RL * LD;
     }
   }
-  auto prefetch = [&](int it, float2* pf) {
+  auto prefetch = [&](int it, float2* pf, float2* pf2) {
     const int b = it / NQ, o = it - (it / NQ) * NQ;
     const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;  // r0 = 0 with split rows
-    const float2* tsrc = SPLIT ? Qt + (int64_t)(o * B + b) * nrows * ntl : nullptr;
+    const float2* tsrc = SPLIT ? Qt + (int64_t)(o * B + b) * nrows * NTL : nullptr;
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
-      if constexpr (SPLIT)
+      if constexpr (SPLIT) {
         pf[p] = tm_ld_stream(offs[p] >= 0 ? src + offs[p] : tsrc + (-1 - offs[p]));
-      else
+        if (tail_slot(p)) pf2[p] = tm_ld_stream(offs2[p] >= 0 ? src + offs2[p] : tsrc + (-1 - offs2[p]));
+      } else {
         pf[p] = tm_ld_stream(src + offs[p]);
+      }
     }
   };
-  auto step = [&](int it, int o, float2* pf) {
+  auto step = [&](int it, int o, float2* pf, float2* pf2) {
     const int b = it / NQ;
     lds_barrier();  // the previous transform has finished reading the tile
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
-      const int f = tid + p * NT;
-      const int l = f / N2c, e = f - l * N2c;
-      lds[f < tile_n ? l * LD + e : RL * LD] = pf[p];  // out-of-tile slots write the junk line
+      float2 x = pf[p];
+      if (tail_slot(p)) {
+        const float2 zt = tm_c2r_pre(x, pf2[p], wtw[p], false);
+        x = tl[p] ? zt : x;
+      }
+      lds[lpos[p]] = x;
     }
-    prefetch(it + PD < iters ? it + PD : iters - 1, pf);
+    prefetch(it + PD < iters ? it + PD : iters - 1, pf, pf2);
     lds_barrier();
     float2 z[EW];
 #pragma unroll
@@ -664,13 +702,11 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
       for (int r = 0; r < PI::R_IN; ++r) {
         int l, k;
         if (!PI::in_pos(tid, i, r, l, k)) l = RL;  // idle slot: junk line (never stored)
-        float2 A = lds[l * LD + k];
-        float2 C = lds[l * LD + (M - k)];
-        A.y = k == 0 ? 0.f : A.y;
-        C.y = k == 0 ? 0.f : C.y;
-        const float2 E = make_float2(A.x + C.x, A.y - C.y);
-        const float2 D = c_mul(make_float2(A.x - C.x, A.y + C.y), tw_load<1>(tws, k));
-        z[i * PI::R_IN + r] = make_float2(E.x - D.y, E.y + D.x);
+        if constexpr (SPLIT) {
+          z[i * PI::R_IN + r] = lds[l * LD + k];
+        } else {
+          z[i * PI::R_IN + r] = tm_c2r_pre(lds[l * LD + k], lds[l * LD + (M - k)], tw_load<1>(tws, k), k == 0);
+        }
       }
     ct_fft<PI, 1, LD, 2>(lds, tws, z, z, tid);
     const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
@@ -725,12 +761,12 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     }
   };
 
-  float2 pf[PD][NPF];
+  float2 pf[PD][NPF], pf2[PD][NPF];
 #pragma unroll
-  for (int q = 0; q < PD; ++q) prefetch(q < iters ? q : iters - 1, pf[q]);
+  for (int q = 0; q < PD; ++q) prefetch(q < iters ? q : iters - 1, pf[q], pf2[q]);
   for (int it0 = 0; it0 < iters; it0 += U) {  // iters is a multiple of U (checked by the launcher)
 #pragma unroll
-    for (int u = 0; u < U; ++u) step(it0 + u, u % NQ, pf[u % PD]);
+    for (int u = 0; u < U; ++u) step(it0 + u, u % NQ, pf[u % PD], pf2[u % PD]);
   }
 #pragma unroll
   for (int i = 0; i < EO / PI::R_OUT; ++i)
@@ -833,11 +869,12 @@ static int launch_pass_a_ct(bool flc, const float2* P, const float2* ft, const f
 
 template <int V, int M, int RL, int NT, int PD3, int PD1, bool SPLIT = false>
 static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const float2* tw2, int64_t nrows, int ldq,
-                            int N0, int N1, int ntl, int B, const double* n_obs, const float* inv_den,
+                            int N0, int N1, int ntl, int ph, int B, const double* n_obs, const float* inv_den,
                             const int* rot_idx, float scale, float eps, float* max_scores, int* max_rot, double bytes,
                             hipStream_t s) {
-  TM_REQUIRE(!SPLIT || (ntl >= 1 && ldq >= M + 1 - ntl && Qt && nrows == (int64_t)N0 * N1),
-             "pass C: split rows need the tail array and nrows = N0 * N1");
+  TM_REQUIRE(!SPLIT || (ntl == M + 1 - (M + 1) / 16 * 16 && ldq >= M + 1 - ntl && Qt && nrows == (int64_t)N0 * N1 &&
+                        (ph == 8 || ph == 4)),
+             "pass C: split rows need the tail array of the last N2c %% 16 bins, mirror pairs and nrows = N0 * N1");
   TM_REQUIRE(!SPLIT || nrows * ldq < (1LL << 31), "pass C: main offsets exceed 32 bits");
   // PD3 / PD1: prefetch depth of the FLC (3 spectra) / correlation kernels;
   // the step count B * NQ must be a multiple of the unroll lcm(NQ, PD)
@@ -852,17 +889,17 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const f
     static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
     TM_TIMED("fused_pass_c", bytes, s,
              hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
-                                Qt, tw2, nrows, ldq, N0, N1, ntl, B, n_obs, (const float*)nullptr, rot_idx, scale, eps,
+                                Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, n_obs, (const float*)nullptr, rot_idx, scale, eps,
                                 max_scores, max_rot));
   } else if (B % PD1 == 0) {
     TM_TIMED("fused_pass_c", bytes, s,
              hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
-                                Qt, tw2, nrows, ldq, N0, N1, ntl, B, (const double*)nullptr, inv_den, rot_idx, scale,
+                                Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, (const double*)nullptr, inv_den, rot_idx, scale,
                                 eps, max_scores, max_rot));
   } else {
     TM_TIMED("fused_pass_c", bytes, s,
              hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
-                                Qt, tw2, nrows, ldq, N0, N1, ntl, B, (const double*)nullptr, inv_den, rot_idx, scale,
+                                Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, (const double*)nullptr, inv_den, rot_idx, scale,
                                 eps, max_scores, max_rot));
   }
   TM_CHECK_LAUNCH();
@@ -877,6 +914,7 @@ struct tm_scan_plan {
   int nq_in, nq_out;
   int N[3], N2c, N2s;  // N2s: row stride of the workspace spectra (N2c rounded up to 16 bins = 128 B)
   int N2m, ntl;        // correlation spectra rows: N2m bins in Q, ntl tail bins in Qt (split rows) or N2s, 0
+  int pair_h;          // split rows: x bins in mirror-pair order (tm_pair_bin, h = pair_h) in every spectrum of the scan
   int T[3], T2c;
   int max_batch;
   int normalize;
@@ -957,8 +995,12 @@ static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
 // Row layout of the correlation spectra.  Split rows where the compile-time
 // passes run (x length 200: 101 bins, 96 per row in Q and the last 5 in the
 // dense tail array instead of 112-bin rows; x length 256: 129 bins, 128 + 1
-// instead of 144; z length 128, 256 or 512, or 1024 for x length 256), so no
-// pass moves the padding bins; otherwise rows padded to N2s bins.
+// instead of 144; z length 128, 256 or 512, or 1024 for x length 256; y
+// length 128, 256, 512 or 1024), so no pass moves the padding bins;
+// otherwise rows padded to N2s bins.  Split rows keep the x bins of every
+// spectrum of the scan in the mirror-pair order of tm_pair_bin: pass B's
+// y transform then ends with the pre-processing of pass C's half-length x
+// inverse (tm_c2r_pre), which needs the pair (k, M - k) of a row.
 #ifndef TM_SPLIT_ROWS
 #define TM_SPLIT_ROWS 1
 #endif
@@ -968,13 +1010,25 @@ static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
 static inline void row_layout(int64_t N0, int64_t N1, int64_t N2, int64_t& N2m, int64_t& ntl) {
   const int64_t N2c = N2 / 2 + 1;
   const bool z_ct = N0 == 128 || N0 == 256 || N0 == 512;
-  const bool split = TM_SPLIT_ROWS && N1 >= 16 && ((N2 == 200 && z_ct) || (N2 == 256 && (z_ct || N0 == 1024)));
+  const bool y_ct = N1 == 128 || N1 == 256 || N1 == 512 || N1 == 1024;
+  const bool split = TM_SPLIT_ROWS && y_ct && ((N2 == 200 && z_ct) || (N2 == 256 && (z_ct || N0 == 1024)));
   const int64_t nm = N2c / 16 * 16;
   N2m = split ? (nm % 64 == 0 ? nm + 16 : nm) : pad_bins(N2c);
   ntl = split ? N2c - nm : 0;
 }
 
-// Copy the caller's target spectra (rows of N2c bins) into the padded copies.
+// Rows of n2c natural-order bins into rows of n2s positions in mirror-pair
+// order (padding positions zero); one block per 8 rows.
+__global__ void __launch_bounds__(256)
+k_pair_rows(const float2* __restrict__ src, float2* __restrict__ dst, int64_t rows, int n2c, int n2s, int nm, int h) {
+  const int64_t r = (int64_t)blockIdx.x * 8 + threadIdx.x / 32;
+  if (r >= rows) return;
+  for (int q = threadIdx.x % 32; q < n2s; q += 32)
+    dst[r * n2s + q] = q < n2c ? src[r * n2c + tm_pair_bin(q, nm, n2c - 1, h)] : make_float2(0.f, 0.f);
+}
+
+// Copy the caller's target spectra (rows of N2c bins) into the padded copies
+// (in mirror-pair order with split rows).
 static int scan_copy_target(tm_scan_plan* p, const void* ft, const void* ft2, hipStream_t s) {
   if (p->N2s == p->N2c) {
     p->ft = (const float2*)ft;
@@ -982,6 +1036,20 @@ static int scan_copy_target(tm_scan_plan* p, const void* ft, const void* ft2, hi
     return TM_OK;
   }
   const size_t rows = (size_t)p->N[0] * p->N[1];
+  if (p->pair_h) {
+    const int nm = p->N2c - p->ntl;
+    hipLaunchKernelGGL(k_pair_rows, dim3((unsigned)tm_ceil_div((int64_t)rows, 8)), dim3(256), 0, s,
+                       (const float2*)ft, p->ftp, (int64_t)rows, p->N2c, p->N2s, nm, p->pair_h);
+    TM_CHECK_LAUNCH();
+    if (p->mode == 0) {
+      hipLaunchKernelGGL(k_pair_rows, dim3((unsigned)tm_ceil_div((int64_t)rows, 8)), dim3(256), 0, s,
+                         (const float2*)ft2, p->ft2p, (int64_t)rows, p->N2c, p->N2s, nm, p->pair_h);
+      TM_CHECK_LAUNCH();
+    }
+    p->ft = p->ftp;
+    p->ft2 = p->mode == 0 ? p->ft2p : nullptr;
+    return TM_OK;
+  }
   const size_t spitch = sizeof(float2) * p->N2c, dpitch = sizeof(float2) * p->N2s;
   TM_CHECK_HIP(hipMemcpy2DAsync(p->ftp, dpitch, ft, spitch, spitch, rows, hipMemcpyDeviceToDevice, s));
   if (p->mode == 0)
@@ -1042,6 +1110,12 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
     row_layout(p->N[0], p->N[1], p->N[2], n2m, ntl);
     p->N2m = (int)n2m;
     p->ntl = (int)ntl;
+    p->pair_h = ntl ? tm_c2c_pair_h(p->N[1]) : 0;
+    if (ntl && !p->pair_h) {
+      tm_set_error("split rows without a mirror-pair kernel for y length %d", p->N[1]);
+      free(p);
+      return TM_ERR_VALUE;
+    }
   }
   p->T2c = p->T[2] / 2 + 1;
   p->max_batch = max_batch;
@@ -1505,12 +1579,13 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   const LinePlan& ly = p->fast->lines[1].lp;
   const LinePlan& lz = p->fast->lines[0].lp;
   const int64_t q1_stride = (int64_t)B * T0 * T1 * N2s;
+  const int pair_nm = p->pair_h ? N2c - p->ntl : 0;
   if (flc) {
-    rc = tm_launch_r2c_rows(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, N2s, s, "fused_x_pass");
+    rc = tm_launch_r2c_rows(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, N2s, s, "fused_x_pass", pair_nm, p->pair_h);
     if (rc != TM_OK) return rc;
   }
   rc = tm_launch_r2c_rows(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, N2s, s,
-                          "fused_x_pass");
+                          "fused_x_pass", pair_nm, p->pair_h);
   if (rc != TM_OK) return rc;
   rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, N2s, -1, 1.0f, s, "fused_y_pass",
                      p->radix);
@@ -1561,10 +1636,11 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
 
   // 6. pass B: inverse y in place
   if (p->ntl) {
-    // split rows: the main rows (whole 16-bin tiles), then the tail array
-    // (lines (z, c) of N0 * ntl, whole 128-byte segments)
-    rc = tm_launch_c2c(ly, p->Q, N1, p->Q, (int64_t)p->nq_out * B * N0, N2c - p->ntl, p->N2m, 1, 1.0f, s,
-                       "fused_pass_b", p->radix);
+    // split rows: the main rows (whole 16-bin tiles of mirror pairs, stored
+    // as pass C's x-inverse inputs), then the tail array (lines (z, c) of
+    // N0 * ntl, whole 128-byte segments; pass C pairs the tail bins itself)
+    rc = tm_launch_c2c_pairs(ly, lx.tw, N2 / 2, p->pair_h, p->Q, (int64_t)p->nq_out * B * N0, N2c - p->ntl, p->N2m,
+                             s, "fused_pass_b", p->radix);
     if (rc != TM_OK) return rc;
     rc = tm_launch_c2c(ly, p->Qt, N1, p->Qt, (int64_t)p->nq_out * B, (int64_t)N0 * p->ntl, (int64_t)N0 * p->ntl, 1,
                        1.0f, s, "fused_pass_b_tail", p->radix);
@@ -1586,23 +1662,23 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
     auto pass_c = [&](auto vtag) -> int {
     constexpr int V = decltype(vtag)::value;
     switch (N2) {
-      case 128: return launch_pass_c_ct<V, 64, 32, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+      case 128: return launch_pass_c_ct<V, 64, 32, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
       case 200:
         if (p->ntl) {  // split rows: main bins in Q (row stride N2m), the tail bins in Qt
           if (p->c_rows == 10)
-            return launch_pass_c_ct<V, 100, 10, 256, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+            return launch_pass_c_ct<V, 100, 10, 256, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, p->pair_h, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
           else
-            return launch_pass_c_ct<V, 100, 5, 128, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+            return launch_pass_c_ct<V, 100, 5, 128, 3, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, p->pair_h, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         } else if (p->c_rows == 10) {
-          return launch_pass_c_ct<V, 100, 10, 256, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 100, 10, 256, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         } else {
-          return launch_pass_c_ct<V, 100, 5, 128, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 100, 5, 128, 3, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         }
       case 256:
         if (p->ntl)  // split rows: 128 bins in Q, the Nyquist bin in Qt
-          return launch_pass_c_ct<V, 128, 16, 256, 1, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 128, 16, 256, 1, 2, true>(flc, p->Q, p->Qt, lx.tw, nrows, p->N2m, N0, N1, p->ntl, p->pair_h, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
         else
-          return launch_pass_c_ct<V, 128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
+          return launch_pass_c_ct<V, 128, 16, 256, 1, 2>(flc, p->Q, nullptr, lx.tw, nrows, N2s, N0, N1, 0, 0, B, p->n_obs, p->inv_den, rot_index_dev, scale, eps, max_scores, max_rot, bytes, s);
       default: done_c = false;
     }
     return TM_OK;

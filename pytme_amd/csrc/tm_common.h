@@ -54,6 +54,40 @@ __device__ __forceinline__ float2 c_scale(float2 a, float s) {
 }
 __device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
 
+// ---- mirror-pair order of half-spectrum rows -------------------------------
+// A half-spectrum row of M + 1 bins (real length 2M) stored with its first
+// nm positions (a multiple of 2h) in tiles of 2h positions, tile j holding
+// bins h*j .. h*j + h - 1 in its first half and their mirrors M - h*j ..
+// M - h*j - h + 1 in its second half, so position c and c ^ h of a tile hold
+// a pair (k, M - k); the remaining M + 1 - nm positions hold the middle bins
+// nm/2 .. M - nm/2 in order.  A transform along another axis that sees whole
+// tiles then has both bins of every pair in lanes l and l ^ h.
+__host__ __device__ __forceinline__ int tm_pair_bin(int q, int nm, int M, int h) {
+  if (q >= nm) return nm / 2 + (q - nm);
+  const int j = q / (2 * h), c = q - j * 2 * h;
+  return c < h ? h * j + c : M - h * j - (c - h);
+}
+__host__ __device__ __forceinline__ int tm_pair_pos(int k, int nm, int M, int h) {
+  const int np = nm / 2;
+  if (k < np) return (k / h) * 2 * h + k % h;
+  if (M - k < np) return ((M - k) / h) * 2 * h + h + (M - k) % h;
+  return nm + (k - np);
+}
+
+// The M-point complex input of a row's C2R (real length N2 = 2M) at bin k:
+// Z[k] = (X[k] + X*[M-k]) + i W^k (X[k] - X*[M-k]), W = exp(2 pi i / N2),
+// from a = X[k], b = X[M-k] and w = W^k; the imaginary parts of the DC and
+// Nyquist bins are dropped (k == 0), as numpy's irfft does.  The inverse
+// M-point transform of Z holds the outputs 2n and 2n + 1 in its real and
+// imaginary parts.
+__device__ __forceinline__ float2 tm_c2r_pre(float2 a, float2 b, float2 w, bool dc) {
+  a.y = dc ? 0.f : a.y;
+  b.y = dc ? 0.f : b.y;
+  const float2 e = make_float2(a.x + b.x, a.y - b.y);
+  const float2 d = c_mul(make_float2(a.x - b.x, a.y + b.y), w);
+  return make_float2(e.x - d.y, e.y + d.x);
+}
+
 // ---- streaming global accesses --------------------------------------------
 // Non-temporal cache policy for data that is larger than every cache and
 // touched once per pass (the correlation spectra of the fused scan, full

@@ -351,14 +351,15 @@ def test_c2_full_size_vs_fp64_oracle():
         _vs_oracle(target, entry, result, -1.0, workers=8)
 
 
-@pytest.mark.parametrize("shape", [(1024, 48, 256), (128, 1024, 256)])
+@pytest.mark.parametrize("shape", [(1024, 128, 256), (128, 1024, 256), (1024, 48, 256)])
 @pytest.mark.parametrize("fused", [False, True])
 def test_1024_point_axes_vs_oracle(shape, fused):
     """The 1024-point kernel instantiations of the large-tomogram scan
-    against the FP64 oracle: pass A's z transform at z = 1024 (1024, 48,
-    256) and pass B's y transform at y = 1024 (128, 1024, 256), both with
-    the split rows of the 256-point x axis; 32^3 template with a threshold
-    mask, 2 rotations."""
+    against the FP64 oracle: pass A's z transform at z = 1024 (1024, 128,
+    256) and pass B's y transform at y = 1024 (128, 1024, 256, mirror
+    pairs at lane distance 4), both with the split rows of the 256-point x
+    axis, and a y length without a compile-time kernel (1024, 48, 256:
+    padded rows); 32^3 template with a threshold mask, 2 rotations."""
     from pytme_amd.synthetic import blob_template, embed_copies, threshold_mask
     from pytme_amd.rotations import uniform_rotations
 
