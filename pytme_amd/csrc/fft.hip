@@ -64,6 +64,12 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
 // predicated; the compiler can then wait for the prefetched tile without
 // draining the previous tile's stores.
 //
+// PRU: pruned input (lin <= N / R_IN, the zero-padded template lines of
+// the fused scan): only the first input of each first-stage butterfly can
+// be non-zero, so only those are loaded (rows past lin clamped to a valid
+// row and replaced by zero) and the others are compile-time zeros; stores
+// as FULL (tiles inside ld).
+//
 // PH > 0 (inverse transforms of mirror-pair rows, tm_pair_bin with h = PH
 // and tiles of L = 2 PH positions): before the store each output is replaced
 // by the C2R input tm_c2r_pre of its bin k from the pair (X[k], X[M-k]),
@@ -77,12 +83,14 @@ __device__ __forceinline__ float tm_lane_xor(float v) {
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (PH << 10)));
 }
 
-template <int N, int L, int NT, int DIR, bool FULL, int V = 0, int PH = 0>
+template <int N, int L, int NT, int DIR, bool FULL, int V = 0, int PH = 0, bool PRU = false>
 __global__ void __launch_bounds__(NT)
 k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner, int ld,
          int64_t tiles_per_outer, int64_t ntiles, float scale, const float2* __restrict__ twx, int M) {
   using PL = CtPlan<N, L, NT, true, false, V>;
   constexpr int EI = PL::E_IN, EO = PL::E_OUT, EW = EI > EO ? EI : EO;
+  constexpr int BI = EI / PL::R_IN;  // first-stage butterflies per thread
+  static_assert(!PRU || (FULL && PH == 0), "pruned lines are stored unpredicated");
   constexpr bool WIDE = L * sizeof(float2) >= 128;  // whole-line row segments: streaming hints
   static_assert(PH == 0 || (L == 2 * PH && NT % L == 0 && FULL && (PH == 8 || PH == 4)),
                 "mirror pairs need whole tiles of 2 PH lines and unpredicated tiles");
@@ -95,12 +103,21 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
     for (int i = tid; i < 2 * M; i += NT) twxs[i] = twx[i];
   // persistent blocks: tile t = blockIdx.x + k * gridDim.x; the next tile's
   // loads are issued before the current tile's transform
-  float2 v[EW], nx[EI];
+  float2 v[EW], nx[PRU ? BI : EI];
   auto load = [&](int64_t t, float2* dstv) {
     const int64_t o = t / tiles_per_outer;
     const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
     const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
     const float2* src = in + o * (int64_t)lin * ld + i0;
+    if constexpr (PRU) {
+#pragma unroll
+      for (int i = 0; i < BI; ++i) {
+        int l, j;
+        PL::template slot<PL::R_IN>(tid, i, l, j);
+        dstv[i] = tm_ld_stream<WIDE>(src + (j < lin ? j : lin - 1) * ld + l);
+      }
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < EI / PL::R_IN; ++i)
 #pragma unroll
@@ -115,11 +132,27 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
   };
   // the blocks of one XCD take consecutive tiles (neighbouring column tiles
   // of the same rows share cache lines and DRAM pages in that XCD's L2)
+  // PRU: the loaded first inputs x into the transform registers w
+  auto expand = [&](const float2* x, float2* w) {
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
+      int l, j;
+      PL::template slot<PL::R_IN>(tid, i, l, j);
+#pragma unroll
+      for (int r = 0; r < PL::R_IN; ++r) w[i * PL::R_IN + r] = r == 0 && j < lin ? x[i] : make_float2(0.f, 0.f);
+    }
+  };
   int64_t t = tm_xcd_block();
   if (t >= ntiles) return;
-  load(t, v);
-  ct_ready<EI>(v);
+  if constexpr (PRU) {
+    load(t, nx);
+    ct_ready<BI>(nx);
+  } else {
+    load(t, v);
+    ct_ready<EI>(v);
+  }
   for (; t < ntiles; t += gridDim.x) {
+    if constexpr (PRU) expand(nx, v);
     // on the last tile the current one is reloaded and discarded: the loop
     // carries no predicated loads, so the wait for the prefetch at the loop
     // end does not also wait for this tile's stores
@@ -152,9 +185,13 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
         else if (ok && l < ti)
           dst[e * ld + l] = c_scale(v[i * PL::R_OUT + r], scale);
       }
+    if constexpr (PRU) {
+      ct_ready<BI>(nx);
+    } else {
 #pragma unroll
-    for (int s = 0; s < EI; ++s) v[s] = nx[s];
-    ct_ready<EI>(v);
+      for (int s = 0; s < EI; ++s) v[s] = nx[s];
+      ct_ready<EI>(v);
+    }
   }
 }
 
@@ -179,17 +216,16 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   const size_t shm = (size_t)N * L * sizeof(float2);
   // persistent grid: exactly the blocks that are resident at once (LDS and
   // register limits), so every block runs from the start
-  static int per_cu = 0;
-  if (per_cu == 0) {
+  auto occupancy = [&](const void* kernel) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_ct<N, L, NT, 1, true, V, PH>, NT, shm) !=
-            hipSuccess ||
-        nb < 1)
-      nb = 1;
-    per_cu = nb;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, shm) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+  };
+  static int per_cu = 0, per_cu_pruned = 0;
+  if (per_cu == 0) {
+    per_cu = occupancy((const void*)k_c2c_ct<N, L, NT, 1, true, V, PH>);
+    per_cu_pruned = occupancy((const void*)k_c2c_ct<N, L, NT, -1, true, V, 0, true>);
   }
-  int64_t blocks = (int64_t)device_cus() * per_cu;
-  if (blocks > ntiles) blocks = ntiles;
   const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
   using PL = CtPlan<N, L, NT, true, false, V>;
   // unpredicated form: all slots valid, no pruning, tiles inside the stride
@@ -198,6 +234,24 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   TM_TIMED(name, bytes, s,                                                                                      \
            hipLaunchKernelGGL((k_c2c_ct<N, L, NT, D, F, V>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out, lp.tw, \
                               lin, inner, (int)ld, tpo, ntiles, scale, (const float2*)nullptr, 0))
+  // pruned form: zero-padded lines whose first-stage butterflies have one
+  // non-zero input, all slots valid, tiles inside the stride
+  const bool pruned = PH == 0 && lin <= N / PL::R_IN && (N / PL::R_IN * L) % NT == 0 &&
+                      (N / PL::R_OUT * L) % NT == 0 && tpo * L <= ld;
+  int64_t blocks = (int64_t)device_cus() * (pruned ? per_cu_pruned : per_cu);
+  if (blocks > ntiles) blocks = ntiles;
+  if (pruned) {
+    if (dir < 0)
+      TM_TIMED(name, bytes, s,
+               hipLaunchKernelGGL((k_c2c_ct<N, L, NT, -1, true, V, 0, true>), dim3((unsigned)blocks), dim3(NT), shm, s,
+                                  in, out, lp.tw, lin, inner, (int)ld, tpo, ntiles, scale, (const float2*)nullptr, 0));
+    else
+      TM_TIMED(name, bytes, s,
+               hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1, true, V, 0, true>), dim3((unsigned)blocks), dim3(NT), shm, s,
+                                  in, out, lp.tw, lin, inner, (int)ld, tpo, ntiles, scale, (const float2*)nullptr, 0));
+    TM_CHECK_LAUNCH();
+    return TM_OK;
+  }
   if constexpr (PH > 0) {
     TM_REQUIRE(full && dir > 0 && in == out && twx && 2 * M <= 256 && inner % L == 0,
                "mirror-pair C2C: in-place inverse over whole unpredicated tiles, rows of at most 129 bins");

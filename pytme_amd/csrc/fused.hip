@@ -314,8 +314,13 @@ k_pass_c(const float2* __restrict__ Q, LinePlan lp, int64_t nrows, int64_t ldq, 
 // columns of one ky; spectra are multiplied in registers between the forward
 // (pruned) and inverse transforms.  KIND 1: X = F t (template slot q_in,
 // output slot o_x); KIND 0 (FLC): S1 = F m, S2 = F2 m (mask slot q_in, output
-// slots 0 and 1).  Splitting the FLC work into two launches keeps the
-// resident spectra and transform buffers within the register file.
+// slots 0 and 1); KIND 2 (FLC): all three per rotation (mask slot 0,
+// template slot 1, outputs 0, 1, 2) with the two target tiles read once.
+// The main launches split the FLC work into KIND 0 + KIND 1, which keeps
+// the resident spectra and transform buffers within the register file (a
+// KIND 2 main kernel with deferred stores spills at 128 VGPRs: 11.6 ms
+// against 6.6 + 4.1 ms on 512x512x200 at B = 64); the short tail launches
+// take KIND 2 (1.04 against 0.73 + 0.41 ms).
 //
 // The spectra rows are padded to a multiple of L bins, so every load and
 // store is unpredicated (padding columns are transformed too and never
@@ -339,7 +344,8 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   using PF = CtPlan<N0, L, NT, true, false, V>;
   using PI = CtPlan<N0, L, NT, true, true, V>;
   static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
-  static_assert(!DEFER || (!TAIL && PF::NST >= 2 && PI::NST == PF::NST), "deferred stores need staged transforms");
+  static_assert(!DEFER || (!TAIL && KIND != 2 && PF::NST >= 2 && PI::NST == PF::NST),
+                "deferred stores need staged transforms (and one or two outputs per rotation)");
   static_assert((N0 / PF::R_IN * L) % NT == 0 && (N0 / PF::R_OUT * L) % NT == 0 &&
                     (N0 / PI::R_OUT * L) % NT == 0,
                 "every butterfly slot must be valid");
@@ -369,10 +375,11 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   const int64_t col = (int64_t)ky * N2s + kx0;
   const int plane_o = N1 * ldo;  // output rows of ldo bins
   const int64_t col_o = (int64_t)ky * ldo + kx0;
+  constexpr bool TWO = KIND != 1;  // both target spectra
   const float2* Ftc = Ft + col;
-  const float2* Ft2c = KIND == 0 ? Ft2 + col : nullptr;
+  const float2* Ft2c = TWO ? Ft2 + col : nullptr;
 
-  float2 F[EO], F2[KIND == 0 ? EO : 1];
+  float2 F[EO], F2[TWO ? EO : 1];
 #pragma unroll
   for (int i = 0; i < EO / PF::R_OUT; ++i)
 #pragma unroll
@@ -382,7 +389,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       const int s = i * PF::R_OUT + r;
       const int lo = TAIL ? tl : l;
       F[s] = Ftc[e * plane + lo];
-      if constexpr (KIND == 0) F2[s] = Ft2c[e * plane + lo];
+      if constexpr (TWO) F2[s] = Ft2c[e * plane + lo];
     }
 
   // input element of load slot k: rows e >= T0 are clamped to a valid row
@@ -397,8 +404,9 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
     }
   };
   float2 x[NX], xn[NX];
-  auto load_in = [&](int b, float2* dst) {
-    const float2* src = P + ((int64_t)(q_in * B + b) * T0) * (int64_t)plane + col;
+  float2 xt[KIND == 2 ? NX : 1], xtn[KIND == 2 ? NX : 1];  // KIND 2: the template's input columns
+  auto load_in = [&](int q, int b, float2* dst) {
+    const float2* src = P + ((int64_t)(q * B + b) * T0) * (int64_t)plane + col;
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       int l, e;
@@ -463,16 +471,13 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   using CN2 = std::integral_constant<int, 2 * NH>;
   using CN1 = std::integral_constant<int, NH>;
 
-  // one rotation; FIRST: no held output to store yet
-  auto rotation = [&](int b, auto first) {
-    constexpr bool FIRST = decltype(first)::value;
-    load_in(b + 1 < b1 ? b + 1 : b, xn);  // the last rotation reloads itself, keeping the loop unpredicated
-    float2 w[EW];
+  // the first-stage input registers of a transform from the loaded columns
+  auto expand = [&](const float2* xin, float2* w) {
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       int l, e;
       in_elem(k, l, e);
-      const float2 v = e < T0 ? x[k] : make_float2(0.f, 0.f);
+      const float2 v = e < T0 ? xin[k] : make_float2(0.f, 0.f);
       if constexpr (PR) {
 #pragma unroll
         for (int r = 0; r < PF::R_IN; ++r) w[k * PF::R_IN + r] = r == 0 ? v : make_float2(0.f, 0.f);
@@ -480,7 +485,32 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
         w[k] = v;
       }
     }
-    if constexpr (DEFER) {
+  };
+
+  // one rotation; FIRST: no held output to store yet
+  auto rotation = [&](int b, auto first) {
+    constexpr bool FIRST = decltype(first)::value;
+    const int bn = b + 1 < b1 ? b + 1 : b;  // the last rotation reloads itself, keeping the loop unpredicated
+    load_in(KIND == 2 ? 0 : q_in, bn, xn);
+    if constexpr (KIND == 2) load_in(1, bn, xtn);
+    float2 w[EW];
+    expand(x, w);
+    if constexpr (KIND == 2) {
+      ct_fft<PF, -1, L>(lds, tws, w, w, tid);
+      float2 t[EO];
+#pragma unroll
+      for (int s = 0; s < EO; ++s) {
+        t[s] = c_mul(F2[s], w[s]);
+        w[s] = c_mul(F[s], w[s]);
+      }
+      inv_store(0, b, w);
+      inv_store(1, b, t);
+      expand(xt, w);
+      ct_fft<PF, -1, L>(lds, tws, w, w, tid);
+#pragma unroll
+      for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
+      inv_store(2, b, w);
+    } else if constexpr (DEFER) {
       // the held output: first half over the forward's stages, second half
       // over the first inverse's
       auto fwd_hook = [&](auto st) {
@@ -531,12 +561,19 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
 #pragma unroll
     for (int k = 0; k < NX; ++k) x[k] = xn[k];
     ct_ready<NX>(x);
+    if constexpr (KIND == 2) {
+#pragma unroll
+      for (int k = 0; k < NX; ++k) xt[k] = xtn[k];
+      ct_ready<NX>(xt);
+    }
   };
 
-  load_in(b0, x);
+  load_in(KIND == 2 ? 0 : q_in, b0, x);
+  if constexpr (KIND == 2) load_in(1, b0, xt);
   ct_ready<EO>(F);
-  if constexpr (KIND == 0) ct_ready<EO>(F2);
+  if constexpr (TWO) ct_ready<EO>(F2);
   ct_ready<NX>(x);
+  if constexpr (KIND == 2) ct_ready<NX>(xt);
   if constexpr (DEFER) {
     if (b0 < b1) rotation(b0, std::true_type{});
     for (int b = b0 + 1; b < b1; ++b) rotation(b, std::false_type{});
@@ -805,10 +842,10 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   const int64_t blocks = (int64_t)N1 * tpr;
   const size_t shm = (size_t)N0 * L * sizeof(float2);
   // algorithmic bytes of a launch over `bins` bins of every row: the target
-  // spectra it multiplies by (nin), the T0 pruned planes of one template-side
-  // input per rotation and its nout outputs per rotation
-  auto bytes_of = [&](int bins, int nin, int nout) {
-    return 8.0 * bins * N1 * ((double)N0 * nin + (double)T0 * B + (double)N0 * nout * B);
+  // spectra it multiplies by (nin), the T0 pruned planes of nq template-side
+  // inputs per rotation and its nout outputs per rotation
+  auto bytes_of = [&](int bins, int nin, int nout, int nq = 1) {
+    return 8.0 * bins * N1 * ((double)N0 * nin + (double)T0 * nq * B + (double)N0 * nout * B);
   };
   const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
   // the tail blocks are fewer than the CUs: split their rotations over gridDim.y
@@ -821,12 +858,20 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   constexpr bool DS = PR && NT0 >= 1024 && CtPlan<N0, L, NT0, true, false, V>::NST >= 2;
   constexpr bool DX = PR && NT >= 1024 && CtPlan<N0, L, NT, true, false, V>::NST >= 2;
   constexpr int NTT = NT > 512 ? 512 : NT;
+  // FLC tails in one launch (KIND 2) for pruned transforms of up to 512
+  // points (1024-point or unpruned ones would spill with five spectra)
+  constexpr bool TAIL1 = N0 <= 512 && PR;
   if (flc) {
     TM_TIMED("fused_pass_a", bytes_of(nm, 2, 2), s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, false, V, DS>), dim3((unsigned)blocks), dim3(NT0), shm, s, P, ft, ft2,
                                 Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, 0, 0));
     TM_CHECK_LAUNCH();
-    if (ntl > 0) {
+    if (ntl > 0 && TAIL1) {
+      TM_TIMED("fused_pass_a_tail", bytes_of(ntl, 2, 3, 2), s,
+               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 2, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT0),
+                                  shm, s, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 2));
+      TM_CHECK_LAUNCH();
+    } else if (ntl > 0) {
       TM_TIMED("fused_pass_a_tail", bytes_of(ntl, 2, 2), s,
                hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT0), shm, s, P,
                                   ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 0));
@@ -838,7 +883,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
                               (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, tpr, 0, ntl, flc ? 1 : 0,
                               flc ? 2 : 0));
   TM_CHECK_LAUNCH();
-  if (ntl > 0) {
+  if (ntl > 0 && !(flc && TAIL1)) {
     TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", bytes_of(ntl, 1, 1), s,
              hipLaunchKernelGGL((k_pass_a_ct<N0, L, NTT, 1, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NTT), shm, s, P, ft,
                                 (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
