@@ -195,7 +195,7 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
   }
 }
 
-static int device_cus() {
+int tm_device_cus() {
   static int cus[64] = {0};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
@@ -238,7 +238,7 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   // non-zero input, all slots valid, tiles inside the stride
   const bool pruned = PH == 0 && lin <= N / PL::R_IN && (N / PL::R_IN * L) % NT == 0 &&
                       (N / PL::R_OUT * L) % NT == 0 && tpo * L <= ld;
-  int64_t blocks = (int64_t)device_cus() * (pruned ? per_cu_pruned : per_cu);
+  int64_t blocks = (int64_t)tm_device_cus() * (pruned ? per_cu_pruned : per_cu);
   if (blocks > ntiles) blocks = ntiles;
   if (pruned) {
     if (dir < 0)

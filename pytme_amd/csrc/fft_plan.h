@@ -34,6 +34,8 @@ int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, in
 // twiddles of the row transform).  Lengths 128, 256, 512 (ph 8), 1024 (ph 4).
 int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, float2* buf, int64_t outer,
                         int64_t inner, int64_t ld, hipStream_t s, const char* name, int variant = 0);
+// Compute units of the current device (cached per device).
+int tm_device_cus();
 // The ph of tm_launch_c2c_pairs for length n, 0 if it has no kernel.
 int tm_c2c_pair_h(int n);
 // R2C of nrows contiguous rows of lin <= n reals (zero-padded to n) into

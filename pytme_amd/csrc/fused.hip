@@ -848,8 +848,28 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
     return 8.0 * bins * N1 * ((double)N0 * nin + (double)T0 * nq * B + (double)N0 * nout * B);
   };
   const unsigned tail_blocks = ntl > 0 ? (unsigned)tm_ceil_div(N1, L / ntl) : 0;  // L / ntl rows per tail tile
-  // the tail blocks are fewer than the CUs: split their rotations over gridDim.y
-  const unsigned tail_split = ntl > 0 ? (unsigned)std::min<int64_t>(B, std::max<int64_t>(1, tm_ceil_div(512, tail_blocks))) : 1;
+  // The tail tiles are fewer than the CUs: their rotations are split over
+  // gridDim.y, s ways, s <= 8 chosen so that the blocks fill the rounds of
+  // resident blocks best (ties: fewer splits, each split re-reads its
+  // target tile).  512 x 512 x 200: 171 tiles, s = 7 fills 93 % of the
+  // last round where s = 3 left it with one block.
+  auto tail_split_for = [&](const void* kernel, int nt) -> unsigned {
+    if (ntl == 0) return 1;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, nt, shm) != hipSuccess || per < 1) per = 1;
+    const int64_t R = (int64_t)tm_device_cus() * per;
+    unsigned best = 1;
+    double best_u = 0.0;
+    for (int64_t sp = 1; sp <= std::min<int64_t>(B, 8); ++sp) {
+      const int64_t n = (int64_t)tail_blocks * sp;
+      const double u = (double)n / (double)(tm_ceil_div(n, R) * R);
+      if (u > best_u + 1e-9) {
+        best_u = u;
+        best = (unsigned)sp;
+      }
+    }
+    return best;
+  };
   // pruned main launches in 1024-thread blocks store deferred (8 outputs
   // per thread; with 16, in the smaller blocks of the 128- and 256-point
   // kernels or unpruned, the held outputs push the kernels past 128 VGPRs or
@@ -868,12 +888,14 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
     TM_CHECK_LAUNCH();
     if (ntl > 0 && TAIL1) {
       TM_TIMED("fused_pass_a_tail", bytes_of(ntl, 2, 3, 2), s,
-               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 2, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT0),
+               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 2, PR, true, V>),
+                                  dim3(tail_blocks, tail_split_for((const void*)k_pass_a_ct<N0, L, NT0, 2, PR, true, V>, NT0)), dim3(NT0),
                                   shm, s, P, ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 2));
       TM_CHECK_LAUNCH();
     } else if (ntl > 0) {
       TM_TIMED("fused_pass_a_tail", bytes_of(ntl, 2, 2), s,
-               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NT0), shm, s, P,
+               hipLaunchKernelGGL((k_pass_a_ct<N0, L, NT0, 0, PR, true, V>),
+                                  dim3(tail_blocks, tail_split_for((const void*)k_pass_a_ct<N0, L, NT0, 0, PR, true, V>, NT0)), dim3(NT0), shm, s, P,
                                   ft, ft2, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, 0, 0));
       TM_CHECK_LAUNCH();
     }
@@ -885,7 +907,8 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
   TM_CHECK_LAUNCH();
   if (ntl > 0 && !(flc && TAIL1)) {
     TM_TIMED(flc ? "fused_pass_a_x_tail" : "fused_pass_a_tail", bytes_of(ntl, 1, 1), s,
-             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NTT, 1, PR, true, V>), dim3(tail_blocks, tail_split), dim3(NTT), shm, s, P, ft,
+             hipLaunchKernelGGL((k_pass_a_ct<N0, L, NTT, 1, PR, true, V>),
+                                dim3(tail_blocks, tail_split_for((const void*)k_pass_a_ct<N0, L, NTT, 1, PR, true, V>, NTT)), dim3(NTT), shm, s, P, ft,
                                 (const float2*)nullptr, Q, Qt, tw, T0, N1, N2s, ldo, B, 1, nm, ntl, flc ? 1 : 0,
                                 flc ? 2 : 0));
     TM_CHECK_LAUNCH();
