@@ -429,6 +429,145 @@ int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, in
   return TM_OK;
 }
 
+// In-place inverse 1024-point C2C over tiles of 16 lines (128-byte row
+// segments, where the 16-line tile of the single 1024-point plan needs
+// 16 elements per thread and spills): the first radix-2 step splits each
+// line into its even and odd outputs, a[n] = x[n] + x[n+512] and
+// b[n] = (x[n] - x[n+512]) W^n (W = exp(2 pi i / 1024)), whose 512-point
+// inverses are outputs 2k and 2k + 1.  A 1024-thread block holds 8
+// elements of each half per thread and runs the two 512-point transforms
+// one after the other (twiddles: every second entry of the 1024 table);
+// persistent blocks prefetch the next tile.  PH: the mirror-pair
+// pre-processing of k_c2c_ct before the stores.
+#ifndef TM_C2C_1024_HALVES
+#define TM_C2C_1024_HALVES 1
+#endif
+template <int V, int PH>
+__global__ void __launch_bounds__(1024)
+k_c2c_1024_halves(float2* buf, const float2* __restrict__ tw, int64_t inner, int ld, int64_t tiles_per_outer,
+                  int64_t ntiles, float scale, const float2* __restrict__ twx, int M) {
+  constexpr int L = 16, NT = 1024;
+  using PL = CtPlan<512, L, NT, true, false, V>;
+  constexpr int EI = PL::E_IN, EO = PL::E_OUT;
+  static_assert(EI == 8 && EO == 8 && (512 / PL::R_IN * L) % NT == 0 && (512 / PL::R_OUT * L) % NT == 0,
+                "8 elements of every slot per thread");
+  static_assert(PH == 0 || PH == 8, "mirror pairs at lane distance 8 in 16-line tiles");
+  extern __shared__ float2 lds[];
+  __shared__ float2 tws[1024];
+  __shared__ float2 twxs[PH > 0 ? 256 : 1];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 1024; i += NT) tws[i] = tw[i];
+  if constexpr (PH > 0)
+    for (int i = tid; i < 2 * M; i += NT) twxs[i] = twx[i];
+  float2 x0[EI], x1[EI];
+  // rows half * 512 + e of tile t
+  auto load = [&](int64_t t, int half, float2* d) {
+    const int64_t o = t / tiles_per_outer;
+    const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
+    const float2* src = buf + o * 1024 * (int64_t)ld + i0 + (int64_t)half * 512 * ld;
+#pragma unroll
+    for (int i = 0; i < EI / PL::R_IN; ++i)
+#pragma unroll
+      for (int r = 0; r < PL::R_IN; ++r) {
+        int l, e;
+        PL::in_pos(tid, i, r, l, e);
+        d[i * PL::R_IN + r] = tm_ld_stream(src + e * ld + l);
+      }
+  };
+  int64_t t = tm_xcd_block();
+  if (t >= ntiles) return;
+  load(t, 0, x0);
+  load(t, 1, x1);
+  ct_ready<EI>(x0);
+  ct_ready<EI>(x1);
+  for (; t < ntiles; t += gridDim.x) {
+    // the next tile's halves are loaded into x0 / x1 once this tile no longer
+    // needs them: the first before the even outputs' transform, the second
+    // before the odd outputs' (the last tile reloads itself)
+    const int64_t tn = t + gridDim.x < ntiles ? t + gridDim.x : t;
+    float2 a[EI], b[EI];
+#pragma unroll
+    for (int i = 0; i < EI / PL::R_IN; ++i)
+#pragma unroll
+      for (int r = 0; r < PL::R_IN; ++r) {
+        int l, e;
+        PL::in_pos(tid, i, r, l, e);
+        const int s2 = i * PL::R_IN + r;
+        a[s2] = c_add(x0[s2], x1[s2]);
+        b[s2] = c_mul(c_sub(x0[s2], x1[s2]), tw_load<1>(tws, e));
+      }
+    const int64_t o = t / tiles_per_outer;
+    const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
+    float2* dst = buf + o * 1024 * (int64_t)ld + i0;
+    int kb = 0;
+    float2 wb = make_float2(1.f, 0.f);
+    if constexpr (PH > 0) {
+      const int jt = (int)(i0 / L), lt = tid % L;
+      kb = lt < PH ? PH * jt + lt : M - PH * jt - (lt - PH);
+      wb = tw_load<1>(twxs, kb);
+    }
+    auto finish = [&](float2* v, int half) {
+      if constexpr (PH > 0) {
+#pragma unroll
+        for (int s2 = 0; s2 < EO; ++s2) {
+          const float2 p = v[s2];
+          const float2 q = make_float2(tm_lane_xor<PH>(p.x), tm_lane_xor<PH>(p.y));
+          v[s2] = tm_c2r_pre(p, q, wb, kb == 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < EO / PL::R_OUT; ++i)
+#pragma unroll
+        for (int r = 0; r < PL::R_OUT; ++r) {
+          int l, k;
+          PL::out_pos(tid, i, r, l, k);
+          tm_st_stream(dst + (2 * k + half) * ld + l, c_scale(v[i * PL::R_OUT + r], scale));
+        }
+    };
+    // b waits in the second LDS region (each thread reads back only its own
+    // slots, so no barrier) while a is transformed
+    float2* park = lds + 512 * L;
+#pragma unroll
+    for (int s2 = 0; s2 < EI; ++s2) park[s2 * NT + tid] = b[s2];
+    load(tn, 0, x0);
+    ct_fft<PL, 1, L, 2>(lds, tws, a, a, tid);
+    finish(a, 0);
+    load(tn, 1, x1);
+#pragma unroll
+    for (int s2 = 0; s2 < EI; ++s2) b[s2] = park[s2 * NT + tid];
+    ct_fft<PL, 1, L, 2>(lds, tws, b, b, tid);
+    finish(b, 1);
+    ct_ready<EI>(x0);
+    ct_ready<EI>(x1);
+  }
+}
+
+template <int V, int PH>
+static int launch_c2c_1024_halves(const LinePlan& lp, float2* buf, int64_t outer, int64_t inner, int64_t ld,
+                                  float scale, hipStream_t s, const char* name, const float2* twx, int M) {
+  TM_REQUIRE(lp.n == 1024 && inner % 16 == 0 && inner <= ld && (PH == 0 || (twx && 2 * M <= 256)),
+             "1024-point halves: whole 16-line tiles inside the stride");
+  const int64_t tpo = inner / 16, ntiles = outer * tpo;
+  const size_t shm = 2 * (size_t)512 * 16 * sizeof(float2);  // the transform's exchange buffer + the parked half
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_c2c_1024_halves<V, PH>, 1024, shm) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    per_cu = nb;
+  }
+  int64_t blocks = (int64_t)tm_device_cus() * per_cu;
+  if (blocks > ntiles) blocks = ntiles;
+  const double bytes = 16.0 * 1024 * (double)inner * (double)outer;
+  TM_TIMED(name, bytes, s,
+           hipLaunchKernelGGL((k_c2c_1024_halves<V, PH>), dim3((unsigned)blocks), dim3(1024), shm, s, buf, lp.tw, inner,
+                              (int)ld, tpo, ntiles, scale, twx, M));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
 int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, float2* buf, int64_t outer,
                         int64_t inner, int64_t ld, hipStream_t s, const char* name, int variant) {
   TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
@@ -442,6 +581,8 @@ int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, fl
       return launch_c2c_ct<256, 16, 256, V, 8>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
     if (ph == 8 && lp.n == 512)
       return launch_c2c_ct<512, 16, 512, V, 8>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    if (ph == 8 && lp.n == 1024)
+      return launch_c2c_1024_halves<V, 8>(lp, buf, outer, inner, ld, 1.0f, s, name, twx, M);
     if (ph == 4 && lp.n == 1024)
       return launch_c2c_ct<1024, 8, 512, V, 4>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
     tm_set_error("mirror-pair C2C: no kernel for length %d with pairs at lane distance %d", lp.n, ph);
@@ -452,7 +593,9 @@ int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, fl
 
 // Lane distance of the mirror pairs for an in-place inverse C2C of length
 // n whose compile-time kernel takes tiles of 2h lines (0: none).
-int tm_c2c_pair_h(int n) { return (n == 128 || n == 256 || n == 512) ? 8 : n == 1024 ? 4 : 0; }
+int tm_c2c_pair_h(int n) {
+  return (n == 128 || n == 256 || n == 512 || (n == 1024 && TM_C2C_1024_HALVES)) ? 8 : n == 1024 ? 4 : 0;
+}
 
 static int rows_per_block(int n, int64_t nrows) {
   int r = 2048 / (n + 1);

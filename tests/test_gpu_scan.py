@@ -124,13 +124,16 @@ def test_filtered_parity(score, fused):
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
 
 
-@pytest.mark.parametrize("shape", [(128, 128, 200), (128, 128, 128), (128, 64, 256)])
+@pytest.mark.parametrize("shape", [(128, 128, 200), (128, 128, 128), (128, 64, 256), (128, 256, 200), (256, 128, 256)])
 @pytest.mark.parametrize("score", ["FLC", "FLCSphericalMask"])
 @pytest.mark.parametrize("fused", [False, True])
 def test_specialised_sizes_parity(shape, score, fused):
-    """Tomogram sizes that take the compile-time FFT kernels (512/128-point
+    """Tomogram sizes that take the compile-time FFT kernels (128/256-point
     axes, half-length 200/128-point C2R) against the FP64 oracle; x lengths
-    200 and 256 run the fused scan on split rows (96 + 5 and 128 + 1 bins)."""
+    200 and 256 run the fused scan on split rows (96 + 5 and 128 + 1 bins)
+    in mirror-pair order where the y length has a compile-time kernel (128
+    and 256 here: pass B ends with the x-inverse pre-processing), padded
+    rows otherwise (y length 64)."""
     target, template, mask, rots = _workload(seed=31, shape=shape, T=16, n_rot=4, copies=4)
     if score == "FLCSphericalMask":
         from pytme_amd.synthetic import spherical_mask
@@ -524,9 +527,9 @@ def test_tile_shapes_give_identical_results(tiles):
     """The tile shapes the autotuner chooses between (pass A 16- or 8-column
     z tiles, pass C 5- or 10-row blocks) change how lines are spread over
     threads, not the arithmetic: the scan state is bit-identical to the
-    default tiles on a 128x96x200 target (both pass A and the 200-point
-    pass C have both variants there)."""
-    target, template, mask, rots = _workload(seed=81, shape=(128, 96, 200), n_rot=8)
+    default tiles on a 128x128x200 target (both pass A and the 200-point
+    pass C on mirror-pair split rows have both variants there)."""
+    target, template, mask, rots = _workload(seed=81, shape=(128, 128, 200), n_rot=8)
     print(f"tile shapes {tiles} vs default", flush=True)
     s0, _, r0, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=4)
     s1, _, r1, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=4,
@@ -651,8 +654,8 @@ def test_installed_pytme_reaches_the_fused_scan(monkeypatch):
     be.free_cache()
 
 
-@pytest.mark.parametrize("shape,T", [((256, 128, 200), 24), ((512, 64, 256), 32), ((128, 256, 128), 24),
-                                     ((1024, 48, 256), 32)])
+@pytest.mark.parametrize("shape,T", [((256, 128, 200), 24), ((512, 256, 256), 32), ((128, 256, 128), 24),
+                                     ((1024, 128, 256), 32)])
 def test_radix_variant_vs_oracle(shape, T):
     """The fused scan with the second radix plan of the compile-time FFTs
     (tm_scan_plan_set_radix 1: 256 = 4*8*8, 512 = 4*4*4*8, 1024 = 4*4*8*8,
