@@ -666,6 +666,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   // (or past the block's rl rows) re-read a valid bin and write the junk
   // line RL.
   int offs[NPF], lpos[NPF];
+  uint32_t boffs[NPF];  // byte offsets of offs (slots that never meet the tail: offs >= 0)
   int offs2[NPF];      // split rows, slots meeting the tail: the pair partner's offset
   bool tl[NPF];        // this lane's element of the slot is a tail bin
   float2 wtw[NPF];     // and W^k of its bin
@@ -703,18 +704,27 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
       offs[p] = l * ldq + e;
       lpos[p] = f < tile_n ? l * LD + e : RL * LD;
     }
+    boffs[p] = (uint32_t)offs[p] * (uint32_t)sizeof(float2);
+  }
+  // FLC: scale / n of every rotation of the batch, once per block (the
+  // same float32 division the epilogue would repeat per rotation)
+  float* snl = reinterpret_cast<float*>(lds + (RL + 1) * LD);
+  if constexpr (NQ == 3) {
+    for (int b = tid; b < B; b += NT) snl[b] = scale / (float)n_obs[b];
   }
   auto prefetch = [&](int it, float2* pf, float2* pf2) {
     const int b = it / NQ, o = it - (it / NQ) * NQ;
     const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;  // r0 = 0 with split rows
     const float2* tsrc = SPLIT ? Qt + (int64_t)(o * B + b) * nrows * NTL : nullptr;
+    // slots that never meet the tail: a uniform base and a 32-bit byte offset
+    const char* srcb = reinterpret_cast<const char*>(src);
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
-      if constexpr (SPLIT) {
+      if (SPLIT && tail_slot(p)) {
         pf[p] = tm_ld_stream(offs[p] >= 0 ? src + offs[p] : tsrc + (-1 - offs[p]));
-        if (tail_slot(p)) pf2[p] = tm_ld_stream(offs2[p] >= 0 ? src + offs2[p] : tsrc + (-1 - offs2[p]));
+        pf2[p] = tm_ld_stream(offs2[p] >= 0 ? src + offs2[p] : tsrc + (-1 - offs2[p]));
       } else {
-        pf[p] = tm_ld_stream(src + offs[p]);
+        pf[p] = tm_ld_stream(reinterpret_cast<const float2*>(srcb + boffs[p]));
       }
     }
   };
@@ -746,8 +756,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
         }
       }
     ct_fft<PI, 1, LD, 2>(lds, tws, z, z, tid);
-    const float nf = NQ == 3 ? (float)n_obs[b] : 1.f;
-    const float sn = NQ == 3 ? scale / nf : 0.f;
+    const float sn = NQ == 3 ? snl[b] : 0.f;
     const int ridx = rot_idx[b];
     // Every output slot is updated (idle slots and rows past rl hold junk
     // that is never stored); the branch on o is uniform.  FLC works on the
@@ -952,7 +961,8 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const f
   const int64_t blocks = tm_ceil_div(row_blocks, 8) * 8;
   TM_REQUIRE(blocks <= 0x7fffffff, "pass C grid too large");
   TM_REQUIRE(!SPLIT || (int64_t)N0 * N1 * ntl < (1LL << 31), "pass C: tail offsets exceed 32 bits");
-  const size_t shm = (size_t)(RL + 1) * pass_c_ld(M) * sizeof(float2);  // + the junk line
+  // the tile + the junk line, then scale / n of the batch's rotations (FLC)
+  const size_t shm = (size_t)(RL + 1) * pass_c_ld(M) * sizeof(float2) + (flc ? sizeof(float) * (size_t)B : 0);
   if (flc) {
     static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
     TM_TIMED("fused_pass_c", bytes, s,
