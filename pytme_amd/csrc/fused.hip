@@ -617,8 +617,13 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   using PI = CtPlan<M, RL, NT, false, true, V>;
   constexpr int N2 = 2 * M, N2c = M + 1, LD = pass_c_ld(M);
   constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
-  constexpr int NPF = (RL * N2c + NT - 1) / NT;  // prefetch registers per thread and tile
   constexpr int NM = N2c / 16 * 16, NTL = N2c - NM, NP = NM / 2;  // split rows
+  // load slots per thread and tile.  Split rows: P4 slots of two
+  // consecutive main positions (16-byte loads, NM is even and the rows
+  // 16-byte aligned), then the tail slots of one bin each; otherwise one
+  // bin per slot
+  constexpr int P4 = SPLIT ? (RL * NM / 2 + NT - 1) / NT : 0;
+  constexpr int NPF = SPLIT ? P4 + (RL * NTL + NT - 1) / NT : (RL * N2c + NT - 1) / NT;
   static_assert(PD >= 1 && PD <= 3, "prefetch depth 1 to 3");
   // the tile loop is unrolled by U so that the spectrum index o and the
   // prefetch buffer of every step are compile-time constants
@@ -666,39 +671,40 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   // (or past the block's rl rows) re-read a valid bin and write the junk
   // line RL.
   int offs[NPF], lpos[NPF];
-  uint32_t boffs[NPF];  // byte offsets of offs (slots that never meet the tail: offs >= 0)
-  int offs2[NPF];      // split rows, slots meeting the tail: the pair partner's offset
-  bool tl[NPF];        // this lane's element of the slot is a tail bin
+  int lpos2[SPLIT ? P4 : 1];  // split rows: LDS position of the slot's second bin
+  uint32_t boffs[NPF];  // byte offsets of offs (main slots: offs >= 0)
+  int offs2[NPF];      // split rows, tail slots: the pair partner's offset
   float2 wtw[NPF];     // and W^k of its bin
-  // slots p whose f range [p NT, (p + 1) NT) meets the tail [RL NM, RL N2c)
-  auto tail_slot = [](int p) { return SPLIT && (p + 1) * NT > RL * NM && p * NT < RL * N2c; };
+  // split rows: slots from P4 on hold tail bins
+  auto tail_slot = [](int p) { return SPLIT && p >= P4; };
 #pragma unroll
   for (int p = 0; p < NPF; ++p) {
-    const int f = tid + p * NT;
-    tl[p] = false;
     if constexpr (SPLIT) {
-      if (f < RL * NM) {
-        const int l = f / NM, q = f - l * NM;
+      if (p < P4) {
+        // positions q, q + 1 of row l (rows past rl re-read row rl - 1 and
+        // write the junk line; slots past the tile re-read position 0)
+        const int g = tid + p * NT, g2 = g < RL * NM / 2 ? g : 0;
+        const int l = g2 / (NM / 2), q = 2 * (g2 - l * (NM / 2));
         const int lc = l < rl ? l : rl - 1;
+        const bool ok = g < RL * NM / 2 && l < rl;
         offs[p] = (int)row_of(lc) * ldq + q;
-        lpos[p] = l < rl ? l * LD + tm_pair_bin(q, NM, M, ph) : RL * LD;
+        lpos[p] = ok ? l * LD + tm_pair_bin(q, NM, M, ph) : RL * LD;
+        lpos2[p] = ok ? l * LD + tm_pair_bin(q + 1, NM, M, ph) : RL * LD;
         offs2[p] = offs[p];
-      } else if (f < RL * N2c) {
-        // row (z, y) = (zg0 + l, yg): its tail bins sit at Qt[y][z][c]
-        const int t = f - RL * NM, l = t / NTL, c = t - l * NTL;
+        wtw[p] = make_float2(0.f, 0.f);
+      } else {
+        // bin NP + c of row (z, y) = (zg0 + l, yg) sits at Qt[y][z][c]
+        const int t = tid + (p - P4) * NT, t2 = t < RL * NTL ? t : 0;
+        const int l = t2 / NTL, c = t2 - l * NTL;
         const int lc = l < rl ? l : rl - 1;
         const int base = (yg * N0 + zg0 + lc) * NTL;
-        offs[p] = -1 - (base + c);
-        offs2[p] = -1 - (base + NTL - 1 - c);
-        lpos[p] = l < rl ? l * LD + NP + c : RL * LD;
-        tl[p] = true;
+        offs[p] = base + c;
+        offs2[p] = base + NTL - 1 - c;
+        lpos[p] = t < RL * NTL && l < rl ? l * LD + NP + c : RL * LD;
         wtw[p] = tw_load<1>(tw2, NP + c);
-      } else {
-        offs[p] = offs2[p] = (int)row_of(0) * ldq;
-        lpos[p] = RL * LD;
       }
-      if (!tail_slot(p)) wtw[p] = make_float2(0.f, 0.f);
     } else {
+      const int f = tid + p * NT;
       const int fc = f < tile_n ? f : tile_n - 1;
       const int l = fc / N2c, e = fc - l * N2c;
       offs[p] = l * ldq + e;
@@ -712,17 +718,25 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   if constexpr (NQ == 3) {
     for (int b = tid; b < B; b += NT) snl[b] = scale / (float)n_obs[b];
   }
+  // pf[p]: the slot's bin (main slots of split rows: the first of two, the
+  // second in pf2[p]); pf2[p] of a tail slot: the pair partner's bin
   auto prefetch = [&](int it, float2* pf, float2* pf2) {
     const int b = it / NQ, o = it - (it / NQ) * NQ;
     const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;  // r0 = 0 with split rows
     const float2* tsrc = SPLIT ? Qt + (int64_t)(o * B + b) * nrows * NTL : nullptr;
-    // slots that never meet the tail: a uniform base and a 32-bit byte offset
+    // a uniform base and a 32-bit byte offset per slot
     const char* srcb = reinterpret_cast<const char*>(src);
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
-      if (SPLIT && tail_slot(p)) {
-        pf[p] = tm_ld_stream(offs[p] >= 0 ? src + offs[p] : tsrc + (-1 - offs[p]));
-        pf2[p] = tm_ld_stream(offs2[p] >= 0 ? src + offs2[p] : tsrc + (-1 - offs2[p]));
+      if constexpr (SPLIT) {
+        if (p < P4) {
+          const float4 v = tm_ld_stream2(reinterpret_cast<const float2*>(srcb + boffs[p]));
+          pf[p] = make_float2(v.x, v.y);
+          pf2[p] = make_float2(v.z, v.w);
+        } else {
+          pf[p] = tm_ld_stream(tsrc + offs[p]);
+          pf2[p] = tm_ld_stream(tsrc + offs2[p]);
+        }
       } else {
         pf[p] = tm_ld_stream(reinterpret_cast<const float2*>(srcb + boffs[p]));
       }
@@ -733,12 +747,16 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     lds_barrier();  // the previous transform has finished reading the tile
 #pragma unroll
     for (int p = 0; p < NPF; ++p) {
-      float2 x = pf[p];
-      if (tail_slot(p)) {
-        const float2 zt = tm_c2r_pre(x, pf2[p], wtw[p], false);
-        x = tl[p] ? zt : x;
+      if constexpr (SPLIT) {
+        if (p < P4) {
+          lds[lpos[p]] = pf[p];
+          lds[lpos2[p]] = pf2[p];
+        } else {
+          lds[lpos[p]] = tm_c2r_pre(pf[p], pf2[p], wtw[p], false);
+        }
+      } else {
+        lds[lpos[p]] = pf[p];
       }
-      lds[lpos[p]] = x;
     }
     prefetch(it + PD < iters ? it + PD : iters - 1, pf, pf2);
     lds_barrier();

@@ -109,6 +109,16 @@ __device__ __forceinline__ float2 tm_ld_stream(const float2* p) {
 #endif
   return *p;
 }
+// Two consecutive complex values (16-byte aligned) in one non-temporal load.
+typedef float tm_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 tm_ld_stream2(const float2* p) {
+#if TM_STREAM_HINTS
+  const tm_f4v v = __builtin_nontemporal_load(reinterpret_cast<const tm_f4v*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+#else
+  return *reinterpret_cast<const float4*>(p);
+#endif
+}
 template <bool ON = true>
 __device__ __forceinline__ void tm_st_stream(float2* p, float2 a) {
 #if TM_STREAM_HINTS
