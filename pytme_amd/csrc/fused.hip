@@ -666,10 +666,10 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   }
   // Each load slot's offset within a slab (o, b) and its LDS position are
   // the same for every tile: computed once.  Rows are consecutive at stride
-  // ldq from row r0; split rows: offset >= 0 the main offset from the slab
-  // start, < 0 the tail offset -1-t.  Unpredicated: slots past the tile
-  // (or past the block's rl rows) re-read a valid bin and write the junk
-  // line RL.
+  // ldq from row r0; split rows: main slots offset from the slab start of
+  // Q, tail slots from the slab start of Qt.  Unpredicated: slots past the
+  // tile (or past the block's rl rows) re-read a valid bin and write the
+  // junk line RL.
   int offs[NPF], lpos[NPF];
   int lpos2[SPLIT ? P4 : 1];  // split rows: LDS position of the slot's second bin
   uint32_t boffs[NPF];  // byte offsets of offs (main slots: offs >= 0)
