@@ -154,9 +154,10 @@ def test_deferred_store_batches_vs_oracle(score, batch):
     hold each output in registers and store it during the next transforms
     (the first rotation of a block stores nothing held, the last output is
     stored after the loop): batches of 1, 2 and 3 rotations over 7 rotations
-    (partial last batches) against the FP64 oracle, for the S1/S2 and X
-    kernels (FLC) and the X kernel alone (FLCSphericalMask)."""
-    target, template, mask, rots = _workload(seed=37, shape=(512, 32, 200), T=16, n_rot=7, copies=3)
+    (partial last batches; pass A's tail launch splits fewer rotations)
+    against the FP64 oracle, for the S1/S2 and X kernels (FLC) and the X
+    kernel alone (FLCSphericalMask), on mirror-pair split rows."""
+    target, template, mask, rots = _workload(seed=37, shape=(512, 128, 200), T=16, n_rot=7, copies=3)
     if score == "FLCSphericalMask":
         from pytme_amd.synthetic import spherical_mask
 
@@ -164,7 +165,7 @@ def test_deferred_store_batches_vs_oracle(score, batch):
     print(f"deferred stores, {score} batch {batch}", flush=True)
     s, off, r, _ = _run(score, target, template, mask, rots, True, threshold=-1.0, rotation_batch=batch)
     _, _, raws, oc = orc.scan(score, target, template, rots, template_mask=mask, score_threshold=-1.0,
-                              return_raw=True)
+                              return_raw=True, workers=8)
     n_diff, err = assert_scan_parity(s, r, raws, oc, -1.0)
     print(f"  max err {err:.2e}, tie mismatches {n_diff}", flush=True)
 
