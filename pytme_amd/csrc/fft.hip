@@ -302,6 +302,66 @@ k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, LinePlan p, i
   }
 }
 
+// R2C of rows of lin <= 2M reals through one M-point compile-time transform:
+// z[n] = x[2n] + i x[2n+1] (zero past lin), Z = FFT_M(z), then bin k of the
+// row is X[k] = (Z[k] + Z*[M-k]) / 2 - i W^k (Z[k] - Z*[M-k]) / 2 with
+// W = exp(-2 pi i / 2M) and Z[M] = Z[0] (tw2: the 2M-entry table of W^k).
+// RL rows per block; bins stored as k_r2c_rows stores them.
+template <int M, int RL, int NT, int V>
+__global__ void __launch_bounds__(NT)
+k_r2c_rows_ct(const float* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw2, int lin,
+              int64_t nrows, int64_t ldo, int pair_nm, int pair_h) {
+  using P = CtPlan<M, RL, NT, false, false, V>;
+  constexpr int LD = M + 1;
+  __shared__ float2 lds[P::LDS_LINES * LD];
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * RL;
+  const int rl = (int)((nrows - r0) < RL ? (nrows - r0) : RL);
+  const float* src = in + r0 * lin;
+  float2 v[P::E_IN], o[P::E_OUT];
+#pragma unroll
+  for (int i = 0; i < P::bpt(P::R_IN); ++i)
+#pragma unroll
+    for (int r = 0; r < P::R_IN; ++r) {
+      int l, e;
+      const bool ok = P::in_pos(tid, i, r, l, e);
+      float2 z = make_float2(0.f, 0.f);
+      if (ok && l < rl && 2 * e < lin) {
+        const float* p = src + (int64_t)l * lin + 2 * e;
+        z.x = p[0];
+        if (2 * e + 1 < lin) z.y = p[1];
+      }
+      v[i * P::R_IN + r] = z;
+    }
+  ct_fft<P, -1, LD, 2>(lds, tw2, v, o, tid);
+  lds_barrier();  // the last stage has read the buffer
+  constexpr int RO = P::R_OUT, NB = M / RO;
+#pragma unroll
+  for (int i = 0; i < P::bpt(RO); ++i) {
+    int l, j;
+    if (ct_slot<P, RO>(tid, i, l, j)) {
+#pragma unroll
+      for (int r = 0; r < RO; ++r) lds[ct_lds<P, LD>(l, j + r * NB)] = o[i * RO + r];
+    }
+  }
+  lds_barrier();
+  float2* dst = out + r0 * ldo;
+  for (int f = tid; f < rl * (M + 1); f += NT) {
+    const int l = f / (M + 1), k = f - l * (M + 1);
+    const float2 a = lds[l * LD + (k == M ? 0 : k)];
+    const float2 b = lds[l * LD + (k == 0 ? 0 : M - k)];
+    float2 x;
+    if (k == 0 || k == M) {
+      x = make_float2(k == 0 ? a.x + a.y : a.x - a.y, 0.f);
+    } else {
+      const float2 e = make_float2(a.x + b.x, a.y - b.y);
+      const float2 d = c_mul(tw_load<-1>(tw2, k), make_float2(a.x - b.x, a.y + b.y));
+      x = make_float2(0.5f * (e.x + d.y), 0.5f * (e.y - d.x));
+    }
+    dst[l * ldo + (pair_h ? tm_pair_pos(k, pair_nm, M, pair_h) : k)] = x;
+  }
+}
+
 // C2R along contiguous rows of n/2+1 complex -> n reals, times scale.  The
 // missing half is the Hermitian mirror; taking the real part of the complex
 // inverse discards the imaginary parts of the DC and Nyquist bins exactly as
@@ -619,6 +679,42 @@ int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out
                               nrows, RL, ldo, pair_nm, pair_h));
   TM_CHECK_LAUNCH();
   return TM_OK;
+}
+
+// 0: the fused scan's x pass runs the generic LDS kernel k_r2c_rows.
+#ifndef TM_X_PASS_CT
+#define TM_X_PASS_CT 1
+#endif
+
+template <int M, int RL, int NT, int V>
+static int launch_r2c_rows_ct(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows,
+                              int64_t ldo, hipStream_t s, const char* name, int pair_nm, int pair_h) {
+  const int64_t blocks = tm_ceil_div(nrows, RL);
+  TM_REQUIRE(blocks <= 0x7fffffff, "FFT grid too large");
+  TM_TIMED(name, (4.0 * lin + 8.0 * (M + 1)) * (double)nrows, s,
+           hipLaunchKernelGGL((k_r2c_rows_ct<M, RL, NT, V>), dim3((unsigned)blocks), dim3(NT), 0, s, in, out, lp.tw,
+                              lin, nrows, ldo, pair_nm, pair_h));
+  TM_CHECK_LAUNCH();
+  return TM_OK;
+}
+
+int tm_launch_r2c_rows_ct(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
+                          hipStream_t s, const char* name, int pair_nm, int pair_h, int variant) {
+  TM_REQUIRE(ldo >= lp.n / 2 + 1, "R2C: output row stride %lld below %d", (long long)ldo, lp.n / 2 + 1);
+  TM_REQUIRE(pair_h == 0 || (pair_h > 0 && pair_nm % (2 * pair_h) == 0 && pair_nm <= lp.n / 2 + 1),
+             "R2C: mirror-pair order needs whole tiles of 2 * %d positions", pair_h);
+  TM_REQUIRE(lin >= 0 && lin <= lp.n, "R2C: row of %d reals longer than %d", lin, lp.n);
+  TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);
+#define TM_R2C_CT(M_, RL_)                                                                                      \
+  return variant ? launch_r2c_rows_ct<M_, RL_, 256, 1>(lp, in, lin, out, nrows, ldo, s, name, pair_nm, pair_h) \
+                 : launch_r2c_rows_ct<M_, RL_, 256, 0>(lp, in, lin, out, nrows, ldo, s, name, pair_nm, pair_h)
+  switch (TM_X_PASS_CT ? lp.n : 0) {
+    case 128: TM_R2C_CT(64, 32);
+    case 200: TM_R2C_CT(100, 10);
+    case 256: TM_R2C_CT(128, 16);
+    default: return tm_launch_r2c_rows(lp, in, lin, out, nrows, ldo, s, name, pair_nm, pair_h);
+  }
+#undef TM_R2C_CT
 }
 
 int tm_launch_c2r_rows(const LinePlan& lp, const float2* in, float* out, int64_t nrows, float scale, hipStream_t s,

@@ -44,6 +44,11 @@ int tm_c2c_pair_h(int n);
 // paired positions.
 int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
                        hipStream_t s, const char* name, int pair_nm = 0, int pair_h = 0);
+// The same transform by the compile-time kernels (one half-length complex
+// transform per row and the split into bins) for n = 128, 200 and 256 in
+// radix plan `variant`; other lengths go to tm_launch_r2c_rows.
+int tm_launch_r2c_rows_ct(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
+                          hipStream_t s, const char* name, int pair_nm, int pair_h, int variant);
 // C2R of nrows contiguous half-spectrum rows, times scale.
 int tm_launch_c2r_rows(const LinePlan& lp, const float2* in, float* out, int64_t nrows, float scale, hipStream_t s,
                        const char* name);

@@ -1677,11 +1677,12 @@ static int scan_batch_launch(tm_scan_plan* p, const float* rots_dev, const int* 
   const int64_t q1_stride = (int64_t)B * T0 * T1 * N2s;
   const int pair_nm = p->pair_h ? N2c - p->ntl : 0;
   if (flc) {
-    rc = tm_launch_r2c_rows(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, N2s, s, "fused_x_pass", pair_nm, p->pair_h);
+    rc = tm_launch_r2c_rows_ct(lx, p->MR, T2, p->Q1, (int64_t)B * T0 * T1, N2s, s, "fused_x_pass", pair_nm,
+                               p->pair_h, p->radix);
     if (rc != TM_OK) return rc;
   }
-  rc = tm_launch_r2c_rows(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, N2s, s,
-                          "fused_x_pass", pair_nm, p->pair_h);
+  rc = tm_launch_r2c_rows_ct(lx, p->TR, T2, p->Q1 + (flc ? q1_stride : 0), (int64_t)B * T0 * T1, N2s, s,
+                             "fused_x_pass", pair_nm, p->pair_h, p->radix);
   if (rc != TM_OK) return rc;
   rc = tm_launch_c2c(ly, p->Q1, T1, p->P, (int64_t)p->nq_in * B * T0, N2c, N2s, -1, 1.0f, s, "fused_y_pass",
                      p->radix);
