@@ -75,9 +75,18 @@ k_c2c_strided(const float2* in, float2* out, LinePlan p, int lin, int64_t inner,
 // by the C2R input tm_c2r_pre of its bin k from the pair (X[k], X[M-k]),
 // whose partner sits in lane l ^ PH; twx holds the 2M twiddles of the row
 // transform.  The row pass then runs its M-point inverse directly.
+//
+// PH == 1 (radix-4 units of tm_pair_bin, L = 16): the pair partner sits in
+// lane c ^ 7 of the 8-lane unit; after the pre-processing each lane quad
+// (one first-stage group j + G r, r = 0..3, of the row's M-point inverse,
+// G = M / 4) runs that radix-4 stage across its lanes, lane m keeping the
+// output of index (m >> 1) | ((m & 1) << 1).  The row pass then starts at
+// its second stage.
 template <int PH>
 __device__ __forceinline__ float tm_lane_xor(float v) {
-  if constexpr (PH == 8)  // row_ror:8 within 16 lanes is lane ^ 8
+  if constexpr (PH == 1)  // row_half_mirror: lane c <-> 7 - c within 8 lanes
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xf, 0xf, false));
+  else if constexpr (PH == 8)  // row_ror:8 within 16 lanes is lane ^ 8
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xf, 0xf, false));
   else  // ds_swizzle bit mode: and 0x1f, or 0, xor PH
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (PH << 10)));
@@ -92,8 +101,8 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
   constexpr int BI = EI / PL::R_IN;  // first-stage butterflies per thread
   static_assert(!PRU || (FULL && PH == 0), "pruned lines are stored unpredicated");
   constexpr bool WIDE = L * sizeof(float2) >= 128;  // whole-line row segments: streaming hints
-  static_assert(PH == 0 || (L == 2 * PH && NT % L == 0 && FULL && (PH == 8 || PH == 4)),
-                "mirror pairs need whole tiles of 2 PH lines and unpredicated tiles");
+  static_assert(PH == 0 || (L == (PH == 1 ? 16 : 2 * PH) && NT % L == 0 && FULL && (PH == 8 || PH == 4 || PH == 1)),
+                "mirror pairs need whole tiles of 2 PH lines (16 for radix-4 units) and unpredicated tiles");
   extern __shared__ float2 lds[];
   __shared__ float2 tws[N];  // twiddles in LDS: no vector-memory waits inside the transforms
   __shared__ float2 twxs[PH > 0 ? 256 : 1];
@@ -162,7 +171,18 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
     const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
     const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
     float2* dst = out + o * (int64_t)N * ld + i0;
-    if constexpr (PH > 0) {
+    if constexpr (PH == 1) {
+      // every output slot of this thread is on line tid % L of tile jt:
+      // position q = L jt + lt, unit lane c, quad lane m
+      const int q = (int)i0 + tid % L, c = q & 7, m = c & 3, j = (q >> 3) + 1, G = M / 4;
+      const int k = c < 4 ? j + G * c : (G - j) + G * (c - 4);
+      const float2 w = tw_load<1>(twxs, k);
+#pragma unroll
+      for (int s2 = 0; s2 < EO; ++s2) {
+        const float2 a = v[s2];
+        v[s2] = tm_quad_idft4(tm_c2r_pre(a, make_float2(tm_lane_xor<1>(a.x), tm_lane_xor<1>(a.y)), w, false), m);
+      }
+    } else if constexpr (PH > 0) {
       // every output slot of this thread is on line tid % L of tile jt
       const int jt = (int)(i0 / L), lt = tid % L;
       const int k = lt < PH ? PH * jt + lt : M - PH * jt - (lt - PH);
@@ -255,6 +275,8 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   if constexpr (PH > 0) {
     TM_REQUIRE(full && dir > 0 && in == out && twx && 2 * M <= 256 && inner % L == 0,
                "mirror-pair C2C: in-place inverse over whole unpredicated tiles, rows of at most 129 bins");
+    TM_REQUIRE(PH != 1 || (M % 4 == 0 && (M / 4) % 2 == 1 && inner == M - 4),
+               "radix-4 units: M = 4 G with G odd and the M - 4 paired positions of a row");
     TM_TIMED(name, bytes, s,
              hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1, true, V, PH>), dim3((unsigned)blocks), dim3(NT), shm, s, in, out,
                                 lp.tw, lin, inner, (int)ld, tpo, ntiles, scale, twx, M));
@@ -635,6 +657,12 @@ int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, fl
              (long long)inner, (long long)ld);
   auto go = [&](auto vtag) -> int {
     constexpr int V = decltype(vtag)::value;
+    if (ph == 1 && lp.n == 128)
+      return launch_c2c_ct<128, 16, 256, V, 1>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    if (ph == 1 && lp.n == 256)
+      return launch_c2c_ct<256, 16, 256, V, 1>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
+    if (ph == 1 && lp.n == 512)
+      return launch_c2c_ct<512, 16, 512, V, 1>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
     if (ph == 8 && lp.n == 128)
       return launch_c2c_ct<128, 16, 256, V, 8>(lp, buf, lp.n, buf, outer, inner, ld, 1, 1.0f, s, name, twx, M);
     if (ph == 8 && lp.n == 256)
@@ -668,7 +696,7 @@ static int rows_per_block(int n, int64_t nrows) {
 int tm_launch_r2c_rows(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
                        hipStream_t s, const char* name, int pair_nm, int pair_h) {
   TM_REQUIRE(ldo >= lp.n / 2 + 1, "R2C: output row stride %lld below %d", (long long)ldo, lp.n / 2 + 1);
-  TM_REQUIRE(pair_h == 0 || (pair_h > 0 && pair_nm % (2 * pair_h) == 0 && pair_nm <= lp.n / 2 + 1),
+  TM_REQUIRE(pair_h == 0 || (pair_h > 0 && pair_nm % (pair_h == 1 ? 8 : 2 * pair_h) == 0 && pair_nm <= lp.n / 2 + 1),
              "R2C: mirror-pair order needs whole tiles of 2 * %d positions", pair_h);
   const int RL = rows_per_block(lp.n, nrows);
   const size_t shm = 2 * (size_t)RL * (lp.n + 1) * sizeof(float2);
@@ -701,7 +729,7 @@ static int launch_r2c_rows_ct(const LinePlan& lp, const float* in, int lin, floa
 int tm_launch_r2c_rows_ct(const LinePlan& lp, const float* in, int lin, float2* out, int64_t nrows, int64_t ldo,
                           hipStream_t s, const char* name, int pair_nm, int pair_h, int variant) {
   TM_REQUIRE(ldo >= lp.n / 2 + 1, "R2C: output row stride %lld below %d", (long long)ldo, lp.n / 2 + 1);
-  TM_REQUIRE(pair_h == 0 || (pair_h > 0 && pair_nm % (2 * pair_h) == 0 && pair_nm <= lp.n / 2 + 1),
+  TM_REQUIRE(pair_h == 0 || (pair_h > 0 && pair_nm % (pair_h == 1 ? 8 : 2 * pair_h) == 0 && pair_nm <= lp.n / 2 + 1),
              "R2C: mirror-pair order needs whole tiles of 2 * %d positions", pair_h);
   TM_REQUIRE(lin >= 0 && lin <= lp.n, "R2C: row of %d reals longer than %d", lin, lp.n);
   TM_REQUIRE(variant == 0 || variant == 1, "radix plan variant %d is not 0 or 1", variant);

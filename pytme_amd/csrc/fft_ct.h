@@ -287,6 +287,28 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
   }
 }
 
+// last stage (reads LDS, results in registers out[E_OUT])
+template <class P, int DIR, int LD, int TWS>
+__device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ tw, float2* out, int tid) {
+  constexpr int S = P::NST - 1;
+  constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    int l, j;
+    if (ct_slot<P, R>(tid, i, l, j)) {
+      float2* v = out + i * R;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = lds[ct_lds<P, LD>(l, j + r * NB)];
+      const int k = j % NS;
+      if (P::JUNK)
+        ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, k * TS));
+      else
+        ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
+      ct_dft<DIR, R>(v);
+    }
+  }
+}
+
 // Transform: in[E_IN] (first-stage ownership) -> out[E_OUT] (last-stage
 // ownership).  All threads of the block must call it.  `lds` holds at least
 // the L lines at stride LD.  TWS: the table tw has TWS*N entries of
@@ -321,22 +343,18 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
     hook(std::integral_constant<int, 0>{});
     lds_barrier();
     ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook);
-    constexpr int S = P::NST - 1;
-    constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
-#pragma unroll
-    for (int i = 0; i < BPT; ++i) {
-      int l, j;
-      if (ct_slot<P, R>(tid, i, l, j)) {
-        float2* v = out + i * R;
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = lds[ct_lds<P, LD>(l, j + r * NB)];
-        const int k = j % NS;
-        if (P::JUNK)
-          ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, k * TS));
-        else
-          ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
-        ct_dft<DIR, R>(v);
-      }
-    }
+    ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
   }
+}
+
+// The transform from its second stage on: `lds` already holds the outputs
+// of the first stage (element j * R0 + r of each line, as ct_fft writes
+// them), computed elsewhere.  Out as ct_fft.  All threads of the block must
+// call it, after a barrier that follows the writes of those outputs.
+template <class P, int DIR, int LD, int TWS = 1>
+__device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restrict__ tw, float2* out, int tid) {
+  static_assert(P::NST >= 2, "a transform of one stage has nothing left");
+  ct_no_hook hook;
+  ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook);
+  ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
 }

@@ -32,6 +32,8 @@ int tm_launch_c2c(const LinePlan& lp, const float2* in, int lin, float2* out, in
 // rows (tm_pair_bin with h = ph) of a half spectrum of M + 1 bins: each
 // output is stored as the C2R input tm_c2r_pre of its bin (twx: the 2M
 // twiddles of the row transform).  Lengths 128, 256, 512 (ph 8), 1024 (ph 4).
+// ph 1: radix-4 units (tm_pair_bin h = 1) for lengths 128, 256 and 512; the
+// stored values are then the first-stage outputs of the row's inverse.
 int tm_launch_c2c_pairs(const LinePlan& lp, const float2* twx, int M, int ph, float2* buf, int64_t outer,
                         int64_t inner, int64_t ld, hipStream_t s, const char* name, int variant = 0);
 // Compute units of the current device (cached per device).

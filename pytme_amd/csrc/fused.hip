@@ -608,22 +608,37 @@ constexpr int pass_c_ld(int M) { return M + 1; }
 // middle bins NM/2 .. M - NM/2, paired here.  Only the load slots that
 // meet the tail (the last one or two) carry the pairing; Z is written to
 // LDS at its bin and the transform reads it directly.
-template <int M, int RL, int NT, int NQ, int PD, bool SPLIT = false, int V = 0>
+//
+// R4 (split rows in the radix-4 units of tm_pair_bin, h = 1): the main
+// positions hold the outputs of the first (radix-4) stage of the row's
+// inverse, computed by pass B, and go to LDS where that stage would write
+// them; the tail lanes pair the bins 0, G, 2G, 3G, M (G = M / 4) and run
+// group 0's radix-4 stage across their lanes; the transform starts at its
+// second stage.
+template <int M, int RL, int NT, int NQ, int PD, bool SPLIT = false, int V = 0, bool R4 = false>
 __global__ void __launch_bounds__(NT)
 k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const float2* __restrict__ tw2,
             int64_t nrows, int ldq, int N0, int N1, int ntl, int ph, int B,
             const double* __restrict__ n_obs, const float* __restrict__ inv_den, const int* __restrict__ rot_idx,
             float scale, float eps, float* __restrict__ max_scores, int* __restrict__ max_rot) {
-  using PI = CtPlan<M, RL, NT, false, true, V>;
+  // the inverse plan: the forward plan's stages reversed, or with R4 the
+  // order whose first stage is radix 4
+  using PI = CtPlan<M, RL, NT, false, R4 ? CtPlan<M, RL, NT, false, false, V>::radix(0) != 4 : true, V>;
   constexpr int N2 = 2 * M, N2c = M + 1, LD = pass_c_ld(M);
   constexpr int EI = PI::E_IN, EO = PI::E_OUT, EW = EI > EO ? EI : EO;
   constexpr int NM = N2c / 16 * 16, NTL = N2c - NM, NP = NM / 2;  // split rows
+  constexpr int G = M / 4;  // R4: first-stage groups
+  static_assert(!R4 || (SPLIT && PI::radix(0) == 4 && M % 4 == 0 && G % 2 == 1 && NM == M - 4 && NTL == 5),
+                "radix-4 units: M = 4 G with G odd, tail bins 0, G, 2G, 3G, M");
   // load slots per thread and tile.  Split rows: P4 slots of two
   // consecutive main positions (16-byte loads, NM is even and the rows
   // 16-byte aligned), then the tail slots of one bin each; otherwise one
   // bin per slot
   constexpr int P4 = SPLIT ? (RL * NM / 2 + NT - 1) / NT : 0;
-  constexpr int NPF = SPLIT ? P4 + (RL * NTL + NT - 1) / NT : (RL * N2c + NT - 1) / NT;
+  // tail lanes per row: one per tail bin, or with R4 one lane quad (lane c
+  // pairs bins G c and M - G c, c = 0..3)
+  constexpr int TLR = R4 ? 4 : NTL;
+  constexpr int NPF = SPLIT ? P4 + (RL * TLR + NT - 1) / NT : (RL * N2c + NT - 1) / NT;
   static_assert(PD >= 1 && PD <= 3, "prefetch depth 1 to 3");
   // the tile loop is unrolled by U so that the spectrum index o and the
   // prefetch buffer of every step are compile-time constants
@@ -688,20 +703,38 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
         const int lc = l < rl ? l : rl - 1;
         const bool ok = g < RL * NM / 2 && l < rl;
         offs[p] = (int)row_of(lc) * ldq + q;
-        lpos[p] = ok ? l * LD + tm_pair_bin(q, NM, M, ph) : RL * LD;
-        lpos2[p] = ok ? l * LD + tm_pair_bin(q + 1, NM, M, ph) : RL * LD;
+        if constexpr (R4) {
+          // first-stage output 4 jj + r of group jj; quad lane m holds r =
+          // (m >> 1) | ((m & 1) << 1)
+          auto stage0 = [](int qq) {
+            const int c = qq & 7, m = c & 3, j = (qq >> 3) + 1;
+            return 4 * (c < 4 ? j : G - j) + ((m >> 1) | ((m & 1) << 1));
+          };
+          lpos[p] = ok ? l * LD + stage0(q) : RL * LD;
+          lpos2[p] = ok ? l * LD + stage0(q + 1) : RL * LD;
+        } else {
+          lpos[p] = ok ? l * LD + tm_pair_bin(q, NM, M, ph) : RL * LD;
+          lpos2[p] = ok ? l * LD + tm_pair_bin(q + 1, NM, M, ph) : RL * LD;
+        }
         offs2[p] = offs[p];
         wtw[p] = make_float2(0.f, 0.f);
       } else {
-        // bin NP + c of row (z, y) = (zg0 + l, yg) sits at Qt[y][z][c]
-        const int t = tid + (p - P4) * NT, t2 = t < RL * NTL ? t : 0;
-        const int l = t2 / NTL, c = t2 - l * NTL;
+        // tail bin c of row (z, y) = (zg0 + l, yg) sits at Qt[y][z][c]
+        const int t = tid + (p - P4) * NT, t2 = t < RL * TLR ? t : 0;
+        const int l = t2 / TLR, c = t2 - l * TLR;
         const int lc = l < rl ? l : rl - 1;
         const int base = (yg * N0 + zg0 + lc) * NTL;
         offs[p] = base + c;
         offs2[p] = base + NTL - 1 - c;
-        lpos[p] = t < RL * NTL && l < rl ? l * LD + NP + c : RL * LD;
-        wtw[p] = tw_load<1>(tw2, NP + c);
+        if constexpr (R4) {
+          // bin G c; the quad's radix-4 leaves first-stage output
+          // (c >> 1) | ((c & 1) << 1) of group 0 in lane c
+          lpos[p] = t < RL * TLR && l < rl ? l * LD + ((c >> 1) | ((c & 1) << 1)) : RL * LD;
+          wtw[p] = tw_load<1>(tw2, G * c);
+        } else {
+          lpos[p] = t < RL * NTL && l < rl ? l * LD + NP + c : RL * LD;
+          wtw[p] = tw_load<1>(tw2, NP + c);
+        }
       }
     } else {
       const int f = tid + p * NT;
@@ -751,6 +784,11 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
         if (p < P4) {
           lds[lpos[p]] = pf[p];
           lds[lpos2[p]] = pf2[p];
+        } else if constexpr (R4) {
+          // group 0 (bins 0, G, 2G, 3G): Z of the pair (G c, M - G c) in
+          // lane c of the row's quad, then the radix-4 stage across the quad
+          const int c = tid & 3;
+          lds[lpos[p]] = tm_quad_idft4(tm_c2r_pre(pf[p], pf2[p], wtw[p], c == 0), c);
         } else {
           lds[lpos[p]] = tm_c2r_pre(pf[p], pf2[p], wtw[p], false);
         }
@@ -761,19 +799,23 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     prefetch(it + PD < iters ? it + PD : iters - 1, pf, pf2);
     lds_barrier();
     float2 z[EW];
+    if constexpr (R4) {
+      ct_fft_from1<PI, 1, LD, 2>(lds, tws, z, tid);
+    } else {
 #pragma unroll
-    for (int i = 0; i < EI / PI::R_IN; ++i)
+      for (int i = 0; i < EI / PI::R_IN; ++i)
 #pragma unroll
-      for (int r = 0; r < PI::R_IN; ++r) {
-        int l, k;
-        if (!PI::in_pos(tid, i, r, l, k)) l = RL;  // idle slot: junk line (never stored)
-        if constexpr (SPLIT) {
-          z[i * PI::R_IN + r] = lds[l * LD + k];
-        } else {
-          z[i * PI::R_IN + r] = tm_c2r_pre(lds[l * LD + k], lds[l * LD + (M - k)], tw_load<1>(tws, k), k == 0);
+        for (int r = 0; r < PI::R_IN; ++r) {
+          int l, k;
+          if (!PI::in_pos(tid, i, r, l, k)) l = RL;  // idle slot: junk line (never stored)
+          if constexpr (SPLIT) {
+            z[i * PI::R_IN + r] = lds[l * LD + k];
+          } else {
+            z[i * PI::R_IN + r] = tm_c2r_pre(lds[l * LD + k], lds[l * LD + (M - k)], tw_load<1>(tws, k), k == 0);
+          }
         }
-      }
-    ct_fft<PI, 1, LD, 2>(lds, tws, z, z, tid);
+      ct_fft<PI, 1, LD, 2>(lds, tws, z, z, tid);
+    }
     const float sn = NQ == 3 ? snl[b] : 0.f;
     const int ridx = rot_idx[b];
     // Every output slot is updated (idle slots and rows past rl hold junk
@@ -968,7 +1010,7 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const f
                             const int* rot_idx, float scale, float eps, float* max_scores, int* max_rot, double bytes,
                             hipStream_t s) {
   TM_REQUIRE(!SPLIT || (ntl == M + 1 - (M + 1) / 16 * 16 && ldq >= M + 1 - ntl && Qt && nrows == (int64_t)N0 * N1 &&
-                        (ph == 8 || ph == 4)),
+                        (ph == 8 || ph == 4 || ph == 1)),
              "pass C: split rows need the tail array of the last N2c %% 16 bins, mirror pairs and nrows = N0 * N1");
   TM_REQUIRE(!SPLIT || nrows * ldq < (1LL << 31), "pass C: main offsets exceed 32 bits");
   // PD3 / PD1: prefetch depth of the FLC (3 spectra) / correlation kernels;
@@ -981,25 +1023,34 @@ static int launch_pass_c_ct(bool flc, const float2* Q, const float2* Qt, const f
   TM_REQUIRE(!SPLIT || (int64_t)N0 * N1 * ntl < (1LL << 31), "pass C: tail offsets exceed 32 bits");
   // the tile + the junk line, then scale / n of the batch's rotations (FLC)
   const size_t shm = (size_t)(RL + 1) * pass_c_ld(M) * sizeof(float2) + (flc ? sizeof(float) * (size_t)B : 0);
-  if (flc) {
-    static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
-    TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
-                                Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, n_obs, (const float*)nullptr, rot_idx, scale, eps,
-                                max_scores, max_rot));
-  } else if (B % PD1 == 0) {
-    TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
-                                Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, (const double*)nullptr, inv_den, rot_idx, scale,
-                                eps, max_scores, max_rot));
-  } else {
-    TM_TIMED("fused_pass_c", bytes, s,
-             hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1, SPLIT, V>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
-                                Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, (const double*)nullptr, inv_den, rot_idx, scale,
-                                eps, max_scores, max_rot));
+  static_assert(PD3 == 1 || PD3 == 3, "FLC prefetch depth must divide or be a multiple of 3");
+  auto go = [&](auto r4tag) -> int {
+    constexpr bool R4 = decltype(r4tag)::value;
+    if (flc) {
+      TM_TIMED("fused_pass_c", bytes, s,
+               hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 3, PD3, SPLIT, V, R4>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+                                  Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, n_obs, (const float*)nullptr, rot_idx, scale, eps,
+                                  max_scores, max_rot));
+    } else if (B % PD1 == 0) {
+      TM_TIMED("fused_pass_c", bytes, s,
+               hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, PD1, SPLIT, V, R4>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+                                  Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, (const double*)nullptr, inv_den, rot_idx, scale,
+                                  eps, max_scores, max_rot));
+    } else {
+      TM_TIMED("fused_pass_c", bytes, s,
+               hipLaunchKernelGGL((k_pass_c_ct<M, RL, NT, 1, 1, SPLIT, V, R4>), dim3((unsigned)blocks), dim3(NT), shm, s, Q,
+                                  Qt, tw2, nrows, ldq, N0, N1, ntl, ph, B, (const double*)nullptr, inv_den, rot_idx, scale,
+                                  eps, max_scores, max_rot));
+    }
+    TM_CHECK_LAUNCH();
+    return TM_OK;
+  };
+  // radix-4 units: rows of M + 1 = 4 G + 1 bins, G odd, 5 tail bins
+  if constexpr (SPLIT && M % 4 == 0 && (M / 4) % 2 == 1 && (M + 1) % 16 == 5) {
+    if (ph == 1) return go(std::true_type{});
   }
-  TM_CHECK_LAUNCH();
-  return TM_OK;
+  TM_REQUIRE(ph != 1, "pass C: radix-4 units need rows of 4 G + 1 bins with G odd and 5 tail bins");
+  return go(std::false_type{});
 }
 
 // --------------------------------------------------------------------------
@@ -1099,6 +1150,11 @@ static inline int64_t pad_bins(int64_t n) { return (n + 15) / 16 * 16; }
 // inverse (tm_c2r_pre), which needs the pair (k, M - k) of a row.
 #ifndef TM_SPLIT_ROWS
 #define TM_SPLIT_ROWS 1
+#endif
+// 1: split rows of the 200-point x axis in radix-4 units (tm_pair_bin h = 1)
+// where the y axis has a 16-line kernel; 0: mirror pairs there too.
+#ifndef TM_R4_UNITS
+#define TM_R4_UNITS 1
 #endif
 // N2m is the row stride of Q: the main bins, plus 16 unused bins where they
 // are a multiple of 64 (with a 1 KB stride the 1024-point pass B over 64-byte
@@ -1206,7 +1262,11 @@ int tm_scan_plan_create(int mode, const int64_t* fast_shape, const int* tshape, 
     row_layout(p->N[0], p->N[1], p->N[2], n2m, ntl);
     p->N2m = (int)n2m;
     p->ntl = (int)ntl;
-    p->pair_h = ntl ? tm_c2c_pair_h(p->N[1]) : 0;
+    // x length 200 (rows of 4 * 25 + 1 bins) with a y kernel of 16-line
+    // tiles: radix-4 units (pass B also runs the first stage of pass C's x
+    // inverse); otherwise mirror pairs
+    const bool r4 = TM_R4_UNITS && ntl && p->N[2] == 200 && (p->N[1] == 128 || p->N[1] == 256 || p->N[1] == 512);
+    p->pair_h = ntl ? (r4 ? 1 : tm_c2c_pair_h(p->N[1])) : 0;
     if (ntl && !p->pair_h) {
       tm_set_error("split rows without a mirror-pair kernel for y length %d", p->N[1]);
       free(p);

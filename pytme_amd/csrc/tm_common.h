@@ -62,12 +62,30 @@ __device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.
 // a pair (k, M - k); the remaining M + 1 - nm positions hold the middle bins
 // nm/2 .. M - nm/2 in order.  A transform along another axis that sees whole
 // tiles then has both bins of every pair in lanes l and l ^ h.
+//
+// h == 1 (radix-4 units; M = 4 G with G odd, nm = 4 (G - 1)): unit u of 8
+// positions holds the first-stage groups j = u + 1 and G - j of the row's
+// M-point inverse whose first stage is radix 4 (bins j + G r at positions
+// 8 u + r and (G - j) + G r at 8 u + 4 + r, r = 0..3), so positions c and
+// 7 - c of a unit hold a pair (k, M - k) and each group sits in one lane
+// quad; the remaining positions hold bins 0, G, 2G, 3G and 4G = M.
 __host__ __device__ __forceinline__ int tm_pair_bin(int q, int nm, int M, int h) {
+  if (h == 1) {
+    const int G = M / 4;
+    if (q >= nm) return G * (q - nm);
+    const int c = q & 7, j = (q >> 3) + 1;
+    return c < 4 ? j + G * c : (G - j) + G * (c - 4);
+  }
   if (q >= nm) return nm / 2 + (q - nm);
   const int j = q / (2 * h), c = q - j * 2 * h;
   return c < h ? h * j + c : M - h * j - (c - h);
 }
 __host__ __device__ __forceinline__ int tm_pair_pos(int k, int nm, int M, int h) {
+  if (h == 1) {
+    const int G = M / 4, j0 = k % G, r = k / G;
+    if (j0 == 0) return nm + r;
+    return j0 <= (G - 1) / 2 ? 8 * (j0 - 1) + r : 8 * (G - j0 - 1) + 4 + r;
+  }
   const int np = nm / 2;
   if (k < np) return (k / h) * 2 * h + k % h;
   if (M - k < np) return ((M - k) / h) * 2 * h + h + (M - k) % h;
@@ -86,6 +104,24 @@ __device__ __forceinline__ float2 tm_c2r_pre(float2 a, float2 b, float2 w, bool 
   const float2 e = make_float2(a.x + b.x, a.y - b.y);
   const float2 d = c_mul(make_float2(a.x - b.x, a.y + b.y), w);
   return make_float2(e.x - d.y, e.y + d.x);
+}
+
+// Lane m ^ X within a lane quad (DPP quad_perm).
+template <int X>
+__device__ __forceinline__ float tm_quad_xor(float v) {
+  constexpr int sel = (0 ^ X) | ((1 ^ X) << 2) | ((2 ^ X) << 4) | ((3 ^ X) << 6);
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), sel, 0xf, 0xf, false));
+}
+
+// Inverse radix-4 DFT across a lane quad: lane m holds z_m on entry and
+// y_r = sum_n z_n i^(r n) on return for r = (m >> 1) | ((m & 1) << 1) (a
+// radix-2 step over lanes m, m ^ 2, lane 3's difference times i, then one
+// over m, m ^ 1).  Every lane of the quad must take part.
+__device__ __forceinline__ float2 tm_quad_idft4(float2 z, int m) {
+  const float sa = (m & 2) ? -1.f : 1.f, sb = (m & 1) ? -1.f : 1.f;
+  float2 t = make_float2(tm_quad_xor<2>(z.x) + sa * z.x, tm_quad_xor<2>(z.y) + sa * z.y);
+  if (m == 3) t = make_float2(-t.y, t.x);
+  return make_float2(tm_quad_xor<1>(t.x) + sb * t.x, tm_quad_xor<1>(t.y) + sb * t.y);
 }
 
 // ---- streaming global accesses --------------------------------------------
