@@ -92,11 +92,17 @@ __device__ __forceinline__ float tm_lane_xor(float v) {
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (PH << 10)));
 }
 
+#ifndef TM_PB_TWIDDLE_TABLE
+#define TM_PB_TWIDDLE_TABLE 1
+#endif
 template <int N, int L, int NT, int DIR, bool FULL, int V = 0, int PH = 0, bool PRU = false>
 __global__ void __launch_bounds__(NT)
 k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, int64_t inner, int ld,
          int64_t tiles_per_outer, int64_t ntiles, float scale, const float2* __restrict__ twx, int M) {
-  using PL = CtPlan<N, L, NT, true, false, V>;
+  // radix-4 units (PH 1) of the 512-point axis: twiddles from the table,
+  // leaving VALU time to the longer epilogue (one block per CU either way;
+  // 15.34 -> 15.15 ms per 64 rotations of 512x512x200)
+  using PL = CtPlan<N, L, NT, true, false, V, PH == 1 && N == 512 && TM_PB_TWIDDLE_TABLE>;
   constexpr int EI = PL::E_IN, EO = PL::E_OUT, EW = EI > EO ? EI : EO;
   constexpr int BI = EI / PL::R_IN;  // first-stage butterflies per thread
   static_assert(!PRU || (FULL && PH == 0), "pruned lines are stored unpredicated");

@@ -105,10 +105,13 @@ constexpr bool ct_supported(int n, int variant = 0) {
   return p == n;
 }
 
-template <int N_, int L_, int NT_, bool LFAST_, bool REV_, int V_ = 0>
+// TT: every twiddle power read from the table (R - 1 table reads per
+// butterfly) instead of one read and the powers by recurrence (R - 2 complex
+// multiplies), for kernels with VALU work to spare less than LDS bandwidth.
+template <int N_, int L_, int NT_, bool LFAST_, bool REV_, int V_ = 0, bool TT_ = false>
 struct CtPlan {
   static constexpr int N = N_, L = L_, NT = NT_, V = V_;
-  static constexpr bool LFAST = LFAST_, REV = REV_;
+  static constexpr bool LFAST = LFAST_, REV = REV_, TT = TT_;
   static constexpr int NST = ct_nstages(N_, V_);
   static_assert(ct_supported(N_, V_), "length not supported by the compile-time FFT");
   static constexpr int radix(int s) { return REV ? ct_factor_v(N_, NST - 1 - s, V_) : ct_factor_v(N_, s, V_); }
@@ -254,7 +257,7 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
     // the twiddle is read with the data, before the barrier: the table is
     // never written during the transform
-    float2 v[BPT][R], w1[BPT];
+    float2 v[BPT][R], w1[BPT], wt[P::TT ? BPT : 1][R];
 #pragma unroll
     for (int i = 0; i < BPT; ++i) {
       int l, j;
@@ -262,7 +265,12 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
         const int k = j % NS;
-        if (P::JUNK || k != 0) w1[i] = ct_tw<DIR, TWS>(tw, k * TS);
+        if constexpr (P::TT) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) wt[i][r] = ct_tw<DIR, TWS>(tw, r * k * TS);
+        } else if (P::JUNK || k != 0) {
+          w1[i] = ct_tw<DIR, TWS>(tw, k * TS);
+        }
       }
     }
     lds_barrier();
@@ -271,7 +279,10 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
       int l, j;
       if (ct_slot<P, R>(tid, i, l, j)) {
         const int k = j % NS;
-        if (P::JUNK)
+        if constexpr (P::TT) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[i][r] = c_mul(v[i][r], wt[i][r]);  // k = 0: W^0 = 1, exact
+        } else if (P::JUNK)
           ct_twiddle_w<R>(v[i], w1[i]);  // k = 0 reads W^0 = 1: exact, and no branch
         else if (k != 0)
           ct_twiddle_w<R>(v[i], w1[i]);
@@ -300,10 +311,14 @@ __device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ 
 #pragma unroll
       for (int r = 0; r < R; ++r) v[r] = lds[ct_lds<P, LD>(l, j + r * NB)];
       const int k = j % NS;
-      if (P::JUNK)
+      if constexpr (P::TT) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = c_mul(v[r], ct_tw<DIR, TWS>(tw, r * k * TS));
+      } else if (P::JUNK) {
         ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, k * TS));
-      else
+      } else {
         ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
+      }
       ct_dft<DIR, R>(v);
     }
   }
