@@ -131,9 +131,10 @@ def test_specialised_sizes_parity(shape, score, fused):
     """Tomogram sizes that take the compile-time FFT kernels (128/256-point
     axes, half-length 200/128-point C2R) against the FP64 oracle; x lengths
     200 and 256 run the fused scan on split rows (96 + 5 and 128 + 1 bins)
-    in mirror-pair order where the y length has a compile-time kernel (128
-    and 256 here: pass B ends with the x-inverse pre-processing), padded
-    rows otherwise (y length 64)."""
+    where the y length has a compile-time kernel (128 and 256 here: pass B
+    ends with the x-inverse pre-processing, for x length 200 in radix-4
+    units also with the inverse's first stage; mirror pairs for x length
+    256), padded rows otherwise (y length 64)."""
     target, template, mask, rots = _workload(seed=31, shape=shape, T=16, n_rot=4, copies=4)
     if score == "FLCSphericalMask":
         from pytme_amd.synthetic import spherical_mask
@@ -156,7 +157,7 @@ def test_deferred_store_batches_vs_oracle(score, batch):
     stored after the loop): batches of 1, 2 and 3 rotations over 7 rotations
     (partial last batches; pass A's tail launch splits fewer rotations)
     against the FP64 oracle, for the S1/S2 and X kernels (FLC) and the X
-    kernel alone (FLCSphericalMask), on mirror-pair split rows."""
+    kernel alone (FLCSphericalMask), on split rows in radix-4 units."""
     target, template, mask, rots = _workload(seed=37, shape=(512, 128, 200), T=16, n_rot=7, copies=3)
     if score == "FLCSphericalMask":
         from pytme_amd.synthetic import spherical_mask
@@ -355,15 +356,17 @@ def test_c2_full_size_vs_fp64_oracle():
         _vs_oracle(target, entry, result, -1.0, workers=8)
 
 
-@pytest.mark.parametrize("shape", [(1024, 128, 256), (128, 1024, 256), (1024, 48, 256)])
+@pytest.mark.parametrize("shape", [(1024, 128, 256), (128, 1024, 256), (1024, 48, 256), (128, 1024, 200)])
 @pytest.mark.parametrize("fused", [False, True])
 def test_1024_point_axes_vs_oracle(shape, fused):
     """The 1024-point kernel instantiations of the large-tomogram scan
     against the FP64 oracle: pass A's z transform at z = 1024 (1024, 128,
     256) and pass B's y transform at y = 1024 (128, 1024, 256, mirror
-    pairs at lane distance 4), both with the split rows of the 256-point x
-    axis, and a y length without a compile-time kernel (1024, 48, 256:
-    padded rows); 32^3 template with a threshold mask, 2 rotations."""
+    pairs), both with the split rows of the 256-point x axis, a y length
+    without a compile-time kernel (1024, 48, 256: padded rows), and the
+    200-point x axis with y = 1024 (128, 1024, 200: mirror pairs there, the
+    radix-4 units being for y lengths up to 512); 32^3 template with a
+    threshold mask, 2 rotations."""
     from pytme_amd.synthetic import blob_template, embed_copies, threshold_mask
     from pytme_amd.rotations import uniform_rotations
 
@@ -529,7 +532,7 @@ def test_tile_shapes_give_identical_results(tiles):
     z tiles, pass C 5- or 10-row blocks) change how lines are spread over
     threads, not the arithmetic: the scan state is bit-identical to the
     default tiles on a 128x128x200 target (both pass A and the 200-point
-    pass C on mirror-pair split rows have both variants there)."""
+    pass C on split rows in radix-4 units have both variants there)."""
     target, template, mask, rots = _workload(seed=81, shape=(128, 128, 200), n_rot=8)
     print(f"tile shapes {tiles} vs default", flush=True)
     s0, _, r0, _ = _run("FLC", target, template, mask, rots, True, threshold=-1.0, rotation_batch=4)
