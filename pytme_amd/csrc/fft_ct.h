@@ -251,10 +251,16 @@ struct ct_no_hook {
 
 // middle stage s (reads LDS, writes LDS); hook(integral_constant<S>) runs
 // after the stage's LDS writes, before the barrier
-template <class P, int DIR, int LD, int TWS, int S, class H>
+// BC (stage 1 only): the first stage's outputs are broadcasts (each
+// butterfly had one non-zero input, so its R0 outputs are equal) and the
+// buffer holds one value per butterfly, element e of the first stage's
+// output at e / R0.
+template <class P, int DIR, int LD, int TWS, int S, class H, bool BC = false>
 __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid, H& hook) {
   if constexpr (S < P::NST - 1) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
+    static_assert(!BC || (S == 1 && NB % P::radix(0) == 0), "broadcast input only for the second stage");
+    constexpr int R0 = P::radix(0);
     // the twiddle is read with the data, before the barrier: the table is
     // never written during the transform
     float2 v[BPT][R], w1[BPT], wt[P::TT ? BPT : 1][R];
@@ -263,7 +269,8 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
       int l, j;
       if (ct_slot<P, R>(tid, i, l, j)) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
+        for (int r = 0; r < R; ++r)
+          v[i][r] = lds[ct_lds<P, LD>(l, BC ? (int)((unsigned)j / R0) + r * (NB / R0) : j + r * NB)];
         const int k = j % NS;
         if constexpr (P::TT) {
 #pragma unroll
@@ -371,5 +378,28 @@ __device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restri
   static_assert(P::NST >= 2, "a transform of one stage has nothing left");
   ct_no_hook hook;
   ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook);
+  ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
+}
+
+// Forward or inverse transform of lines whose only non-zero inputs are the
+// first input of each first-stage butterfly (zero-padded lines with at most
+// N / R0 leading values): the first stage then only broadcasts, so each
+// thread writes its butterflies' inputs x[i] (first-stage ownership, one
+// per butterfly) to the buffer once and the second stage reads them there
+// (one LDS write per butterfly instead of R0).  Out and hooks as ct_fft.
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
+__device__ __forceinline__ void ct_fft_bc0(float2* lds, const float2* __restrict__ tw, const float2* x, float2* out,
+                                           int tid, H hook = H{}) {
+  static_assert(P::NST >= 3, "the broadcast is read by a middle stage");
+  constexpr int R0 = P::radix(0), B0 = P::bpt(R0);
+  lds_barrier();  // the buffer may still be read by a previous transform
+#pragma unroll
+  for (int i = 0; i < B0; ++i) {
+    int l, j;
+    if (ct_slot<P, R0>(tid, i, l, j)) lds[ct_lds<P, LD>(l, j)] = x[i];
+  }
+  hook(std::integral_constant<int, 0>{});
+  lds_barrier();
+  ct_mid<P, DIR, LD, TWS, 1, H, true>(lds, tw, tid, hook);  // and the later middle stages
   ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
 }

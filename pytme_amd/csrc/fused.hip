@@ -40,6 +40,10 @@
 #define TM_FU_THREADS 256
 #define TM_FU_MAXP 16  // positions per thread: tile elements <= 4096 = 16 * 256
 #define TM_MOM_BLOCKS 32
+// 1: pass A's pruned forward transforms skip their broadcast first stage
+#ifndef TM_PA_BCAST
+#define TM_PA_BCAST 1
+#endif
 
 int tm_rotate_launch(const float* src, const float* srcm, const int* shape, const float* rots, int nbatch,
                      float* dst, float* dstm, int64_t ds0, int64_t ds1, int64_t dbatch, double* n_obs,
@@ -487,6 +491,25 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
     }
   };
 
+  // forward transform of the loaded columns into w.  Pruned columns: the
+  // first stage only broadcasts each loaded value, so it is skipped and the
+  // second stage reads the values themselves (ct_fft_bc0)
+  auto forward = [&](const float2* xin, float2* w, auto hook) {
+    if constexpr (PR && TM_PA_BCAST) {
+      float2 xb[NX];
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        int l, e;
+        in_elem(k, l, e);
+        xb[k] = e < T0 ? xin[k] : make_float2(0.f, 0.f);
+      }
+      ct_fft_bc0<PF, -1, L>(lds, tws, xb, w, tid, hook);
+    } else {
+      expand(xin, w);
+      ct_fft<PF, -1, L>(lds, tws, w, w, tid, hook);
+    }
+  };
+
   // one rotation; FIRST: no held output to store yet
   auto rotation = [&](int b, auto first) {
     constexpr bool FIRST = decltype(first)::value;
@@ -494,9 +517,8 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
     load_in(KIND == 2 ? 0 : q_in, bn, xn);
     if constexpr (KIND == 2) load_in(1, bn, xtn);
     float2 w[EW];
-    expand(x, w);
     if constexpr (KIND == 2) {
-      ct_fft<PF, -1, L>(lds, tws, w, w, tid);
+      forward(x, w, ct_no_hook{});
       float2 t[EO];
 #pragma unroll
       for (int s = 0; s < EO; ++s) {
@@ -505,8 +527,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       }
       inv_store(0, b, w);
       inv_store(1, b, t);
-      expand(xt, w);
-      ct_fft<PF, -1, L>(lds, tws, w, w, tid);
+      forward(xt, w, ct_no_hook{});
 #pragma unroll
       for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
       inv_store(2, b, w);
@@ -519,7 +540,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       auto inv_hook = [&](auto st) {
         if constexpr (!FIRST) store_piece(held_dst, held, std::integral_constant<int, NH + decltype(st)::value>{}, CN2{});
       };
-      ct_fft<PF, -1, L>(lds, tws, w, w, tid, fwd_hook);
+      forward(x, w, fwd_hook);
       if constexpr (KIND == 1) {
 #pragma unroll
         for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
@@ -542,7 +563,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
         held_dst = out_ptr(1, b);
       }
     } else {
-      ct_fft<PF, -1, L>(lds, tws, w, w, tid);
+      forward(x, w, ct_no_hook{});
       if constexpr (KIND == 1) {
 #pragma unroll
         for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
