@@ -338,7 +338,10 @@ __device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ 
 // LDS writes of stage s = 0 .. NST-2, before the barrier that ends it: a
 // caller issues independent work there (stores of an earlier result) so
 // that it spreads over the transform instead of coming in one burst.
-template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
+// FB: start with a barrier, for a buffer whose previous transform may still
+// be reading it; without (FB false) the caller alternates between buffers
+// so that at least one barrier lies between two transforms on one buffer.
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook, bool FB = true>
 __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ tw, float2* in, float2* out,
                                        int tid, H hook = H{}) {
   constexpr int R0 = P::radix(0);
@@ -352,7 +355,7 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
     }
     return;
   } else {
-    lds_barrier();  // the buffer may still be read by a previous transform
+    if constexpr (FB) lds_barrier();  // the buffer may still be read by a previous transform
 #pragma unroll
     for (int i = 0; i < B0; ++i) {
       int l, j;
@@ -386,13 +389,14 @@ __device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restri
 // N / R0 leading values): the first stage then only broadcasts, so each
 // thread writes its butterflies' inputs x[i] (first-stage ownership, one
 // per butterfly) to the buffer once and the second stage reads them there
-// (one LDS write per butterfly instead of R0).  Out and hooks as ct_fft.
-template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
+// (one LDS write per butterfly instead of R0).  Out, hooks and FB as
+// ct_fft.
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook, bool FB = true>
 __device__ __forceinline__ void ct_fft_bc0(float2* lds, const float2* __restrict__ tw, const float2* x, float2* out,
                                            int tid, H hook = H{}) {
   static_assert(P::NST >= 3, "the broadcast is read by a middle stage");
   constexpr int R0 = P::radix(0), B0 = P::bpt(R0);
-  lds_barrier();  // the buffer may still be read by a previous transform
+  if constexpr (FB) lds_barrier();  // the buffer may still be read by a previous transform
 #pragma unroll
   for (int i = 0; i < B0; ++i) {
     int l, j;

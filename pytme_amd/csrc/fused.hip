@@ -40,6 +40,10 @@
 #define TM_FU_THREADS 256
 #define TM_FU_MAXP 16  // positions per thread: tile elements <= 4096 = 16 * 256
 #define TM_MOM_BLOCKS 32
+// 1: pass A's transforms alternate between two LDS exchange buffers
+#ifndef TM_PA_PINGPONG
+#define TM_PA_PINGPONG 1
+#endif
 // 1: pass A's pruned forward transforms skip their broadcast first stage
 #ifndef TM_PA_BCAST
 #define TM_PA_BCAST 1
@@ -360,6 +364,15 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   __shared__ float2 tws[N0];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
   for (int i = tid; i < N0; i += NT) tws[i] = tw[i];
+  // the transforms alternate between two exchange buffers (every transform
+  // has a barrier between its first and last LDS access, so the next-but-one
+  // transform may write a buffer without waiting): no barrier before each
+  int pp = 0;
+  auto next_buf = [&]() {
+    float2* b = lds + (TM_PA_PINGPONG ? pp * (N0 * L) : 0);
+    pp ^= 1;
+    return b;
+  };
   const int bid = tm_xcd_block();
   const int ky = TAIL ? bid * (L / ntl) : bid / tiles_per_row;
   const int kx0 = kx_base + (TAIL ? 0 : (bid - ky * tiles_per_row) * L);
@@ -420,7 +433,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   };
 
   auto inv_store = [&](int o, int b, float2* w) {
-    ct_fft<PI, 1, L>(lds, tws, w, w, tid);
+    ct_fft<PI, 1, L, 1, ct_no_hook, !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid);
     if constexpr (TAIL) {
       float2* dst = Qt + ((int64_t)(o * B + b) * N1 + tky) * (int64_t)(N0 * ntl) + tc;
 #pragma unroll
@@ -503,10 +516,10 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
         in_elem(k, l, e);
         xb[k] = e < T0 ? xin[k] : make_float2(0.f, 0.f);
       }
-      ct_fft_bc0<PF, -1, L>(lds, tws, xb, w, tid, hook);
+      ct_fft_bc0<PF, -1, L, 1, decltype(hook), !TM_PA_PINGPONG>(next_buf(), tws, xb, w, tid, hook);
     } else {
       expand(xin, w);
-      ct_fft<PF, -1, L>(lds, tws, w, w, tid, hook);
+      ct_fft<PF, -1, L, 1, decltype(hook), !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid, hook);
     }
   };
 
@@ -544,7 +557,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       if constexpr (KIND == 1) {
 #pragma unroll
         for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
-        ct_fft<PI, 1, L>(lds, tws, w, w, tid, inv_hook);
+        ct_fft<PI, 1, L, 1, decltype(inv_hook), !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid, inv_hook);
 #pragma unroll
         for (int q = 0; q < NE; ++q) held[q] = w[q];
         held_dst = out_ptr(o_x, b);
@@ -555,9 +568,10 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
           t[s] = c_mul(F2[s], w[s]);
           w[s] = c_mul(F[s], w[s]);
         }
-        ct_fft<PI, 1, L>(lds, tws, w, w, tid, inv_hook);
+        ct_fft<PI, 1, L, 1, decltype(inv_hook), !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid, inv_hook);
         float2* d1 = out_ptr(0, b);
-        ct_fft<PI, 1, L>(lds, tws, t, t, tid, [&](auto st) { store_piece(d1, w, st, CN1{}); });
+        auto s1_hook = [&](auto st) { store_piece(d1, w, st, CN1{}); };
+        ct_fft<PI, 1, L, 1, decltype(s1_hook), !TM_PA_PINGPONG>(next_buf(), tws, t, t, tid, s1_hook);
 #pragma unroll
         for (int q = 0; q < NE; ++q) held[q] = t[q];
         held_dst = out_ptr(1, b);
@@ -930,7 +944,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
                                float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int nm, int ntl,
                                int tpr, int B, hipStream_t s) {
   const int64_t blocks = (int64_t)N1 * tpr;
-  const size_t shm = (size_t)N0 * L * sizeof(float2);
+  const size_t shm = (TM_PA_PINGPONG ? 2 : 1) * (size_t)N0 * L * sizeof(float2);
   // algorithmic bytes of a launch over `bins` bins of every row: the target
   // spectra it multiplies by (nin), the T0 pruned planes of nq template-side
   // inputs per rotation and its nout outputs per rotation
