@@ -407,3 +407,100 @@ __device__ __forceinline__ void ct_fft_bc0(float2* lds, const float2* __restrict
   ct_mid<P, DIR, LD, TWS, 1, H, true>(lds, tw, tid, hook);  // and the later middle stages
   ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
 }
+
+// ---- two-buffer forms ------------------------------------------------------
+// The same transforms over two LDS buffers X and Y: stage 0 writes X and
+// every later stage reads one buffer and writes the other, so a stage needs
+// no barrier between its reads and its writes, only one after its writes.
+// A transform begins by writing X without a barrier.  That is safe when the
+// caller's previous transform on these buffers read X only before its own
+// last barrier, i.e. its last stage read Y (ct2_last_in_x false: an odd
+// stage count); otherwise the caller swaps X and Y for the next transform.
+
+// middle stage S: reads src, writes dst
+template <class P, int DIR, int LD, int TWS, int S, class H, bool BC = false>
+__device__ __forceinline__ void ct_mid2(float2* src, float2* dst, const float2* __restrict__ tw, int tid, H& hook) {
+  if constexpr (S < P::NST - 1) {
+    constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
+    constexpr int R0 = P::radix(0);
+    static_assert(!BC || (S == 1 && NB % R0 == 0), "broadcast input only for the second stage");
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      int l, j;
+      if (ct_slot<P, R>(tid, i, l, j)) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          v[r] = src[ct_lds<P, LD>(l, BC ? (int)((unsigned)j / R0) + r * (NB / R0) : j + r * NB)];
+        const int k = j % NS;
+        if constexpr (P::TT) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[r] = c_mul(v[r], ct_tw<DIR, TWS>(tw, r * k * TS));
+        } else if (P::JUNK) {
+          ct_twiddle_w<R>(v, ct_tw<DIR, TWS>(tw, k * TS));
+        } else {
+          ct_twiddle<DIR, TWS, R>(v, tw, k * TS);
+        }
+        ct_dft<DIR, R>(v);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[ct_lds<P, LD>(l, base + r * NS)] = v[r];
+      }
+    }
+    hook(std::integral_constant<int, S>{});
+    lds_barrier();
+    ct_mid2<P, DIR, LD, TWS, S + 1>(dst, src, tw, tid, hook);
+  }
+}
+
+// the buffer the last stage reads: X after an odd number of stages
+// before it (stage 0 included), else Y
+template <class P>
+constexpr bool ct2_last_in_x() { return (P::NST - 1) % 2 == 1; }
+
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
+__device__ __forceinline__ void ct_fft2(float2* X, float2* Y, const float2* __restrict__ tw, float2* in, float2* out,
+                                        int tid, H hook = H{}) {
+  static_assert(P::NST >= 2, "two-buffer transforms have at least two stages");
+  constexpr int R0 = P::radix(0), B0 = P::bpt(R0);
+#pragma unroll
+  for (int i = 0; i < B0; ++i) {
+    int l, j;
+    if (ct_slot<P, R0>(tid, i, l, j)) {
+      ct_dft<DIR, R0>(in + i * R0);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) X[ct_lds<P, LD>(l, j * R0 + r)] = in[i * R0 + r];
+    }
+  }
+  hook(std::integral_constant<int, 0>{});
+  lds_barrier();
+  ct_mid2<P, DIR, LD, TWS, 1>(X, Y, tw, tid, hook);
+  ct_last<P, DIR, LD, TWS>(ct2_last_in_x<P>() ? X : Y, tw, out, tid);
+}
+
+// ct_fft_bc0 over two buffers
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
+__device__ __forceinline__ void ct_fft2_bc0(float2* X, float2* Y, const float2* __restrict__ tw, const float2* x,
+                                            float2* out, int tid, H hook = H{}) {
+  static_assert(P::NST >= 3, "the broadcast is read by a middle stage");
+  constexpr int R0 = P::radix(0), B0 = P::bpt(R0);
+#pragma unroll
+  for (int i = 0; i < B0; ++i) {
+    int l, j;
+    if (ct_slot<P, R0>(tid, i, l, j)) X[ct_lds<P, LD>(l, j)] = x[i];
+  }
+  hook(std::integral_constant<int, 0>{});
+  lds_barrier();
+  ct_mid2<P, DIR, LD, TWS, 1, H, true>(X, Y, tw, tid, hook);
+  ct_last<P, DIR, LD, TWS>(ct2_last_in_x<P>() ? X : Y, tw, out, tid);
+}
+
+// ct_fft_from1 over two buffers: X holds the first stage's outputs
+template <class P, int DIR, int LD, int TWS = 1>
+__device__ __forceinline__ void ct_fft2_from1(float2* X, float2* Y, const float2* __restrict__ tw, float2* out,
+                                              int tid) {
+  static_assert(P::NST >= 2, "a transform of one stage has nothing left");
+  ct_no_hook hook;
+  ct_mid2<P, DIR, LD, TWS, 1>(X, Y, tw, tid, hook);
+  ct_last<P, DIR, LD, TWS>(ct2_last_in_x<P>() ? X : Y, tw, out, tid);
+}

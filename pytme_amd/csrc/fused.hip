@@ -40,10 +40,6 @@
 #define TM_FU_THREADS 256
 #define TM_FU_MAXP 16  // positions per thread: tile elements <= 4096 = 16 * 256
 #define TM_MOM_BLOCKS 32
-// 1: pass A's transforms alternate between two LDS exchange buffers
-#ifndef TM_PA_PINGPONG
-#define TM_PA_PINGPONG 1
-#endif
 // 1: pass A's pruned forward transforms skip their broadcast first stage
 #ifndef TM_PA_BCAST
 #define TM_PA_BCAST 1
@@ -351,7 +347,7 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
             int ldo, int B, int tiles_per_row, int kx_base, int ntl, int q_in, int o_x) {
   using PF = CtPlan<N0, L, NT, true, false, V>;
   using PI = CtPlan<N0, L, NT, true, true, V>;
-  static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN, "ownership mismatch");
+  static_assert(PF::E_OUT == PI::E_IN && PF::R_OUT == PI::R_IN && PF::NST == PI::NST, "ownership mismatch");
   static_assert(!DEFER || (!TAIL && KIND != 2 && PF::NST >= 2 && PI::NST == PF::NST),
                 "deferred stores need staged transforms (and one or two outputs per rotation)");
   static_assert((N0 / PF::R_IN * L) % NT == 0 && (N0 / PF::R_OUT * L) % NT == 0 &&
@@ -364,14 +360,16 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   __shared__ float2 tws[N0];  // twiddles in LDS: no vector-memory waits inside the transforms
   const int tid = threadIdx.x;
   for (int i = tid; i < N0; i += NT) tws[i] = tw[i];
-  // the transforms alternate between two exchange buffers (every transform
-  // has a barrier between its first and last LDS access, so the next-but-one
-  // transform may write a buffer without waiting): no barrier before each
-  int pp = 0;
-  auto next_buf = [&]() {
-    float2* b = lds + (TM_PA_PINGPONG ? pp * (N0 * L) : 0);
-    pp ^= 1;
-    return b;
+  // the transforms run over two exchange buffers (ct_fft2: no barrier
+  // inside a stage nor before a transform); with an even stage count the
+  // last stage reads bx, so the next transform starts in the other buffer
+  float2 *bx = lds, *by = lds + N0 * L;
+  auto after_transform = [&]() {
+    if constexpr (ct2_last_in_x<PF>()) {
+      float2* t = bx;
+      bx = by;
+      by = t;
+    }
   };
   const int bid = tm_xcd_block();
   const int ky = TAIL ? bid * (L / ntl) : bid / tiles_per_row;
@@ -433,7 +431,8 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
   };
 
   auto inv_store = [&](int o, int b, float2* w) {
-    ct_fft<PI, 1, L, 1, ct_no_hook, !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid);
+    ct_fft2<PI, 1, L>(bx, by, tws, w, w, tid);
+    after_transform();
     if constexpr (TAIL) {
       float2* dst = Qt + ((int64_t)(o * B + b) * N1 + tky) * (int64_t)(N0 * ntl) + tc;
 #pragma unroll
@@ -516,10 +515,12 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
         in_elem(k, l, e);
         xb[k] = e < T0 ? xin[k] : make_float2(0.f, 0.f);
       }
-      ct_fft_bc0<PF, -1, L, 1, decltype(hook), !TM_PA_PINGPONG>(next_buf(), tws, xb, w, tid, hook);
+      ct_fft2_bc0<PF, -1, L>(bx, by, tws, xb, w, tid, hook);
+      after_transform();
     } else {
       expand(xin, w);
-      ct_fft<PF, -1, L, 1, decltype(hook), !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid, hook);
+      ct_fft2<PF, -1, L>(bx, by, tws, w, w, tid, hook);
+      after_transform();
     }
   };
 
@@ -557,7 +558,8 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
       if constexpr (KIND == 1) {
 #pragma unroll
         for (int s = 0; s < EO; ++s) w[s] = c_mul(F[s], w[s]);
-        ct_fft<PI, 1, L, 1, decltype(inv_hook), !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid, inv_hook);
+        ct_fft2<PI, 1, L>(bx, by, tws, w, w, tid, inv_hook);
+        after_transform();
 #pragma unroll
         for (int q = 0; q < NE; ++q) held[q] = w[q];
         held_dst = out_ptr(o_x, b);
@@ -568,10 +570,11 @@ k_pass_a_ct(const float2* __restrict__ P, const float2* __restrict__ Ft, const f
           t[s] = c_mul(F2[s], w[s]);
           w[s] = c_mul(F[s], w[s]);
         }
-        ct_fft<PI, 1, L, 1, decltype(inv_hook), !TM_PA_PINGPONG>(next_buf(), tws, w, w, tid, inv_hook);
+        ct_fft2<PI, 1, L>(bx, by, tws, w, w, tid, inv_hook);
+        after_transform();
         float2* d1 = out_ptr(0, b);
-        auto s1_hook = [&](auto st) { store_piece(d1, w, st, CN1{}); };
-        ct_fft<PI, 1, L, 1, decltype(s1_hook), !TM_PA_PINGPONG>(next_buf(), tws, t, t, tid, s1_hook);
+        ct_fft2<PI, 1, L>(bx, by, tws, t, t, tid, [&](auto st) { store_piece(d1, w, st, CN1{}); });
+        after_transform();
 #pragma unroll
         for (int q = 0; q < NE; ++q) held[q] = t[q];
         held_dst = out_ptr(1, b);
@@ -944,7 +947,7 @@ static int launch_pass_a_ct_pr(bool flc, const float2* P, const float2* ft, cons
                                float2* Qt, const float2* tw, int T0, int N1, int N2s, int ldo, int nm, int ntl,
                                int tpr, int B, hipStream_t s) {
   const int64_t blocks = (int64_t)N1 * tpr;
-  const size_t shm = (TM_PA_PINGPONG ? 2 : 1) * (size_t)N0 * L * sizeof(float2);
+  const size_t shm = 2 * (size_t)N0 * L * sizeof(float2);  // two exchange buffers
   // algorithmic bytes of a launch over `bins` bins of every row: the target
   // spectra it multiplies by (nin), the T0 pruned planes of nq template-side
   // inputs per rotation and its nout outputs per rotation
