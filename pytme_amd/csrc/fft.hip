@@ -92,6 +92,10 @@ __device__ __forceinline__ float tm_lane_xor(float v) {
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | (PH << 10)));
 }
 
+// 1: the 512-point compile-time C2C passes over two LDS exchange buffers
+#ifndef TM_C2C_TWO_BUFFERS
+#define TM_C2C_TWO_BUFFERS 1
+#endif
 #ifndef TM_PB_TWIDDLE_TABLE
 #define TM_PB_TWIDDLE_TABLE 1
 #endif
@@ -172,7 +176,13 @@ k_c2c_ct(const float2* in, float2* out, const float2* __restrict__ tw, int lin, 
     // carries no predicated loads, so the wait for the prefetch at the loop
     // end does not also wait for this tile's stores
     load(t + gridDim.x < ntiles ? t + gridDim.x : t, nx);
-    ct_fft<PL, DIR, L>(lds, tws, v, v, tid);
+    if constexpr (TM_C2C_TWO_BUFFERS && N == 512 && !PRU && !ct2_last_in_x<PL>()) {
+      // two exchange buffers: one barrier per stage, none before the tile
+      // (the previous tile's last stage read the second buffer)
+      ct_fft2<PL, DIR, L>(lds, lds + N * L, tws, v, v, tid);
+    } else {
+      ct_fft<PL, DIR, L>(lds, tws, v, v, tid);
+    }
     const int64_t o = t / tiles_per_outer;
     const int64_t i0 = (t - o * tiles_per_outer) * (int64_t)L;
     const int ti = (int)((inner - i0) < L ? (inner - i0) : L);
@@ -239,18 +249,21 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
                          const float2* twx = nullptr, int M = 0) {
   const int64_t tpo = tm_ceil_div(inner, L);
   const int64_t ntiles = outer * tpo;
-  const size_t shm = (size_t)N * L * sizeof(float2);
+  // LDS: one exchange buffer, two for the 512-point unpruned kernels
+  const size_t shm_pr = (size_t)N * L * sizeof(float2);
+  const size_t shm = (TM_C2C_TWO_BUFFERS && N == 512 && !ct2_last_in_x<CtPlan<N, L, NT, true, false, V>>() ? 2 : 1) *
+                     shm_pr;
   // persistent grid: exactly the blocks that are resident at once (LDS and
   // register limits), so every block runs from the start
-  auto occupancy = [&](const void* kernel) {
+  auto occupancy = [&](const void* kernel, size_t bytes) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, shm) != hipSuccess || nb < 1) nb = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, bytes) != hipSuccess || nb < 1) nb = 1;
     return nb;
   };
   static int per_cu = 0, per_cu_pruned = 0;
   if (per_cu == 0) {
-    per_cu = occupancy((const void*)k_c2c_ct<N, L, NT, 1, true, V, PH>);
-    per_cu_pruned = occupancy((const void*)k_c2c_ct<N, L, NT, -1, true, V, 0, true>);
+    per_cu = occupancy((const void*)k_c2c_ct<N, L, NT, 1, true, V, PH>, shm);
+    per_cu_pruned = occupancy((const void*)k_c2c_ct<N, L, NT, -1, true, V, 0, true>, shm_pr);
   }
   const double bytes = 8.0 * ((double)lin + (double)N) * (double)inner * (double)outer;
   using PL = CtPlan<N, L, NT, true, false, V>;
@@ -269,11 +282,11 @@ static int launch_c2c_ct(const LinePlan& lp, const float2* in, int lin, float2* 
   if (pruned) {
     if (dir < 0)
       TM_TIMED(name, bytes, s,
-               hipLaunchKernelGGL((k_c2c_ct<N, L, NT, -1, true, V, 0, true>), dim3((unsigned)blocks), dim3(NT), shm, s,
+               hipLaunchKernelGGL((k_c2c_ct<N, L, NT, -1, true, V, 0, true>), dim3((unsigned)blocks), dim3(NT), shm_pr, s,
                                   in, out, lp.tw, lin, inner, (int)ld, tpo, ntiles, scale, (const float2*)nullptr, 0));
     else
       TM_TIMED(name, bytes, s,
-               hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1, true, V, 0, true>), dim3((unsigned)blocks), dim3(NT), shm, s,
+               hipLaunchKernelGGL((k_c2c_ct<N, L, NT, 1, true, V, 0, true>), dim3((unsigned)blocks), dim3(NT), shm_pr, s,
                                   in, out, lp.tw, lin, inner, (int)ld, tpo, ntiles, scale, (const float2*)nullptr, 0));
     TM_CHECK_LAUNCH();
     return TM_OK;
