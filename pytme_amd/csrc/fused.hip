@@ -791,6 +791,11 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   }
   // pf[p]: the slot's bin (main slots of split rows: the first of two, the
   // second in pf2[p]); pf2[p] of a tail slot: the pair partner's bin
+  // tail slot p has live lanes in this wave (wave-uniform: the tail lanes
+  // are the first RL * TLR of the slot, so other waves skip its loads and
+  // its pairing)
+  const int wave0 = __builtin_amdgcn_readfirstlane(tid);
+  auto tail_live = [&](int p) { return wave0 + (p - P4) * NT < RL * TLR; };
   auto prefetch = [&](int it, float2* pf, float2* pf2) {
     const int b = it / NQ, o = it - (it / NQ) * NQ;
     const float2* src = Q + ((int64_t)(o * B + b) * nrows + r0) * ldq;  // r0 = 0 with split rows
@@ -804,7 +809,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
           const float4 v = tm_ld_stream2(reinterpret_cast<const float2*>(srcb + boffs[p]));
           pf[p] = make_float2(v.x, v.y);
           pf2[p] = make_float2(v.z, v.w);
-        } else {
+        } else if (tail_live(p)) {
           pf[p] = tm_ld_stream(tsrc + offs[p]);
           pf2[p] = tm_ld_stream(tsrc + offs2[p]);
         }
@@ -822,6 +827,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
         if (p < P4) {
           lds[lpos[p]] = pf[p];
           lds[lpos2[p]] = pf2[p];
+        } else if (!tail_live(p)) {
         } else if constexpr (R4) {
           // group 0 (bins 0, G, 2G, 3G): Z of the pair (G c, M - G c) in
           // lane c of the row's quad, then the radix-4 stage across the quad
