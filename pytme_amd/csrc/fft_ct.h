@@ -251,16 +251,10 @@ struct ct_no_hook {
 
 // middle stage s (reads LDS, writes LDS); hook(integral_constant<S>) runs
 // after the stage's LDS writes, before the barrier
-// BC (stage 1 only): the first stage's outputs are broadcasts (each
-// butterfly had one non-zero input, so its R0 outputs are equal) and the
-// buffer holds one value per butterfly, element e of the first stage's
-// output at e / R0.
-template <class P, int DIR, int LD, int TWS, int S, class H, bool BC = false>
+template <class P, int DIR, int LD, int TWS, int S, class H>
 __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid, H& hook) {
   if constexpr (S < P::NST - 1) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
-    static_assert(!BC || (S == 1 && NB % P::radix(0) == 0), "broadcast input only for the second stage");
-    constexpr int R0 = P::radix(0);
     // the twiddle is read with the data, before the barrier: the table is
     // never written during the transform
     float2 v[BPT][R], w1[BPT], wt[P::TT ? BPT : 1][R];
@@ -269,8 +263,7 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
       int l, j;
       if (ct_slot<P, R>(tid, i, l, j)) {
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          v[i][r] = lds[ct_lds<P, LD>(l, BC ? (int)((unsigned)j / R0) + r * (NB / R0) : j + r * NB)];
+        for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
         const int k = j % NS;
         if constexpr (P::TT) {
 #pragma unroll
@@ -338,10 +331,7 @@ __device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ 
 // LDS writes of stage s = 0 .. NST-2, before the barrier that ends it: a
 // caller issues independent work there (stores of an earlier result) so
 // that it spreads over the transform instead of coming in one burst.
-// FB: start with a barrier, for a buffer whose previous transform may still
-// be reading it; without (FB false) the caller alternates between buffers
-// so that at least one barrier lies between two transforms on one buffer.
-template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook, bool FB = true>
+template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
 __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ tw, float2* in, float2* out,
                                        int tid, H hook = H{}) {
   constexpr int R0 = P::radix(0);
@@ -355,7 +345,7 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
     }
     return;
   } else {
-    if constexpr (FB) lds_barrier();  // the buffer may still be read by a previous transform
+    lds_barrier();  // the buffer may still be read by a previous transform
 #pragma unroll
     for (int i = 0; i < B0; ++i) {
       int l, j;
@@ -384,30 +374,6 @@ __device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restri
   ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
 }
 
-// Forward or inverse transform of lines whose only non-zero inputs are the
-// first input of each first-stage butterfly (zero-padded lines with at most
-// N / R0 leading values): the first stage then only broadcasts, so each
-// thread writes its butterflies' inputs x[i] (first-stage ownership, one
-// per butterfly) to the buffer once and the second stage reads them there
-// (one LDS write per butterfly instead of R0).  Out, hooks and FB as
-// ct_fft.
-template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook, bool FB = true>
-__device__ __forceinline__ void ct_fft_bc0(float2* lds, const float2* __restrict__ tw, const float2* x, float2* out,
-                                           int tid, H hook = H{}) {
-  static_assert(P::NST >= 3, "the broadcast is read by a middle stage");
-  constexpr int R0 = P::radix(0), B0 = P::bpt(R0);
-  if constexpr (FB) lds_barrier();  // the buffer may still be read by a previous transform
-#pragma unroll
-  for (int i = 0; i < B0; ++i) {
-    int l, j;
-    if (ct_slot<P, R0>(tid, i, l, j)) lds[ct_lds<P, LD>(l, j)] = x[i];
-  }
-  hook(std::integral_constant<int, 0>{});
-  lds_barrier();
-  ct_mid<P, DIR, LD, TWS, 1, H, true>(lds, tw, tid, hook);  // and the later middle stages
-  ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
-}
-
 // ---- two-buffer forms ------------------------------------------------------
 // The same transforms over two LDS buffers X and Y: stage 0 writes X and
 // every later stage reads one buffer and writes the other, so a stage needs
@@ -417,7 +383,9 @@ __device__ __forceinline__ void ct_fft_bc0(float2* lds, const float2* __restrict
 // last barrier, i.e. its last stage read Y (ct2_last_in_x false: an odd
 // stage count); otherwise the caller swaps X and Y for the next transform.
 
-// middle stage S: reads src, writes dst
+// middle stage S: reads src, writes dst.  BC (S = 1 only): the first
+// stage's outputs are broadcasts, src holding one value per first-stage
+// butterfly (element e of that stage's output at e / R0).
 template <class P, int DIR, int LD, int TWS, int S, class H, bool BC = false>
 __device__ __forceinline__ void ct_mid2(float2* src, float2* dst, const float2* __restrict__ tw, int tid, H& hook) {
   if constexpr (S < P::NST - 1) {
@@ -478,7 +446,12 @@ __device__ __forceinline__ void ct_fft2(float2* X, float2* Y, const float2* __re
   ct_last<P, DIR, LD, TWS>(ct2_last_in_x<P>() ? X : Y, tw, out, tid);
 }
 
-// ct_fft_bc0 over two buffers
+// Transform of lines whose only non-zero inputs are the first input of each
+// first-stage butterfly (zero-padded lines with at most N / R0 leading
+// values): the first stage then only broadcasts, so each thread writes its
+// butterflies' inputs x[i] (first-stage ownership, one per butterfly) to X
+// once and the second stage reads them there (one LDS write per butterfly
+// instead of R0).  Out and hooks as ct_fft2.
 template <class P, int DIR, int LD, int TWS = 1, class H = ct_no_hook>
 __device__ __forceinline__ void ct_fft2_bc0(float2* X, float2* Y, const float2* __restrict__ tw, const float2* x,
                                             float2* out, int tid, H hook = H{}) {
@@ -492,15 +465,5 @@ __device__ __forceinline__ void ct_fft2_bc0(float2* X, float2* Y, const float2* 
   hook(std::integral_constant<int, 0>{});
   lds_barrier();
   ct_mid2<P, DIR, LD, TWS, 1, H, true>(X, Y, tw, tid, hook);
-  ct_last<P, DIR, LD, TWS>(ct2_last_in_x<P>() ? X : Y, tw, out, tid);
-}
-
-// ct_fft_from1 over two buffers: X holds the first stage's outputs
-template <class P, int DIR, int LD, int TWS = 1>
-__device__ __forceinline__ void ct_fft2_from1(float2* X, float2* Y, const float2* __restrict__ tw, float2* out,
-                                              int tid) {
-  static_assert(P::NST >= 2, "a transform of one stage has nothing left");
-  ct_no_hook hook;
-  ct_mid2<P, DIR, LD, TWS, 1>(X, Y, tw, tid, hook);
   ct_last<P, DIR, LD, TWS>(ct2_last_in_x<P>() ? X : Y, tw, out, tid);
 }
