@@ -298,9 +298,29 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
   }
 }
 
-// last stage (reads LDS, results in registers out[E_OUT])
+// The last stage's twiddle powers of this thread (wl[i * R + r] for slot i,
+// r = 1..R-1; r = 0 unused): they depend on the slot alone, so a kernel
+// that runs many transforms can compute them once and keep them.
+template <class P, int DIR, int TWS>
+__device__ __forceinline__ void ct_last_twiddles(const float2* __restrict__ tw, float2* wl, int tid) {
+  constexpr int S = P::NST - 1;
+  constexpr int R = P::radix(S), NS = P::ns(S), TS = P::N / (NS * R), BPT = P::bpt(R);
+#pragma unroll
+  for (int i = 0; i < BPT; ++i) {
+    int l, j;
+    ct_slot<P, R>(tid, i, l, j);
+    const int k = j % NS;
+    wl[i * R] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int r = 1; r < R; ++r) wl[i * R + r] = ct_tw<DIR, TWS>(tw, r * k * TS);
+  }
+}
+
+// last stage (reads LDS, results in registers out[E_OUT]); wl: the powers
+// of ct_last_twiddles, or null to form them here
 template <class P, int DIR, int LD, int TWS>
-__device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ tw, float2* out, int tid) {
+__device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ tw, float2* out, int tid,
+                                        const float2* wl = nullptr) {
   constexpr int S = P::NST - 1;
   constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
 #pragma unroll
@@ -311,7 +331,10 @@ __device__ __forceinline__ void ct_last(float2* lds, const float2* __restrict__ 
 #pragma unroll
       for (int r = 0; r < R; ++r) v[r] = lds[ct_lds<P, LD>(l, j + r * NB)];
       const int k = j % NS;
-      if constexpr (P::TT) {
+      if (wl) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = c_mul(v[r], wl[i * R + r]);
+      } else if constexpr (P::TT) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[r] = c_mul(v[r], ct_tw<DIR, TWS>(tw, r * k * TS));
       } else if (P::JUNK) {
@@ -367,11 +390,12 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
 // them), computed elsewhere.  Out as ct_fft.  All threads of the block must
 // call it, after a barrier that follows the writes of those outputs.
 template <class P, int DIR, int LD, int TWS = 1>
-__device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restrict__ tw, float2* out, int tid) {
+__device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restrict__ tw, float2* out, int tid,
+                                             const float2* wl = nullptr) {
   static_assert(P::NST >= 2, "a transform of one stage has nothing left");
   ct_no_hook hook;
   ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook);
-  ct_last<P, DIR, LD, TWS>(lds, tw, out, tid);
+  ct_last<P, DIR, LD, TWS>(lds, tw, out, tid, wl);
 }
 
 // ---- two-buffer forms ------------------------------------------------------

@@ -40,6 +40,11 @@
 #define TM_FU_THREADS 256
 #define TM_FU_MAXP 16  // positions per thread: tile elements <= 4096 = 16 * 256
 #define TM_MOM_BLOCKS 32
+// 1: pass C on radix-4 units keeps its last-stage twiddle powers in
+// registers for all of a block's transforms
+#ifndef TM_PC_HOLD_TWIDDLES
+#define TM_PC_HOLD_TWIDDLES 1
+#endif
 // 1: pass A's pruned forward transforms skip their broadcast first stage
 #ifndef TM_PA_BCAST
 #define TM_PA_BCAST 1
@@ -791,6 +796,11 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   }
   // pf[p]: the slot's bin (main slots of split rows: the first of two, the
   // second in pf2[p]); pf2[p] of a tail slot: the pair partner's bin
+  // R4: the last stage's twiddle powers of this thread, held for the
+  // block's 3B transforms (read from the twiddle table in global memory:
+  // the LDS copy is not yet ordered for every thread here)
+  float2 wlast[R4 && TM_PC_HOLD_TWIDDLES ? EO : 1];
+  if constexpr (R4 && TM_PC_HOLD_TWIDDLES) ct_last_twiddles<PI, 1, 2>(tw2, wlast, tid);
   // tail slot p has live lanes in this wave (wave-uniform: the tail lanes
   // are the first RL * TLR of the slot, so other waves skip its loads and
   // its pairing)
@@ -844,7 +854,7 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     lds_barrier();
     float2 z[EW];
     if constexpr (R4) {
-      ct_fft_from1<PI, 1, LD, 2>(lds, tws, z, tid);
+      ct_fft_from1<PI, 1, LD, 2>(lds, tws, z, tid, TM_PC_HOLD_TWIDDLES ? wlast : nullptr);
     } else {
 #pragma unroll
       for (int i = 0; i < EI / PI::R_IN; ++i)
