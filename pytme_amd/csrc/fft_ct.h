@@ -251,8 +251,11 @@ struct ct_no_hook {
 
 // middle stage s (reads LDS, writes LDS); hook(integral_constant<S>) runs
 // after the stage's LDS writes, before the barrier
+// wm: this stage's twiddle powers (ct_last_twiddles<P, DIR, TWS, S>), or
+// null to form them here
 template <class P, int DIR, int LD, int TWS, int S, class H>
-__device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid, H& hook) {
+__device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ tw, int tid, H& hook,
+                                       const float2* wm = nullptr) {
   if constexpr (S < P::NST - 1) {
     constexpr int R = P::radix(S), NS = P::ns(S), NB = P::N / R, TS = P::N / (NS * R), BPT = P::bpt(R);
     // the twiddle is read with the data, before the barrier: the table is
@@ -265,7 +268,8 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i][r] = lds[ct_lds<P, LD>(l, j + r * NB)];
         const int k = j % NS;
-        if constexpr (P::TT) {
+        if (wm) {
+        } else if constexpr (P::TT) {
 #pragma unroll
           for (int r = 1; r < R; ++r) wt[i][r] = ct_tw<DIR, TWS>(tw, r * k * TS);
         } else if (P::JUNK || k != 0) {
@@ -279,7 +283,10 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
       int l, j;
       if (ct_slot<P, R>(tid, i, l, j)) {
         const int k = j % NS;
-        if constexpr (P::TT) {
+        if (wm) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[i][r] = c_mul(v[i][r], wm[i * R + r]);
+        } else if constexpr (P::TT) {
 #pragma unroll
           for (int r = 1; r < R; ++r) v[i][r] = c_mul(v[i][r], wt[i][r]);  // k = 0: W^0 = 1, exact
         } else if (P::JUNK)
@@ -301,9 +308,8 @@ __device__ __forceinline__ void ct_mid(float2* lds, const float2* __restrict__ t
 // The last stage's twiddle powers of this thread (wl[i * R + r] for slot i,
 // r = 1..R-1; r = 0 unused): they depend on the slot alone, so a kernel
 // that runs many transforms can compute them once and keep them.
-template <class P, int DIR, int TWS>
+template <class P, int DIR, int TWS, int S = P::NST - 1>
 __device__ __forceinline__ void ct_last_twiddles(const float2* __restrict__ tw, float2* wl, int tid) {
-  constexpr int S = P::NST - 1;
   constexpr int R = P::radix(S), NS = P::ns(S), TS = P::N / (NS * R), BPT = P::bpt(R);
 #pragma unroll
   for (int i = 0; i < BPT; ++i) {
@@ -391,10 +397,10 @@ __device__ __forceinline__ void ct_fft(float2* lds, const float2* __restrict__ t
 // call it, after a barrier that follows the writes of those outputs.
 template <class P, int DIR, int LD, int TWS = 1>
 __device__ __forceinline__ void ct_fft_from1(float2* lds, const float2* __restrict__ tw, float2* out, int tid,
-                                             const float2* wl = nullptr) {
+                                             const float2* wl = nullptr, const float2* w1 = nullptr) {
   static_assert(P::NST >= 2, "a transform of one stage has nothing left");
   ct_no_hook hook;
-  ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook);
+  ct_mid<P, DIR, LD, TWS, 1>(lds, tw, tid, hook, w1);
   ct_last<P, DIR, LD, TWS>(lds, tw, out, tid, wl);
 }
 

@@ -40,10 +40,10 @@
 #define TM_FU_THREADS 256
 #define TM_FU_MAXP 16  // positions per thread: tile elements <= 4096 = 16 * 256
 #define TM_MOM_BLOCKS 32
-// 1: pass C on radix-4 units keeps its last-stage twiddle powers in
-// registers for all of a block's transforms
+// pass C on radix-4 units keeps the twiddle powers of its last stage (1)
+// or of both its stages (2) in registers for all of a block's transforms
 #ifndef TM_PC_HOLD_TWIDDLES
-#define TM_PC_HOLD_TWIDDLES 1
+#define TM_PC_HOLD_TWIDDLES 2
 #endif
 // 1: pass A's pruned forward transforms skip their broadcast first stage
 #ifndef TM_PA_BCAST
@@ -801,6 +801,10 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
   // the LDS copy is not yet ordered for every thread here)
   float2 wlast[R4 && TM_PC_HOLD_TWIDDLES ? EO : 1];
   if constexpr (R4 && TM_PC_HOLD_TWIDDLES) ct_last_twiddles<PI, 1, 2>(tw2, wlast, tid);
+  // and, with TM_PC_HOLD_TWIDDLES 2, those of the second stage
+  constexpr int W1N = PI::bpt(PI::radix(1)) * PI::radix(1);
+  float2 wfirst[R4 && TM_PC_HOLD_TWIDDLES == 2 ? W1N : 1];
+  if constexpr (R4 && TM_PC_HOLD_TWIDDLES == 2) ct_last_twiddles<PI, 1, 2, 1>(tw2, wfirst, tid);
   // tail slot p has live lanes in this wave (wave-uniform: the tail lanes
   // are the first RL * TLR of the slot, so other waves skip its loads and
   // its pairing)
@@ -854,7 +858,8 @@ k_pass_c_ct(const float2* __restrict__ Q, const float2* __restrict__ Qt, const f
     lds_barrier();
     float2 z[EW];
     if constexpr (R4) {
-      ct_fft_from1<PI, 1, LD, 2>(lds, tws, z, tid, TM_PC_HOLD_TWIDDLES ? wlast : nullptr);
+      ct_fft_from1<PI, 1, LD, 2>(lds, tws, z, tid, TM_PC_HOLD_TWIDDLES ? wlast : nullptr,
+                                 TM_PC_HOLD_TWIDDLES == 2 ? wfirst : nullptr);
     } else {
 #pragma unroll
       for (int i = 0; i < EI / PI::R_IN; ++i)
