@@ -232,8 +232,9 @@ int tm_refine_adam_grad(const double* grad, double* t, double* angles, const dou
  * primitive (tme/analyzer/peaks.py:662-670): scipy maximum_filter of edge
  * size (1..64) with mode="constant" (zero borders), peaks where the score
  * equals the filtered score, as int64 (z, y, x) rows in C order.  Buffers of
- * a volume of F voxels, nblk = ceil(F / 4096): tmp F floats, mask 64 nblk
- * words, cnt nblk and off nblk + 1 int64; off[nblk] is the peak count.
+ * a volume of F voxels, nblk = ceil(F / 4096): tmp F floats (used for edges
+ * above 8 only, may be NULL otherwise), mask 64 nblk words, cnt nblk and
+ * off nblk + 1 int64; off[nblk] is the peak count.
  * tm_topk_f32 replaces topk_indices (npfftw_backend.py:176-181, used by
  * PeakCaller._top_peaks and PeakCallerSort, peaks.py:501-529, 644-659):
  * indices of the k largest values, largest first, ties by lower index, NaN

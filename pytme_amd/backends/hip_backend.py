@@ -800,13 +800,14 @@ class HipBackend(MatchingBackend):
         size = max(1, int(np.max(min_distance)))
         F = s.numel()
         nblk = -(-F // 4096)
-        tmp = t.empty(F, dtype=t.float32, device=s.device)
+        # edges up to 8 filter in one pass without the xy-maxima volume
+        tmp = t.empty(F, dtype=t.float32, device=s.device) if size > 8 else None
         mask = t.empty(64 * nblk, dtype=t.int64, device=s.device)
         cnt = t.empty(nblk, dtype=t.int64, device=s.device)
         off = t.empty(nblk + 1, dtype=t.int64, device=s.device)
         shape = _lib.i64_array(s.shape)
-        _lib.call("tm_peaks_max_filter", s.data_ptr(), shape, size, tmp.data_ptr(), mask.data_ptr(), cnt.data_ptr(),
-                  off.data_ptr(), self._stream())
+        _lib.call("tm_peaks_max_filter", s.data_ptr(), shape, size, None if tmp is None else tmp.data_ptr(),
+                  mask.data_ptr(), cnt.data_ptr(), off.data_ptr(), self._stream())
         n = int(off[nblk].item())
         coords = t.empty((n, 3), dtype=t.int64, device=s.device)
         if n:

@@ -196,57 +196,115 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s
   if (threadIdx.x == 0) cnt[blockIdx.x] = (int64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// z maxima and the peak test for edges up to 8: one thread per (y, x)
-// column walks a chunk of 64 z planes (plus the halo planes) holding the
-// last D xy maxima in registers, so every xy maximum is read about once and
-// consecutive threads read consecutive voxels.  A wave's 64 columns are 64
-// consecutive voxels of a plane; its ballot is OR-ed into the linear-order
-// words (zeroed beforehand), split across a word boundary when needed.
-#define PK_ZC 64
+// The whole filter and the peak test for edges up to 8 in one pass over the
+// volume: a block owns a tile of tx columns (tx <= 64, splitting the rows
+// evenly) x 16 rows and a chunk of PK_ZC z planes, and walks the planes
+// z0 - LO .. z0 + PK_ZC - 1 + HI.  Each plane's tile with its (D - 1) halo
+// goes into LDS (two buffers, the next plane loaded into registers while
+// the current one is filtered), its x maxima into a second LDS array and
+// its y maxima into a window of the last D planes held in registers, one
+// per row of a thread: wave w owns rows w, w + 4, w + 8, w + 12, lane c
+// column x0 + c.  The window's maximum is the box maximum of plane zi - HI,
+// tested against that plane's tile centre (kept in registers since).  A
+// wave's ballot covers 64 consecutive voxels of a row and is OR-ed into the
+// linear-order words (zeroed beforehand), split across a word boundary when
+// needed.  Outside the volume the input is the constant 0.  Blocks take the
+// tiles in the order (z chunk, x tile, y tile) with the y tile fastest, so
+// neighbouring y tiles, which share halo rows, run together on one XCD.
+#ifndef PK_ZC
+#define PK_ZC 32
+#endif
 #define PK_ZCOL_MAX 8
 
 template <int D>
-__global__ __launch_bounds__(256) void k_mf_zcol(const float* __restrict__ s, const float* __restrict__ mxy,
-                                                 unsigned long long* __restrict__ mask, int n0, int64_t plane) {
+__global__ __launch_bounds__(256) void k_mf_fused(const float* __restrict__ s, unsigned long long* __restrict__ mask,
+                                                  int n0, int n1, int n2, int tx, int ntx, int nty, int64_t nb) {
   constexpr int LO = D / 2, HI = D - 1 - LO;
-  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  const bool col = q < plane;
-  const int z0 = blockIdx.y * PK_ZC, zend = min(z0 + PK_ZC, n0);
-  float w[D];
+  constexpr int HY = PK_MF_TY + D - 1, HXM = PK_MF_TX + D - 1;  // tile rows, widest tile row
+  constexpr int NPF = (HY * HXM + 255) / 256;                   // tile values per thread
+  __shared__ float tin[2][HY * HXM];
+  __shared__ float tmx[HY * PK_MF_TX];
+  const int64_t lb = tm_xcd_block();
+  if (lb >= nb) return;  // padding blocks of the XCD-grouped grid
+  const int yt = (int)(lb % nty), rest = (int)(lb / nty);
+  const int xt = rest % ntx, zc = rest / ntx;
+  const int x0 = xt * tx, y0 = yt * PK_MF_TY, z0 = zc * PK_ZC, zend = min(z0 + PK_ZC, n0);
+  const int hx = tx + D - 1, nin = HY * hx;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cl = lane < tx ? lane : tx - 1;  // lanes past the tile read a valid column and test nothing
+  const int64_t plane = (int64_t)n1 * n2;
+  // offsets in the plane of this thread's tile values (-1 outside the volume)
+  int goff[NPF];
 #pragma unroll
-  for (int k = 0; k < D; ++k) w[k] = 0.f;
-  // the xy maximum of plane zi (0 outside the volume) and the score of the
-  // output plane zi - HI (needed inside the chunk only), loaded two planes
-  // ahead so a wave keeps loads in flight while it tests the current plane
-  auto ld_m = [&](int zi) { return (col && zi >= 0 && zi < n0) ? mxy[(int64_t)zi * plane + q] : 0.f; };
-  auto ld_s = [&](int zo) { return (col && zo >= z0 && zo < zend) ? s[(int64_t)zo * plane + q] : 0.f; };
-  const int zfirst = z0 - LO;
-  float pm0 = ld_m(zfirst), pm1 = ld_m(zfirst + 1);
-  float ps0 = ld_s(zfirst - HI), ps1 = ld_s(zfirst + 1 - HI);
-  // window slots hold planes zi - D + 1 .. zi
-  for (int zi = zfirst; zi <= zend - 1 + HI; ++zi) {
-    const float cm = pm0, cs = ps0;
-    pm0 = pm1;
-    ps0 = ps1;
-    pm1 = ld_m(zi + 2);
-    ps1 = ld_s(zi + 2 - HI);
+  for (int k = 0; k < NPF; ++k) {
+    const int i = tid + k * 256, r = i / hx, c = i - r * hx;
+    const int y = y0 - LO + r, x = x0 - LO + c;
+    goff[k] = (i < nin && y >= 0 && y < n1 && x >= 0 && x < n2) ? y * n2 + x : -1;
+  }
+  float pre[NPF];
+  auto fetch = [&](int z) {
+    const bool zin = z >= 0 && z < n0;
+    const float* p = s + (int64_t)(zin ? z : 0) * plane;
 #pragma unroll
-    for (int k = 0; k < D - 1; ++k) w[k] = w[k + 1];
-    w[D - 1] = cm;
+    for (int k = 0; k < NPF; ++k) pre[k] = (zin && goff[k] >= 0) ? p[goff[k]] : 0.f;
+  };
+  bool valid[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) valid[j] = lane < tx && x0 + lane < n2 && y0 + w + 4 * j < n1;
+  float win[4][D], ctr[4][HI + 1];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) win[j][k] = 0.f;
+#pragma unroll
+    for (int k = 0; k <= HI; ++k) ctr[j][k] = 0.f;
+  }
+  const int nst = zend - z0 + D - 1;
+  fetch(z0 - LO);
+  for (int t = 0; t < nst; ++t) {
+    const int zi = z0 - LO + t;
+    float* cur = tin[t & 1];
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+      const int i = tid + k * 256;
+      if (i < nin) cur[i] = pre[k];
+    }
+    if (t + 1 < nst) fetch(zi + 1);
+    __syncthreads();  // this plane's tile is in LDS; the last plane's y maxima are read
+    for (int r = w; r < HY; r += 4) {
+      const float* row = cur + r * hx + cl;
+      float m = row[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) m = fmaxf(m, row[k]);
+      tmx[r * PK_MF_TX + lane] = m;
+    }
+    __syncthreads();  // x maxima in LDS
     const int zo = zi - HI;
-    if (zo < z0) continue;  // uniform
-    float m = w[0];
 #pragma unroll
-    for (int k = 1; k < D; ++k) m = fmaxf(m, w[k]);
-    const bool peak = col && cs == m;
-    const unsigned long long b = __ballot(peak);
-    if (lane == 0 && b) {
-      const int64_t L0 = (int64_t)zo * plane + (q - lane);
-      const int64_t wd = L0 >> 6;
-      const int sh = (int)(L0 & 63);
-      atomicOr(&mask[wd], b << sh);
-      if (sh && (b >> (64 - sh))) atomicOr(&mask[wd + 1], b >> (64 - sh));
+    for (int j = 0; j < 4; ++j) {
+      const int r = w + 4 * j;
+      float m = tmx[r * PK_MF_TX + lane];
+#pragma unroll
+      for (int k = 1; k < D; ++k) m = fmaxf(m, tmx[(r + k) * PK_MF_TX + lane]);
+#pragma unroll
+      for (int k = 0; k < D - 1; ++k) win[j][k] = win[j][k + 1];
+      win[j][D - 1] = m;
+#pragma unroll
+      for (int k = 0; k < HI; ++k) ctr[j][k] = ctr[j][k + 1];
+      ctr[j][HI] = cur[(r + LO) * hx + cl + LO];
+      if (zo < z0) continue;  // uniform: the window is not full yet
+      float mz = win[j][0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) mz = fmaxf(mz, win[j][k]);
+      const bool peak = valid[j] && ctr[j][0] == mz;
+      const unsigned long long b = __ballot(peak);
+      if (lane == 0 && b) {
+        const int64_t L0 = (int64_t)zo * plane + (int64_t)(y0 + r) * n2 + x0;
+        const int64_t wd = L0 >> 6;
+        const int sh = (int)(L0 & 63);
+        atomicOr(&mask[wd], b << sh);
+        if (sh && (b >> (64 - sh))) atomicOr(&mask[wd + 1], b >> (64 - sh));
+      }
     }
   }
 }
@@ -269,12 +327,14 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_write(const unsigned long lon
                                                          const int64_t* __restrict__ off, int64_t* __restrict__ xyz,
                                                          int n1, int n2, int64_t F) {
   // a block owns 64 words (4096 voxels), wave w the 16 words from 16 w on;
-  // every wave reads all 64 counts for its prefix, so no barrier is needed
+  // every wave loads all 64 words (one per lane) for its prefix, so no
+  // barrier is needed
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t plane = (int64_t)n1 * n2;
   const int64_t nw = (F + 63) / 64;
   const int64_t wd0 = (int64_t)blockIdx.x * 64;
-  const int c = wd0 + lane < nw ? __popcll(mask[wd0 + lane]) : 0;
+  const unsigned long long wv = wd0 + lane < nw ? mask[wd0 + lane] : 0ull;
+  const int c = __popcll(wv);
   // inclusive prefix of the 64 counts over the lanes
   int pre = c;
 #pragma unroll
@@ -287,12 +347,23 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_write(const unsigned long lon
     const int q = 16 * w + j;  // word of the block
     const int64_t wd = wd0 + q;
     if (wd >= nw) break;  // uniform over the wave
-    const unsigned long long b = mask[wd];
+    const unsigned long long b = __shfl(wv, q, 64);
     const int before = __shfl(pre, q, 64) - __shfl(c, q, 64);  // set bits in the block's words before q
     const int64_t L = wd * 64 + lane;
     if (((b >> lane) & 1ull) && L < F) {
       const int64_t o = base + before + __popcll(b & ((1ull << lane) - 1ull));
-      const int64_t z = L / plane, r = L - z * plane, y = r / n2, x = r - y * n2;
+      int64_t z, y, x;
+      if (F <= 0xffffffffll) {  // uniform: 32-bit divisions
+        const unsigned Lu = (unsigned)L, pu = (unsigned)plane, zu = Lu / pu, ru = Lu - zu * pu, yu = ru / (unsigned)n2;
+        z = zu;
+        y = yu;
+        x = ru - yu * (unsigned)n2;
+      } else {
+        z = L / plane;
+        const int64_t r = L - z * plane;
+        y = r / n2;
+        x = r - y * n2;
+      }
       xyz[3 * o + 0] = z;
       xyz[3 * o + 1] = y;
       xyz[3 * o + 2] = x;
@@ -863,14 +934,15 @@ __global__ __launch_bounds__(256) void k_fold_out(const int* __restrict__ ord, c
 extern "C" {
 
 // Peaks of a 3-D float32 volume under a box maximum filter of edge `size`
-// (constant-zero borders), first half: filters, marks and counts (separable:
-// x and y maxima in one pass, then z maxima and the peak test; for edges up
-// to 8 the z pass walks columns with the window in registers).  Buffers:
-// tmp F floats, mask 64*nblk words, cnt nblk and off nblk+1 int64 with
-// nblk = ceil(F/4096).  off[nblk] receives the number of peaks.
+// (constant-zero borders), first half: filters, marks and counts.  Edges up
+// to 8 (planes below 2^31 voxels, below 2^30 tiles): one pass with the x and y maxima in LDS
+// and the z window in registers; larger edges: x and y maxima in one pass
+// into tmp, then z maxima and the peak test.  Buffers: tmp F floats (may be
+// null for edges up to 8), mask 64*nblk words, cnt nblk and off nblk+1
+// int64 with nblk = ceil(F/4096).  off[nblk] receives the number of peaks.
 int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, float* tmp,
                         unsigned long long* mask, int64_t* cnt, int64_t* off, void* stream) {
-  TM_REQUIRE(scores && shape && tmp && mask && cnt && off, "tm_peaks_max_filter: null pointer");
+  TM_REQUIRE(scores && shape && mask && cnt && off, "tm_peaks_max_filter: null pointer");
   TM_REQUIRE(size >= 1 && size <= PK_MAX_SIZE, "tm_peaks_max_filter: size must be in [1, %d], got %d", PK_MAX_SIZE,
              size);
   const int64_t n0 = shape[0], n1 = shape[1], n2 = shape[2];
@@ -878,32 +950,29 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
   // gridDim.z (<= 65535); coordinates stay far inside int32
   TM_REQUIRE(n0 > 0 && n1 > 0 && n2 > 0 && n0 < 65536 && n1 < (1 << 20) && n2 < (1 << 20),
              "tm_peaks_max_filter: shape must be positive with z < 65536 and y, x < 2^20");
-  const int64_t F = n0 * n1 * n2, nblk = tm_ceil_div(F, PK_MF_CHUNK);
+  const int64_t F = n0 * n1 * n2, nblk = tm_ceil_div(F, PK_MF_CHUNK), plane = n1 * n2;
   hipStream_t s = (hipStream_t)stream;
   // x tiles of at most 64 columns that split the rows evenly (200 -> 4 x 50)
   const int ntx = (int)tm_ceil_div(n2, PK_MF_TX);
   const int tx = (int)tm_ceil_div(n2, ntx);
-  const int hx = tx + size - 1, hy = PK_MF_TY + size - 1;
-  const size_t shm = sizeof(float) * (size_t)hy * (hx + tx);
-  dim3 g1((unsigned)ntx, (unsigned)tm_ceil_div(n1, PK_MF_TY), (unsigned)tm_ceil_div(n0, PK_MF_TZ));
-  TM_TIMED("peaks_max_filter_xy", 8.0 * F, s,
-           hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n0, (int)n1, (int)n2, size, tx));
-  TM_CHECK_LAUNCH();
-  if (size <= PK_ZCOL_MAX) {
-    const int64_t words = 64 * nblk, plane = n1 * n2;
+  const int nty = (int)tm_ceil_div(n1, PK_MF_TY);
+  const int64_t nb = (int64_t)ntx * nty * tm_ceil_div(n0, PK_ZC);  // blocks of the one-pass filter
+  if (size <= PK_ZCOL_MAX && plane < (1ll << 31) && nb < (1ll << 30)) {
+    const int64_t words = 64 * nblk;
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<int64_t>(tm_ceil_div(words, 256), 4096)), dim3(256), 0, s,
                        mask, words);
     TM_CHECK_LAUNCH();
-    dim3 gz((unsigned)tm_ceil_div(plane, 256), (unsigned)tm_ceil_div(n0, PK_ZC));
-#define PK_ZCOL(D_)                                                                                       \
-  case D_:                                                                                                \
-    TM_TIMED("peaks_max_filter_z", 8.0 * F + F / 8.0, s,                                                  \
-             hipLaunchKernelGGL(k_mf_zcol<D_>, gz, dim3(256), 0, s, scores, tmp, mask, (int)n0, plane)); \
+    const dim3 g((unsigned)(tm_ceil_div(nb, 8) * 8));  // a multiple of 8 for the XCD grouping
+#define PK_FUSED(D_)                                                                                            \
+  case D_:                                                                                                      \
+    TM_TIMED("peaks_max_filter", 4.0 * F + F / 8.0, s,                                                          \
+             hipLaunchKernelGGL(k_mf_fused<D_>, g, dim3(256), 0, s, scores, mask, (int)n0, (int)n1, (int)n2, tx, \
+                                ntx, nty, nb));                                                                 \
     break;
     switch (size) {
-      PK_ZCOL(1) PK_ZCOL(2) PK_ZCOL(3) PK_ZCOL(4) PK_ZCOL(5) PK_ZCOL(6) PK_ZCOL(7) PK_ZCOL(8)
+      PK_FUSED(1) PK_FUSED(2) PK_FUSED(3) PK_FUSED(4) PK_FUSED(5) PK_FUSED(6) PK_FUSED(7) PK_FUSED(8)
     }
-#undef PK_ZCOL
+#undef PK_FUSED
     TM_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_mask_count, dim3((unsigned)nblk), dim3(64), 0, s, mask, cnt);
     TM_CHECK_LAUNCH();
@@ -911,6 +980,13 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
     TM_CHECK_LAUNCH();
     return TM_OK;
   }
+  TM_REQUIRE(tmp, "tm_peaks_max_filter: null tmp (needed for edges above %d)", PK_ZCOL_MAX);
+  const int hx = tx + size - 1, hy = PK_MF_TY + size - 1;
+  const size_t shm = sizeof(float) * (size_t)hy * (hx + tx);
+  dim3 g1((unsigned)ntx, (unsigned)tm_ceil_div(n1, PK_MF_TY), (unsigned)tm_ceil_div(n0, PK_MF_TZ));
+  TM_TIMED("peaks_max_filter_xy", 8.0 * F, s,
+           hipLaunchKernelGGL(k_mf_xy, g1, dim3(256), shm, s, scores, tmp, (int)n0, (int)n1, (int)n2, size, tx));
+  TM_CHECK_LAUNCH();
   TM_TIMED("peaks_max_filter_z", 8.0 * F + F / 8.0, s,
            hipLaunchKernelGGL(k_mf_z, dim3((unsigned)nblk), dim3(PK_THREADS), 0, s, scores, tmp, mask, cnt, (int)n0,
                               (int)n1, (int)n2, size));
