@@ -196,32 +196,37 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s
   if (threadIdx.x == 0) cnt[blockIdx.x] = (int64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// The whole filter and the peak test for edges up to 8 in one pass over the
-// volume: a block owns a tile of tx columns (tx <= 64, splitting the rows
-// evenly) x 16 rows and a chunk of PK_ZC z planes, and walks the planes
+// The whole filter and the peak test for edges up to 16 in one pass over
+// the volume: a block owns a tile of tx columns (tx <= 64, splitting the
+// rows evenly) x 16 rows and a chunk of PK_ZC z planes, and walks the planes
 // z0 - LO .. z0 + PK_ZC - 1 + HI.  Each plane's tile with its (D - 1) halo
 // goes into LDS (two buffers, the next plane loaded into registers while
 // the current one is filtered), its x maxima into a second LDS array and
 // its y maxima into a window of the last D planes held in registers, one
-// per row of a thread: wave w owns rows w, w + 4, w + 8, w + 12, lane c
-// column x0 + c.  The window's maximum is the box maximum of plane zi - HI,
-// tested against that plane's tile centre (kept in registers since).  A
-// wave's ballot covers 64 consecutive voxels of a row and is OR-ed into the
-// linear-order words (zeroed beforehand), split across a word boundary when
-// needed.  Outside the volume the input is the constant 0.  Blocks take the
-// tiles in the order (z chunk, x tile, y tile) with the y tile fastest, so
-// neighbouring y tiles, which share halo rows, run together on one XCD.
+// per row of a thread: of the block's NW waves, wave w owns rows w, w + NW,
+// ... (16 / NW rows), lane c column x0 + c.  The window's maximum is the box
+// maximum of plane zi - HI, tested against that plane's tile centre (kept
+// in registers since).  A wave's ballot covers 64 consecutive voxels of a
+// row and is OR-ed into the linear-order words (zeroed beforehand), split
+// across a word boundary when needed.  Outside the volume the input is the
+// constant 0.  Blocks take the tiles in the order (z chunk, x tile, y tile)
+// with the y tile fastest, so neighbouring y tiles, which share halo rows,
+// run together on one XCD.  Edges up to 8 run 4 waves per block, larger
+// ones 8 (their windows would not fit 4 rows per thread in registers).
 #ifndef PK_ZC
 #define PK_ZC 32
 #endif
-#define PK_ZCOL_MAX 8
+#define PK_ZCOL_MAX 16
 
-template <int D>
-__global__ __launch_bounds__(256) void k_mf_fused(const float* __restrict__ s, unsigned long long* __restrict__ mask,
-                                                  int n0, int n1, int n2, int tx, int ntx, int nty, int64_t nb) {
+template <int D, int NW>
+__global__ __launch_bounds__(64 * NW) void k_mf_fused(const float* __restrict__ s,
+                                                      unsigned long long* __restrict__ mask, int n0, int n1, int n2,
+                                                      int tx, int ntx, int nty, int64_t nb) {
+  constexpr int NT = 64 * NW, RPT = PK_MF_TY / NW;  // threads, rows per thread
+  static_assert(PK_MF_TY % NW == 0, "whole rows per wave");
   constexpr int LO = D / 2, HI = D - 1 - LO;
   constexpr int HY = PK_MF_TY + D - 1, HXM = PK_MF_TX + D - 1;  // tile rows, widest tile row
-  constexpr int NPF = (HY * HXM + 255) / 256;                   // tile values per thread
+  constexpr int NPF = (HY * HXM + NT - 1) / NT;                 // tile values per thread
   __shared__ float tin[2][HY * HXM];
   __shared__ float tmx[HY * PK_MF_TX];
   const int64_t lb = tm_xcd_block();
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(256) void k_mf_fused(const float* __restrict__ s, u
   int goff[NPF];
 #pragma unroll
   for (int k = 0; k < NPF; ++k) {
-    const int i = tid + k * 256, r = i / hx, c = i - r * hx;
+    const int i = tid + k * NT, r = i / hx, c = i - r * hx;
     const int y = y0 - LO + r, x = x0 - LO + c;
     goff[k] = (i < nin && y >= 0 && y < n1 && x >= 0 && x < n2) ? y * n2 + x : -1;
   }
@@ -248,12 +253,12 @@ __global__ __launch_bounds__(256) void k_mf_fused(const float* __restrict__ s, u
 #pragma unroll
     for (int k = 0; k < NPF; ++k) pre[k] = (zin && goff[k] >= 0) ? p[goff[k]] : 0.f;
   };
-  bool valid[4];
+  bool valid[RPT];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) valid[j] = lane < tx && x0 + lane < n2 && y0 + w + 4 * j < n1;
-  float win[4][D], ctr[4][HI + 1];
+  for (int j = 0; j < RPT; ++j) valid[j] = lane < tx && x0 + lane < n2 && y0 + w + NW * j < n1;
+  float win[RPT][D], ctr[RPT][HI + 1];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < RPT; ++j) {
 #pragma unroll
     for (int k = 0; k < D; ++k) win[j][k] = 0.f;
 #pragma unroll
@@ -266,12 +271,12 @@ __global__ __launch_bounds__(256) void k_mf_fused(const float* __restrict__ s, u
     float* cur = tin[t & 1];
 #pragma unroll
     for (int k = 0; k < NPF; ++k) {
-      const int i = tid + k * 256;
+      const int i = tid + k * NT;
       if (i < nin) cur[i] = pre[k];
     }
     if (t + 1 < nst) fetch(zi + 1);
     __syncthreads();  // this plane's tile is in LDS; the last plane's y maxima are read
-    for (int r = w; r < HY; r += 4) {
+    for (int r = w; r < HY; r += NW) {
       const float* row = cur + r * hx + cl;
       float m = row[0];
 #pragma unroll
@@ -281,8 +286,8 @@ __global__ __launch_bounds__(256) void k_mf_fused(const float* __restrict__ s, u
     __syncthreads();  // x maxima in LDS
     const int zo = zi - HI;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = w + 4 * j;
+    for (int j = 0; j < RPT; ++j) {
+      const int r = w + NW * j;
       float m = tmx[r * PK_MF_TX + lane];
 #pragma unroll
       for (int k = 1; k < D; ++k) m = fmaxf(m, tmx[(r + k) * PK_MF_TX + lane]);
@@ -935,10 +940,10 @@ extern "C" {
 
 // Peaks of a 3-D float32 volume under a box maximum filter of edge `size`
 // (constant-zero borders), first half: filters, marks and counts.  Edges up
-// to 8 (planes below 2^31 voxels, below 2^30 tiles): one pass with the x and y maxima in LDS
-// and the z window in registers; larger edges: x and y maxima in one pass
-// into tmp, then z maxima and the peak test.  Buffers: tmp F floats (may be
-// null for edges up to 8), mask 64*nblk words, cnt nblk and off nblk+1
+// to 16 (planes below 2^31 voxels, below 2^30 tiles): one pass with the x
+// and y maxima in LDS and the z window in registers; larger edges: x and y
+// maxima in one pass into tmp, then z maxima and the peak test.  Buffers: tmp F floats (may be
+// null for edges up to 16), mask 64*nblk words, cnt nblk and off nblk+1
 // int64 with nblk = ceil(F/4096).  off[nblk] receives the number of peaks.
 int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, float* tmp,
                         unsigned long long* mask, int64_t* cnt, int64_t* off, void* stream) {
@@ -963,14 +968,15 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
                        mask, words);
     TM_CHECK_LAUNCH();
     const dim3 g((unsigned)(tm_ceil_div(nb, 8) * 8));  // a multiple of 8 for the XCD grouping
-#define PK_FUSED(D_)                                                                                            \
-  case D_:                                                                                                      \
-    TM_TIMED("peaks_max_filter", 4.0 * F + F / 8.0, s,                                                          \
-             hipLaunchKernelGGL(k_mf_fused<D_>, g, dim3(256), 0, s, scores, mask, (int)n0, (int)n1, (int)n2, tx, \
-                                ntx, nty, nb));                                                                 \
+#define PK_FUSED(D_)                                                                                        \
+  case D_:                                                                                                  \
+    TM_TIMED("peaks_max_filter", 4.0 * F + F / 8.0, s,                                                      \
+             hipLaunchKernelGGL((k_mf_fused<D_, (D_ <= 8 ? 4 : 8)>), g, dim3(D_ <= 8 ? 256 : 512), 0, s, scores, \
+                                mask, (int)n0, (int)n1, (int)n2, tx, ntx, nty, nb));                        \
     break;
     switch (size) {
       PK_FUSED(1) PK_FUSED(2) PK_FUSED(3) PK_FUSED(4) PK_FUSED(5) PK_FUSED(6) PK_FUSED(7) PK_FUSED(8)
+      PK_FUSED(9) PK_FUSED(10) PK_FUSED(11) PK_FUSED(12) PK_FUSED(13) PK_FUSED(14) PK_FUSED(15) PK_FUSED(16)
     }
 #undef PK_FUSED
     TM_CHECK_LAUNCH();
