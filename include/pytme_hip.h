@@ -233,7 +233,7 @@ int tm_refine_adam_grad(const double* grad, double* t, double* angles, const dou
  * size (1..64) with mode="constant" (zero borders), peaks where the score
  * equals the filtered score, as int64 (z, y, x) rows in C order.  Buffers of
  * a volume of F voxels, nblk = ceil(F / 4096): tmp F floats (used for edges
- * above 16 only, may be NULL otherwise), mask 64 nblk words, cnt nblk and
+ * above 32 only, may be NULL otherwise), mask 64 nblk words, cnt nblk and
  * off nblk + 1 int64; off[nblk] is the peak count.
  * tm_topk_f32 replaces topk_indices (npfftw_backend.py:176-181, used by
  * PeakCaller._top_peaks and PeakCallerSort, peaks.py:501-529, 644-659):

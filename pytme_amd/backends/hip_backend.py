@@ -800,8 +800,8 @@ class HipBackend(MatchingBackend):
         size = max(1, int(np.max(min_distance)))
         F = s.numel()
         nblk = -(-F // 4096)
-        # edges up to 16 filter in one pass without the xy-maxima volume
-        tmp = t.empty(F, dtype=t.float32, device=s.device) if size > 16 else None
+        # edges up to 32 filter in one pass without the xy-maxima volume
+        tmp = t.empty(F, dtype=t.float32, device=s.device) if size > 32 else None
         mask = t.empty(64 * nblk, dtype=t.int64, device=s.device)
         cnt = t.empty(nblk, dtype=t.int64, device=s.device)
         off = t.empty(nblk + 1, dtype=t.int64, device=s.device)

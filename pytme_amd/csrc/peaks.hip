@@ -196,7 +196,7 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s
   if (threadIdx.x == 0) cnt[blockIdx.x] = (int64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// The whole filter and the peak test for edges up to 16 in one pass over
+// The whole filter and the peak test for edges up to 32 in one pass over
 // the volume: a block owns a tile of tx columns (tx <= 64, splitting the
 // rows evenly) x 16 rows and a chunk of PK_ZC z planes, and walks the planes
 // z0 - LO .. z0 + PK_ZC - 1 + HI.  Each plane's tile with its (D - 1) halo
@@ -211,12 +211,13 @@ __global__ __launch_bounds__(PK_THREADS) void k_mf_z(const float* __restrict__ s
 // across a word boundary when needed.  Outside the volume the input is the
 // constant 0.  Blocks take the tiles in the order (z chunk, x tile, y tile)
 // with the y tile fastest, so neighbouring y tiles, which share halo rows,
-// run together on one XCD.  Edges up to 8 run 4 waves per block, larger
-// ones 8 (their windows would not fit 4 rows per thread in registers).
+// run together on one XCD.  Edges up to 8 run 4 waves per block, up to 16
+// 8 waves and larger ones 16 (the windows of 4 or 2 rows per thread would
+// not fit in registers).
 #ifndef PK_ZC
 #define PK_ZC 32
 #endif
-#define PK_ZCOL_MAX 16
+#define PK_ZCOL_MAX 32
 
 template <int D, int NW>
 __global__ __launch_bounds__(64 * NW) void k_mf_fused(const float* __restrict__ s,
@@ -940,10 +941,10 @@ extern "C" {
 
 // Peaks of a 3-D float32 volume under a box maximum filter of edge `size`
 // (constant-zero borders), first half: filters, marks and counts.  Edges up
-// to 16 (planes below 2^31 voxels, below 2^30 tiles): one pass with the x
+// to 32 (planes below 2^31 voxels, below 2^30 tiles): one pass with the x
 // and y maxima in LDS and the z window in registers; larger edges: x and y
 // maxima in one pass into tmp, then z maxima and the peak test.  Buffers: tmp F floats (may be
-// null for edges up to 16), mask 64*nblk words, cnt nblk and off nblk+1
+// null for edges up to 32), mask 64*nblk words, cnt nblk and off nblk+1
 // int64 with nblk = ceil(F/4096).  off[nblk] receives the number of peaks.
 int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, float* tmp,
                         unsigned long long* mask, int64_t* cnt, int64_t* off, void* stream) {
@@ -968,15 +969,18 @@ int tm_peaks_max_filter(const float* scores, const int64_t* shape, int size, flo
                        mask, words);
     TM_CHECK_LAUNCH();
     const dim3 g((unsigned)(tm_ceil_div(nb, 8) * 8));  // a multiple of 8 for the XCD grouping
-#define PK_FUSED(D_)                                                                                        \
-  case D_:                                                                                                  \
-    TM_TIMED("peaks_max_filter", 4.0 * F + F / 8.0, s,                                                      \
-             hipLaunchKernelGGL((k_mf_fused<D_, (D_ <= 8 ? 4 : 8)>), g, dim3(D_ <= 8 ? 256 : 512), 0, s, scores, \
-                                mask, (int)n0, (int)n1, (int)n2, tx, ntx, nty, nb));                        \
+#define PK_FUSED(D_)                                                                                       \
+  case D_:                                                                                                 \
+    TM_TIMED("peaks_max_filter", 4.0 * F + F / 8.0, s,                                                     \
+             hipLaunchKernelGGL((k_mf_fused<D_, (D_ <= 8 ? 4 : D_ <= 16 ? 8 : 16)>), g,                    \
+                                dim3(D_ <= 8 ? 256 : D_ <= 16 ? 512 : 1024), 0, s, scores, mask, (int)n0,  \
+                                (int)n1, (int)n2, tx, ntx, nty, nb));                                      \
     break;
     switch (size) {
       PK_FUSED(1) PK_FUSED(2) PK_FUSED(3) PK_FUSED(4) PK_FUSED(5) PK_FUSED(6) PK_FUSED(7) PK_FUSED(8)
       PK_FUSED(9) PK_FUSED(10) PK_FUSED(11) PK_FUSED(12) PK_FUSED(13) PK_FUSED(14) PK_FUSED(15) PK_FUSED(16)
+      PK_FUSED(17) PK_FUSED(18) PK_FUSED(19) PK_FUSED(20) PK_FUSED(21) PK_FUSED(22) PK_FUSED(23) PK_FUSED(24)
+      PK_FUSED(25) PK_FUSED(26) PK_FUSED(27) PK_FUSED(28) PK_FUSED(29) PK_FUSED(30) PK_FUSED(31) PK_FUSED(32)
     }
 #undef PK_FUSED
     TM_CHECK_LAUNCH();

@@ -30,7 +30,7 @@ def _volume(rng, shape, kind):
 @pytest.mark.parametrize(
     "shape", [(37, 41, 53), (64, 64, 64), (5, 300, 7), (1, 1, 1), (3, 1, 200), (2, 70, 130), (70, 33, 200)]
 )
-@pytest.mark.parametrize("size", [1, 2, 3, 5, 8, 9, 12, 16, 17, 64])
+@pytest.mark.parametrize("size", [1, 2, 3, 5, 8, 9, 12, 16, 17, 24, 32, 33, 64])
 @pytest.mark.parametrize("kind", ["normal", "plateau"])
 def test_max_filter_coordinates(be, shape, size, kind):
     import torch
