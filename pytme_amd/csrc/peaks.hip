@@ -27,7 +27,7 @@
 #define PK_MF_TX 64                    // max-filter plane tile: up to 64 x 16
 #define PK_MF_TY 16
 #define PK_MF_TZ 8                     // z planes per block of the xy pass
-#define PK_XY_PF 7                     // prefetched tile values per thread (tiles of edges <= 8)
+#define PK_XY_PF 7                     // prefetched tile values per thread (the xy pass's small tiles)
 #define PK_MAX_SIZE 64                 // largest filter edge
 #define PK_SORT_LDS 8192               // bitonic sort in LDS up to this many keys
 #define PK_SORT_MAX 65536              // larger sorts run on a global buffer
@@ -102,8 +102,8 @@ __global__ __launch_bounds__(256) void k_mf_xy(const float* __restrict__ s, floa
   float* mx = sm + hy * hx;   // hy x tx
   const int x0 = blockIdx.x * tx, y0 = blockIdx.y * PK_MF_TY;
   const int zb = blockIdx.z * PK_MF_TZ, ze = min(zb + PK_MF_TZ, n0);
-  // edges up to 8: the next plane's tile is loaded into registers while the
-  // current one is filtered (at most PK_XY_PF values per thread)
+  // tiles of at most PK_XY_PF values per thread (narrow volumes): the next
+  // plane's tile is loaded into registers while the current one is filtered
   const bool pf = hy * hx <= PK_XY_PF * 256;
   float pre[PK_XY_PF];
   auto fetch = [&](int64_t z) {
