@@ -35,6 +35,7 @@
 #define PK_TOPK_ITEMS 16               // elements per thread in the top-k scans
 #define PK_MF_ITEMS 16                 // voxels per thread in the peak test and write
 #define PK_MF_CHUNK (PK_THREADS * PK_MF_ITEMS)
+#define PK_WRITE_BLOCKS 2048           // blocks of the coordinate write (8 per CU)
 #define PK_CF_MAX 8                    // earlier conflicts listed per candidate
 #define PK_PAR_MAX 16384               // candidates the parallel distance filter takes
 #define PK_TOPK_CHUNK (PK_THREADS * PK_TOPK_ITEMS)
@@ -328,52 +329,66 @@ __global__ __launch_bounds__(64) void k_mask_count(const unsigned long long* __r
   if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
 
-// C-ordered coordinates of the set bits
+// C-ordered coordinates of the set bits.  Blocks stride over the 64-word
+// groups (4096 voxels; group g's first output is off[g]), loading the next
+// group's words and offset before writing the current group's
+// coordinates; a wave visits only its words that hold set bits.
 __global__ __launch_bounds__(PK_THREADS) void k_mf_write(const unsigned long long* __restrict__ mask,
                                                          const int64_t* __restrict__ off, int64_t* __restrict__ xyz,
                                                          int n1, int n2, int64_t F) {
-  // a block owns 64 words (4096 voxels), wave w the 16 words from 16 w on;
-  // every wave loads all 64 words (one per lane) for its prefix, so no
-  // barrier is needed
+  // wave w writes the 16 words from 16 w on of a group; every wave loads
+  // all 64 words (one per lane) for its prefix, so no barrier is needed
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t plane = (int64_t)n1 * n2;
-  const int64_t nw = (F + 63) / 64;
-  const int64_t wd0 = (int64_t)blockIdx.x * 64;
-  const unsigned long long wv = wd0 + lane < nw ? mask[wd0 + lane] : 0ull;
-  const int c = __popcll(wv);
-  // inclusive prefix of the 64 counts over the lanes
-  int pre = c;
+  const int64_t nw = (F + 63) / 64, ng = (nw + 63) / 64;
+  auto word = [&](int64_t g) { return g < ng && g * 64 + lane < nw ? mask[g * 64 + lane] : 0ull; };
+  int64_t g = blockIdx.x;
+  unsigned long long wv = word(g);
+  int64_t base = g < ng ? off[g] : 0;
+  for (; g < ng; g += gridDim.x) {
+    const int64_t gn = g + gridDim.x;
+    const unsigned long long wvn = word(gn);
+    const int64_t basen = gn < ng ? off[gn] : 0;
+    const int64_t wd0 = g * 64;
+    const int c = __popcll(wv);
+    // inclusive prefix of the 64 counts over the lanes
+    int pre = c;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(pre, o, 64);
-    if (lane >= o) pre += v;
-  }
-  const int64_t base = off[blockIdx.x];
-  for (int j = 0; j < 16; ++j) {
-    const int q = 16 * w + j;  // word of the block
-    const int64_t wd = wd0 + q;
-    if (wd >= nw) break;  // uniform over the wave
-    const unsigned long long b = __shfl(wv, q, 64);
-    const int before = __shfl(pre, q, 64) - __shfl(c, q, 64);  // set bits in the block's words before q
-    const int64_t L = wd * 64 + lane;
-    if (((b >> lane) & 1ull) && L < F) {
-      const int64_t o = base + before + __popcll(b & ((1ull << lane) - 1ull));
-      int64_t z, y, x;
-      if (F <= 0xffffffffll) {  // uniform: 32-bit divisions
-        const unsigned Lu = (unsigned)L, pu = (unsigned)plane, zu = Lu / pu, ru = Lu - zu * pu, yu = ru / (unsigned)n2;
-        z = zu;
-        y = yu;
-        x = ru - yu * (unsigned)n2;
-      } else {
-        z = L / plane;
-        const int64_t r = L - z * plane;
-        y = r / n2;
-        x = r - y * n2;
-      }
-      xyz[3 * o + 0] = z;
-      xyz[3 * o + 1] = y;
-      xyz[3 * o + 2] = x;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(pre, o, 64);
+      if (lane >= o) pre += v;
     }
+    // this wave's words with set bits (words past the volume are zero)
+    unsigned long long todo = (__ballot(wv != 0ull) >> (16 * w)) & 0xffffull;
+    while (todo) {  // uniform over the wave
+      const int q = 16 * w + __ffsll((long long)todo) - 1;  // word of the group
+      todo &= todo - 1ull;
+      const int64_t wd = wd0 + q;
+      const unsigned long long b = __shfl(wv, q, 64);
+      const int before = __shfl(pre, q, 64) - __shfl(c, q, 64);  // set bits in the group's words before q
+      const int64_t L = wd * 64 + lane;
+      if (((b >> lane) & 1ull) && L < F) {
+        const int64_t o = base + before + __popcll(b & ((1ull << lane) - 1ull));
+        int64_t z, y, x;
+        if (F <= 0xffffffffll) {  // uniform: 32-bit divisions
+          const unsigned Lu = (unsigned)L, pu = (unsigned)plane, zu = Lu / pu, ru = Lu - zu * pu,
+                         yu = ru / (unsigned)n2;
+          z = zu;
+          y = yu;
+          x = ru - yu * (unsigned)n2;
+        } else {
+          z = L / plane;
+          const int64_t r = L - z * plane;
+          y = r / n2;
+          x = r - y * n2;
+        }
+        xyz[3 * o + 0] = z;
+        xyz[3 * o + 1] = y;
+        xyz[3 * o + 2] = x;
+      }
+    }
+    wv = wvn;
+    base = basen;
   }
 }
 
@@ -1014,7 +1029,8 @@ int tm_peaks_write(const unsigned long long* mask, const int64_t* off, const int
   const int64_t F = shape[0] * shape[1] * shape[2], nblk = tm_ceil_div(F, PK_MF_CHUNK);
   hipStream_t s = (hipStream_t)stream;
   TM_TIMED("peaks_write", F / 8.0, s,
-           hipLaunchKernelGGL(k_mf_write, dim3((unsigned)nblk), dim3(PK_THREADS), 0, s, mask, off, coords,
+           hipLaunchKernelGGL(k_mf_write, dim3((unsigned)std::min<int64_t>(nblk, PK_WRITE_BLOCKS)),
+                              dim3(PK_THREADS), 0, s, mask, off, coords,
                               (int)shape[1], (int)shape[2], F));
   TM_CHECK_LAUNCH();
   return TM_OK;
